@@ -1,0 +1,184 @@
+/*
+ * xlb_biot.h -- C ABI of the MI355X (gfx950) lattice-Boltzmann elasticity / Darcy /
+ * Biot kernels.
+ *
+ * Every entry point replaces one jitted method of the reference solvers
+ * (s-lunowa/xlb-biot); the replaced method is cited in the comment above it.
+ * The Python solver classes in xlb-biot_amd/src bind these symbols with ctypes.
+ *
+ * Conventions
+ *  - Plain device pointers; no allocation inside any call.  Buffers that a call
+ *    needs beyond its inputs and outputs (norm partials) are passed in, sized by
+ *    the matching *_count() query.
+ *  - Populations are structure-of-arrays: f[d][x][y], y fastest, shape (q, nx, ny).
+ *    Vector fields are (2, nx, ny), stresses (3, nx, ny), scalars (nx, ny).
+ *    The reference's (nx, ny, c) arrays are the permuted views of these.
+ *  - dtype: XB_F64 or XB_F32; all operands of one call share it (the reference's
+ *    "f64/f64" and "f32/f32" policies).  Norms are always accumulated in FP64.
+ *  - Every call takes the HIP stream to launch on (hipStream_t as void*) and
+ *    returns an int status: XB_OK, or a negative XB_E* code; the message of the
+ *    last failure on the calling thread is returned by xb_last_error().
+ *  - The domain is periodic (the reference streams with jnp.roll); walls are
+ *    the per-cell rule tables (xb_bc_table) compiled on the host from the
+ *    boundary-condition objects.
+ */
+#ifndef XLB_BIOT_H
+#define XLB_BIOT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XB_ABI_VERSION 1
+
+enum { XB_F64 = 0, XB_F32 = 1 };
+
+enum {
+  XB_OK = 0,
+  XB_EINVAL = -1,       /* bad argument (null pointer, shape, box, code) */
+  XB_EUNSUPPORTED = -2, /* dtype or mode not supported */
+  XB_EHIP = -3,         /* HIP runtime error on launch */
+  XB_ENONFINITE = -4    /* non-finite value found by a checked reduction */
+};
+
+/* Sweep modes of xb_elast_step. */
+enum {
+  XB_MODE_STEP = 0,     /* f_out = S(f_in)                                     */
+  XB_MODE_SMOOTH = 1,   /* f_out = (1-gamma) f_in + gamma S(f_in - r)          */
+  XB_MODE_RESIDUAL = 2  /* coarse_out = restrict(f_in - S(f_in - r)) on the
+                           interior box, norm_partials = per-wave sums of squares */
+};
+
+/* Rule codes of xb_bc_table (per slot and direction d):
+ *   0           keep the default value (collision output / streamed value)
+ *   k in 1..q   +src[k-1]      (src = pre-collision populations of the cell for
+ *   -k          -src[k-1]       the post-collision stage, post-collision
+ *                               populations of the cell for the post-streaming stage)
+ *   XB_RULE_ZERO  0
+ * and the constant const_*[slot*q + d] is added afterwards. */
+#define XB_RULE_ZERO 100
+
+typedef struct xb_bc_table {
+  int32_t nslots;           /* boundary cells carrying a rule                       */
+  const int32_t *cell_slot; /* (nx*ny) slot of each cell or -1                      */
+  const int8_t *code_pc;    /* (nslots*q) post-collision rules (may be null)        */
+  const int8_t *code_ps;    /* (nslots*q) post-streaming rules (may be null)        */
+  const void *const_pc;     /* (nslots*q) dtype                                     */
+  const void *const_ps;     /* (nslots*q) dtype                                     */
+} xb_bc_table;
+
+typedef struct xb_elast_args {
+  int32_t nx, ny;           /* grid of this level                                   */
+  int32_t dtype;
+  int32_t mode;             /* XB_MODE_*                                            */
+  double lam, mu;           /* Lame parameters (lattice units)                      */
+  double gamma;             /* damping of XB_MODE_SMOOTH                            */
+  const void *f_in;         /* (8, nx, ny)                                          */
+  void *f_out;              /* STEP/SMOOTH: (8, nx, ny); RESIDUAL: unused           */
+  void *fpc_out;            /* STEP only, optional: post-collision populations      */
+  const void *force_a;      /* optional (2, nx, ny): body force (finest level)      */
+  const void *force_b;      /* optional (2, nx, ny): external force                 */
+  const void *resid;        /* optional (8, ix1-ix0, iy1-iy0): subtracted on box    */
+  int32_t ix0, ix1, iy0, iy1; /* interior box (cells without multigrid BC layers) */
+  const xb_bc_table *bc;    /* optional                                             */
+  void *coarse_out;         /* RESIDUAL: (8, (ix1-ix0)/2, (iy1-iy0)/2)              */
+  double *norm_partials;    /* RESIDUAL: xb_elast_partials_count() doubles          */
+} xb_elast_args;
+
+typedef struct xb_darcy_args {
+  int32_t nx, ny;
+  int32_t dtype;
+  int32_t pad_;
+  double omega;             /* scalar relaxation rate (used if omega_field null)    */
+  const void *omega_field;  /* optional (nx, ny) per-cell relaxation rate           */
+  const void *f_in;         /* (9, nx, ny)                                          */
+  void *f_out;              /* (9, nx, ny)                                          */
+  void *fpc_out;            /* optional post-collision populations                  */
+  const void *source_a;     /* optional (nx, ny): static source                     */
+  const void *source_b;     /* optional (nx, ny): external source                   */
+  const xb_bc_table *bc;    /* optional                                             */
+} xb_darcy_args;
+
+/* ---- library ---------------------------------------------------------------- */
+int xb_abi_version(void);
+const char *xb_last_error(void);
+
+/* ---- elasticity (D2Q8) --------------------------------------------------------- */
+
+/* One fused collide -> post-collision BCs -> stream -> post-streaming BCs pass.
+ * STEP replaces LBMElasticity._step (reference src/elasticity.py:1169-1219, with
+ * collision :959-1000, apply_bc :1060-1091, streaming :738-803); SMOOTH is one
+ * damped multigrid smoothing step (:1258-1267, :1296-1305 with
+ * substract_residual :1314-1344); RESIDUAL is the residual, its 2x2 restriction
+ * and the squared norm of the V-cycle (:1278-1287, _restrict :850-876, the norm
+ * at :1307-1311). */
+int xb_elast_step(const xb_elast_args *args, void *stream);
+
+/* Number of doubles in norm_partials for a RESIDUAL call with these args. */
+int64_t xb_elast_partials_count(const xb_elast_args *args);
+
+/* f[interior] += prolong(coarse) (piecewise-constant 2x2 injection).
+ * Replaces _prolong (src/elasticity.py:878-906) and the .at[].add of :1290-1293.
+ * coarse is the full coarse array (8, cnx, cny); its interior starts at (ix0, iy0). */
+int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f,
+                         const void *coarse, int32_t cnx, int32_t cny,
+                         int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1,
+                         void *stream);
+
+/* eta (2,nx,ny), sigma (3,nx,ny) from f and the optional forces.
+ * Replaces update_macroscopic (src/elasticity.py:1002-1038). Either output may be null. */
+int xb_elast_macro(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
+                   const void *f, const void *force_a, const void *force_b,
+                   void *eta_out, void *sigma_out, void *stream);
+
+/* div = (-1.5/(3(lam+mu)+1)) * m_s(f - f_sub) (f_sub optional).
+ * Replaces update_divergence (src/elasticity.py:1040-1058). */
+int xb_elast_divergence(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
+                        const void *f, const void *f_sub, void *div_out, void *stream);
+
+/* f = equilibrium(eta, sigma) with eta shifted by -F/2 for each given force.
+ * Replaces equilibrium (src/elasticity.py:908-957). */
+int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
+                         const void *eta, const void *sigma, const void *force_a,
+                         const void *force_b, void *f_out, void *stream);
+
+/* ---- Darcy (D2Q9 BGK advection-diffusion-reaction, zero velocity) ------------------ */
+
+/* Fused BGK collide -> BCs -> stream -> BCs.  Replaces
+ * AdvectionDiffusionReactionBGK.step (src/darcy.py:131-171, collision :106-129). */
+int xb_darcy_step(const xb_darcy_args *args, void *stream);
+
+/* p = sum_d f_d + (s_a + s_b)/2 ; grad = sum_d f_d (-3 omega) c_d.
+ * Replaces update_macroscopic (src/darcy.py:36-73). Either output may be null. */
+int xb_darcy_macro(int32_t nx, int32_t ny, int32_t dtype, double omega,
+                   const void *omega_field, const void *f, const void *source_a,
+                   const void *source_b, void *p_out, void *grad_out, void *stream);
+
+/* out = a*base + b*G(q) with G(q) = sum_d qs_d (-3 scale w_d c_d), qs_d(x) = q(x - c_d)
+ * or the boundary rule of the cell (bounce-back / anti-bounce-back walls only).
+ * Replaces gradient (src/darcy.py:75-104). base may be null (then a is ignored). */
+int xb_darcy_gradient(int32_t nx, int32_t ny, int32_t dtype, const void *q, double scale,
+                      const xb_bc_table *bc, const void *base, double a, double b,
+                      void *out, void *stream);
+
+/* ---- helpers of the Biot coupler (src/poroelasticity.py:303-339) --------------------- */
+
+/* out = a*x + b*y over n elements (y may be null). */
+int xb_lincomb(int64_t n, int32_t dtype, double a, const void *x, double b, const void *y,
+               void *out, void *stream);
+
+/* partials[k] = sum over a slice of (a - b)^2 (b may be null) in FP64; returns the
+ * number of partials written through *count.  Used by convergence checks. */
+int xb_sqdiff_partials(int64_t n, int32_t dtype, const void *a, const void *b,
+                       double *partials, int32_t max_partials, int32_t *count, void *stream);
+
+/* out[0] = sqrt(sum partials[0..n)) in a fixed order; out[1] = 1 if non-finite. */
+int xb_norm_finalize(const double *partials, int32_t n, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XLB_BIOT_H */

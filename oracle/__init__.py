@@ -1,0 +1,40 @@
+"""CPU oracle for the lattice-Boltzmann elasticity / Darcy / Biot path.
+
+TEST INFRASTRUCTURE ONLY.  This package is a plain torch-CPU restatement of the
+reference solvers (s-lunowa/xlb-biot, ``src/elasticity.py``, ``src/darcy.py``,
+``src/poroelasticity.py``, ``src/boundary_conditions.py``, ``src/lattice.py``,
+``src/base.py``).  It follows the reference operation for operation, including
+its quirks (duplicate-index scatters inside the halfway boundary conditions,
+row-major mask pairing of the elastic anti-bounce-back, last-BC-wins, the
+halo marked solid in the grid mask, ``t_max + 1`` steps in ``run``).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+may import it, and only as the checker (or as the timed CPU restatement of the
+reference algorithm).  The product in ``xlb-biot_amd/src`` never imports it.
+
+Arrays use the reference layout ``(nx, ny, q)`` (cell-major), in the dtype of the
+precision policy (float64 by default).  Every function cites the reference
+file:line it restates.
+
+Parity pins (what checks each part; see ``tests/test_oracle_*.py``):
+
+* lattice tables: closed-form constants, moment identities;
+* transfer matrices: ``(m_to_f @ f_to_m)[:5] = I``;
+* equilibrium/macroscopic: exact round trip;
+* collision: one-cell brute force against the moment relaxation written out;
+* streaming: ``numpy.roll`` brute force, per-direction sums conserved;
+* restrict/prolong: 2x2 sum / injection, ``restrict(prolong(c)) = 4c``;
+* V-cycle: ``levels=1`` reduces to ``_step``; residual equals ``|f - S(f)|``;
+* elasticity: manufactured periodic solution converges at O(nx^-2);
+* Darcy: mass conservation, ``rho*w`` fixed point, analytic mode decay, O(nx^-2);
+* gradient: exact on linear fields / spectral symbol of the D2Q9 stencil;
+* boundary conditions: hand-derived missing-direction tables on small grids,
+  Dirichlet/Neumann convergence orders of the reference drivers;
+* Biot: manufactured periodic solution (``examples/poroelasticity/periodic.py``)
+  converges at O(dx^2).
+
+The JAX out-of-bounds semantics used when fine-level Darcy boundary indices are
+applied to coarse multigrid arrays (clamp on gather, drop on scatter) are taken
+from JAX's documented ``.at[]`` behaviour; no reference run pins them
+("parity unpinned" for that combination: multigrid + Biot + Darcy walls).
+"""

@@ -1,0 +1,212 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Tolerances (relative, max-norm over the compared array): 1e-12 in FP64, 1e-5 in FP32.
+"""
+import numpy as np
+import pytest
+import torch
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64/f64": 1e-12, "f32/f32": 1e-5}
+
+
+def rel(a, b):
+    a = torch.as_tensor(a).detach().to("cpu", torch.float64)
+    b = torch.as_tensor(b).detach().to("cpu", torch.float64)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    den = b.abs().max().item()
+    return (a - b).abs().max().item() / (den if den > 0 else 1.0)
+
+
+def say(*args):
+    print(*args, flush=True)
+
+
+@pytest.mark.parametrize("precision", ["f64/f64", "f32/f32"])
+def test_elasticity_periodic_steps(precision):
+    say("elasticity periodic", precision)
+    g = P.periodic_elasticity("gpu", nx=64, precision=precision)
+    o = P.periodic_elasticity("oracle", nx=64, precision=precision)
+    fg, fo = g.assign_fields_sharded(), o.assign_fields_sharded()
+    for t in range(40):
+        fg, pg = g.step(fg, t, return_fpost=(t == 39))
+        fo, po = o.step(fo, t, return_fpost=(t == 39))
+    assert rel(fg, fo) < TOL[precision]
+    assert rel(pg, po) < TOL[precision]
+    eg, sg = g.update_macroscopic(fg)
+    eo, so = o.update_macroscopic(fo)
+    assert rel(eg, eo) < TOL[precision]
+    assert rel(sg, so) < TOL[precision]
+
+
+def test_elasticity_periodic_nonsquare_and_ragged():
+    # ny not a multiple of the 62-cell strip, odd nx: partial strips and row blocks
+    mg, mo = P.modules("gpu"), P.modules("oracle")
+    for nx, ny in ((37, 75), (5, 130), (1, 3), (64, 62), (3, 63)):
+        say("elasticity ragged", nx, ny)
+        kw = {"lambda": 1.3, "mu": 0.6, "nx": nx, "ny": ny, "nz": 0, "precision": "f64/f64"}
+        g = mg["Elasticity"](lattice=mg["D2Q8"]("f64/f64"), print_info_rate=0, **kw)
+        o = mo["Elasticity"](lattice=mo["D2Q8"](), **kw)
+        rng = np.random.default_rng(7)
+        f0 = rng.standard_normal((nx, ny, 8)) * 1e-3
+        fg = torch.as_tensor(f0, device="cuda")
+        fo = torch.as_tensor(f0)
+        ext = rng.standard_normal((nx, ny, 2)) * 1e-4
+        for t in range(7):
+            fg, _ = g.step(fg, t, external_force=torch.as_tensor(ext, device="cuda"))
+            fo, _ = o.step(fo, t, external_force=torch.as_tensor(ext))
+        assert rel(fg, fo) < 1e-12, (nx, ny)
+
+
+@pytest.mark.parametrize("name,precision", [("neumann", "f64/f64"), ("dirichlet", "f64/f64"),
+                                            ("mixed", "f64/f64"), ("neumann", "f32/f32"), ("mixed", "f32/f32")])
+def test_elasticity_walls_steps(name, precision):
+    say("elasticity walls", name, precision)
+    make = {"neumann": P.neumann_elasticity, "dirichlet": P.dirichlet_elasticity,
+            "mixed": P.mixed_halfway_elasticity}[name]
+    g, o = make("gpu", precision=precision), make("oracle", precision=precision)
+    rng = np.random.default_rng(3)
+    f0 = rng.standard_normal((g.nx, g.ny, 8)) * 1e-3
+    fg = torch.as_tensor(f0, device="cuda").to(g.precisionPolicy.torch_compute)
+    fo = torch.as_tensor(f0).to(o.precisionPolicy.compute_dtype)
+    for t in range(25):
+        fg, _ = g.step(fg, t)
+        fo, _ = o.step(fo, t)
+    assert rel(fg, fo) < TOL[precision]
+
+
+@pytest.mark.parametrize("name,levels", [("periodic", 3), ("neumann", 2), ("neumann", 3), ("dirichlet", 3)])
+def test_elasticity_vcycle(name, levels):
+    say("elasticity V-cycle", name, levels)
+    if name == "periodic":
+        g, o = P.periodic_elasticity("gpu", nx=32, levels=levels), P.periodic_elasticity("oracle", nx=32, levels=levels)
+    else:
+        make = {"neumann": P.neumann_elasticity, "dirichlet": P.dirichlet_elasticity}[name]
+        g, o = make("gpu", n=16, levels=levels), make("oracle", n=16, levels=levels)
+    fg, fo = g.assign_fields_sharded(), o.assign_fields_sharded()
+    for t in range(4):
+        fg, rg = g.step(fg, t)
+        fo, ro = o.step(fo, t)
+        assert abs(rg.item() - ro.item()) <= 1e-12 * max(abs(ro.item()), 1e-300), (t, rg.item(), ro.item())
+    assert rel(fg, fo) < 1e-12
+
+
+def test_elasticity_macro_div_equilibrium():
+    say("elasticity macro/div/equilibrium")
+    g = P.neumann_elasticity("gpu", n=16)
+    o = P.neumann_elasticity("oracle", n=16)
+    rng = np.random.default_rng(11)
+    eta = rng.standard_normal((g.nx, g.ny, 2))
+    sig = rng.standard_normal((g.nx, g.ny, 3))
+    ext = rng.standard_normal((g.nx, g.ny, 2)) * 1e-2
+    fg = g.equilibrium(torch.as_tensor(eta, device="cuda"), torch.as_tensor(sig, device="cuda"),
+                       external_force=torch.as_tensor(ext, device="cuda"))
+    fo = o.equilibrium(torch.as_tensor(eta), torch.as_tensor(sig), external_force=torch.as_tensor(ext))
+    assert rel(fg, fo) < 1e-13
+    eg, sg = g.update_macroscopic(fg, torch.as_tensor(ext, device="cuda"))
+    eo, so = o.update_macroscopic(fo, torch.as_tensor(ext))
+    assert rel(eg, eo) < 1e-13 and rel(sg, so) < 1e-13
+    # round trip of the inverse pair
+    assert rel(eg, torch.as_tensor(eta)) < 1e-12 and rel(sg, torch.as_tensor(sig)) < 1e-12
+    assert rel(g.update_divergence(fg), o.update_divergence(fo)) < 1e-13
+    cg = g.collision(fg, torch.as_tensor(ext, device="cuda"))
+    co = o.collision(fo, torch.as_tensor(ext))
+    assert rel(cg, co) < 1e-13
+
+
+@pytest.mark.parametrize("precision,hetero,source", [("f64/f64", True, False), ("f32/f32", True, False),
+                                                      ("f64/f64", False, True)])
+def test_darcy_steps(precision, hetero, source):
+    say("darcy C2-like", precision, hetero, source)
+    g = P.darcy_c2("gpu", n=40, precision=precision, heterogeneous=hetero, source=source)
+    o = P.darcy_c2("oracle", n=40, precision=precision, heterogeneous=hetero, source=source)
+    fg, fo = g.assign_fields_sharded(), o.assign_fields_sharded()
+    assert rel(fg, fo) < TOL[precision]
+    ext = 1e-3 * np.sin(np.arange(g.nx * g.ny).reshape(g.nx, g.ny, 1))
+    for t in range(30):
+        e = ext if t % 2 else None
+        fg, pg = g.step(fg, t, return_fpost=(t == 29),
+                        external_source=None if e is None else torch.as_tensor(e, device="cuda"))
+        fo, po = o.step(fo, t, return_fpost=(t == 29),
+                        external_source=None if e is None else torch.as_tensor(e).to(o.precisionPolicy.compute_dtype))
+    assert rel(fg, fo) < TOL[precision]
+    assert rel(pg, po) < TOL[precision]
+    rg, gg = g.update_macroscopic(fg, gradient=True)
+    ro, go = o.update_macroscopic(fo, gradient=True)
+    assert rel(rg, ro) < TOL[precision] and rel(gg, go) < TOL[precision]
+    for scale in (1.0, 0.37):
+        assert rel(g.gradient(rg, scale), o.gradient(ro, scale)) < TOL[precision]
+
+
+def test_darcy_equilibrium_walls():
+    say("darcy equilibrium walls")
+    g, o = P.darcy_equilibrium("gpu"), P.darcy_equilibrium("oracle")
+    fg, fo = g.assign_fields_sharded(), o.assign_fields_sharded()
+    assert rel(fg, fo) < 1e-14
+    for t in range(20):
+        fg, _ = g.step(fg, t)
+        fo, _ = o.step(fo, t)
+    assert rel(fg, fo) < 1e-12
+
+
+@pytest.mark.parametrize("levels,coupling,alt", [(1, "centered", False), (1, "implicit", True), (3, "centered", False),
+                                                 (2, "explicit", False)])
+def test_biot_periodic_steps(levels, coupling, alt):
+    say("biot periodic", levels, coupling, alt)
+    g = P.biot_periodic("gpu", nx=32, levels=levels, steps=2, coupling=coupling, alt=alt)
+    o = P.biot_periodic("oracle", nx=32, levels=levels, steps=2, coupling=coupling, alt=alt)
+    sg = g.initial_state()
+    so = o.initial_state()
+    for a, b in zip(sg, so):
+        assert rel(a.permute(1, 2, 0), b) < 1e-12
+    fe, fd, dde = (t.permute(1, 2, 0) for t in sg)
+    oe, od, odde = so
+    for t in range(3):
+        outg = g.step(fe, fd, dde, t)
+        outo = o.step(oe, od, odde, t)
+        for k, (a, b) in enumerate(zip(outg, outo)):
+            assert rel(a, b) < 1e-11, (t, k, rel(a, b))
+        fe, fd, dde = outg[0], outg[1], outg[4]
+        oe, od, odde = outo[0], outo[1], outo[4]
+
+
+def test_biot_consolidation_mg_walls():
+    say("biot consolidation MG walls")
+    g = P.biot_consolidation("gpu", nx=32, levels=3, steps=2)
+    o = P.biot_consolidation("oracle", nx=32, levels=3, steps=2)
+    sg = g.initial_state()
+    so = o.initial_state()
+    fe, fd, dde = (t.permute(1, 2, 0) for t in sg)
+    oe, od, odde = so
+    for t in range(3):
+        outg = g.step(fe, fd, dde, t)
+        outo = o.step(oe, od, odde, t)
+        for k, (a, b) in enumerate(zip(outg, outo)):
+            assert rel(a, b) < 1e-11, (t, k, rel(a, b))
+        fe, fd, dde = outg[0], outg[1], outg[4]
+        oe, od, odde = outo[0], outo[1], outo[4]
+
+
+def test_c1_converges_to_closed_form():
+    say("C1 64^2 to residual 1e-10")
+    nx = 64
+    g = P.periodic_elasticity("gpu", nx=nx)
+    f = g.assign_fields_sharded()
+    r0 = None
+    for k in range(20000):
+        fn, _ = g.step(f, k)
+        if k % 10 == 0:
+            r = torch.linalg.norm(fn - f).item()
+            r0 = r if r0 is None else r0
+            if r / r0 < 1e-10:
+                f = fn
+                break
+        f = fn
+    eta, _ = g.update_macroscopic(f)
+    ue = P.manufactured_c1(nx) / (8 * np.pi ** 2)
+    err = np.linalg.norm(eta.cpu().numpy() - ue) / np.linalg.norm(ue)
+    # second order: 0.0256 / 0.0064 / 0.0016 at 16 / 32 / 64 cells (oracle)
+    assert 1.4e-3 < err < 1.8e-3, err

@@ -1,0 +1,704 @@
+// gfx950 kernels and C ABI of the lattice-Boltzmann elasticity / Darcy / Biot path.
+//
+// The hot kernel is k_sweep: one wavefront owns a strip of 62 output cells along
+// y (plus one halo lane on each side) and walks down a block of rows along x.
+// For every row it loads the populations once (coalesced SoA loads), collides
+// them in registers, applies the post-collision boundary rules, and keeps three
+// consecutive rows of post-collision populations in registers.  Streaming is a
+// pull: direction d of the output row x takes row x - cx_d of the register ring
+// and lane (lane - cy_d) through a one-lane wavefront shift.  Post-streaming
+// boundary rules, damping (multigrid smoothing) or residual + 2x2 restriction +
+// squared norm are fused behind it.  Each population is read and written once
+// per step; halo rows and lanes are re-read from L2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+
+#include "xb_device.h"
+
+using namespace xb;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(XB_EHIP, "%s: %s", what, hipGetErrorString(e));
+  return XB_OK;
+}
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+struct SweepGeom {
+  int nx, ny;
+  int ox, oy, ex, ey;     // output box
+  int rows;               // rows per wavefront item
+  int nstrips, nitems;
+  int ix0, ix1, iy0, iy1; // box carrying the subtracted residual
+};
+
+// Rows per wavefront: enough items to give every SIMD several waves, never fewer
+// than 2 (residual restriction pairs rows) and at most 64.
+SweepGeom make_geom(int nx, int ny, int ox, int oy, int ex, int ey, int ix0, int ix1, int iy0, int iy1) {
+  SweepGeom g{};
+  g.nx = nx; g.ny = ny; g.ox = ox; g.oy = oy; g.ex = ex; g.ey = ey;
+  g.ix0 = ix0; g.ix1 = ix1; g.iy0 = iy0; g.iy1 = iy1;
+  g.nstrips = cdiv(ey, STRIP);
+  const int target = 8192;
+  int rows = 64;
+  while (rows > 2 && (int64_t)g.nstrips * cdiv(ex, rows) < target) rows >>= 1;
+  g.rows = rows;
+  g.nitems = g.nstrips * cdiv(ex, rows);
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES>
+__global__ void __launch_bounds__(BLOCK)
+k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
+        T *__restrict__ fpc, const T *__restrict__ res, const BCDev<T> bc, const T omg, const T gamma,
+        T *__restrict__ coarse, double *__restrict__ partials) {
+  constexpr int Q = Coll::Q;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int item = xcd_block_id() * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
+  if (item >= g.nitems) return; // uniform per wavefront
+
+  const int strip = item % g.nstrips;
+  const int rb = item / g.nstrips;
+  const int ys = g.oy + strip * STRIP;
+  const int yend = min(ys + STRIP, g.oy + g.ey);
+  const int yv = ys - 1 + lane;
+  const bool out_lane = lane >= 1 && yv < yend;
+  const int y = wrapi(min(yv, yend), g.ny);
+  const int xs = g.ox + rb * g.rows;
+  const int xe = min(xs + g.rows, g.ox + g.ex);
+  const size_t N = (size_t)g.nx * (size_t)g.ny;
+  const int nyi = g.iy1 - g.iy0;
+  const size_t Ni = (size_t)(g.ix1 - g.ix0) * (size_t)nyi;
+  const bool y_in_box = y >= g.iy0 && y < g.iy1;
+
+  // Load row x, subtract the residual where it lives, collide, post-collision rules.
+  auto load_collide = [&](int x, T(&fr)[Q], T(&pc)[Q], int &slot) {
+    const size_t cell = (size_t)x * g.ny + y;
+    T pre[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      fr[d] = fin[d * N + cell];
+      pre[d] = fr[d];
+    }
+    if constexpr (HAS_RES) {
+      if (y_in_box && x >= g.ix0 && x < g.ix1) {
+        const size_t ci = (size_t)(x - g.ix0) * nyi + (size_t)(y - g.iy0);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) pre[d] = pre[d] - res[d * Ni + ci];
+      }
+    }
+    coll.collide(pre, pc, cell);
+    slot = -1;
+    if constexpr (HAS_BC) {
+      slot = bc.slot[cell];
+      if (slot >= 0 && bc.code_pc != nullptr)
+        apply_rules<T, Q>(pc, pre, bc.code_pc + (size_t)slot * Q, bc.const_pc + (size_t)slot * Q);
+    }
+  };
+
+  T R0[Q], R1[Q], R2[Q], F1[Q], F2[Q];
+  int S1 = -1, S2 = -1;
+  load_collide(wrapi(xs - 1, g.nx), F2, R0, S2);
+  load_collide(xs, F1, R1, S1);
+
+  double nrm = 0.0;
+  T acc[Q];
+#pragma unroll
+  for (int d = 0; d < Q; ++d) acc[d] = T(0);
+  const int ncy = g.ey >> 1;
+  const size_t Nc = (size_t)(g.ex >> 1) * (size_t)ncy;
+  const bool first_of_pair = out_lane && (((yv - g.oy) & 1) == 0);
+
+  for (int x = xs; x < xe; ++x) {
+    load_collide(wrapi(x + 1, g.nx), F2, R2, S2);
+
+    T v[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      T s = Coll::cx(d) == 1 ? R0[d] : (Coll::cx(d) == 0 ? R1[d] : R2[d]);
+      if (Coll::cy(d) == 1) s = from_prev_lane(s);
+      else if (Coll::cy(d) == -1) s = from_next_lane(s);
+      v[d] = s;
+    }
+    if constexpr (HAS_BC) {
+      if (S1 >= 0 && bc.code_ps != nullptr)
+        apply_rules<T, Q>(v, R1, bc.code_ps + (size_t)S1 * Q, bc.const_ps + (size_t)S1 * Q);
+    }
+
+    const size_t cell = (size_t)x * g.ny + y;
+    if constexpr (MODE == XB_MODE_STEP) {
+      if (out_lane) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) fout[d * N + cell] = v[d];
+        if (fpc != nullptr) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) fpc[d * N + cell] = R1[d];
+        }
+      }
+    } else if constexpr (MODE == XB_MODE_SMOOTH) {
+      if (out_lane) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) fout[d * N + cell] = omg * F1[d] + gamma * v[d];
+      }
+    } else {
+      // residual r = f - S(f - r_in) on the interior box, 2x2 sum, squared norm
+      T pair[Q];
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        const T r = F1[d] - v[d];
+        if (out_lane) nrm += (double)r * (double)r;
+        pair[d] = r + from_next_lane(r);
+      }
+      if (((x - g.ox) & 1) == 0) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) acc[d] = pair[d];
+      } else if (first_of_pair) {
+        const size_t cc = (size_t)((x - g.ox) >> 1) * ncy + (size_t)((yv - g.oy) >> 1);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) coarse[d * Nc + cc] = acc[d] + pair[d];
+      }
+    }
+
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      R0[d] = R1[d];
+      R1[d] = R2[d];
+      F1[d] = F2[d];
+    }
+    S1 = S2;
+  }
+
+  if constexpr (MODE == XB_MODE_RESIDUAL) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off, WAVE);
+    if (lane == 0) partials[item] = nrm;
+  }
+}
+
+template <typename T, class Coll, int MODE>
+int launch_sweep(const SweepGeom &g, const Coll &coll, const void *fin, void *fout, void *fpc, const void *res,
+                 const xb_bc_table *bct, double gamma, void *coarse, double *partials, hipStream_t st) {
+  BCDev<T> bc;
+  const bool has_bc = bct != nullptr && bct->cell_slot != nullptr;
+  if (has_bc) {
+    bc.slot = bct->cell_slot;
+    bc.code_pc = bct->code_pc;
+    bc.code_ps = bct->code_ps;
+    bc.const_pc = static_cast<const T *>(bct->const_pc);
+    bc.const_ps = static_cast<const T *>(bct->const_ps);
+  }
+  const bool has_res = res != nullptr;
+  const dim3 grid(cdiv(g.nitems, WAVES_PER_BLOCK)), block(BLOCK);
+  const T omg = T(1.0 - gamma), gm = T(gamma);
+  const T *fi = static_cast<const T *>(fin);
+  T *fo = static_cast<T *>(fout);
+  T *fp = static_cast<T *>(fpc);
+  const T *rs = static_cast<const T *>(res);
+  T *co = static_cast<T *>(coarse);
+#define XB_SWEEP(BC, RES) \
+  hipLaunchKernelGGL((k_sweep<T, Coll, MODE, BC, RES>), grid, block, 0, st, g, coll, fi, fo, fp, rs, bc, omg, gm, co, partials)
+  if (has_bc && has_res) XB_SWEEP(true, true);
+  else if (has_bc) XB_SWEEP(true, false);
+  else if (has_res) XB_SWEEP(false, true);
+  else XB_SWEEP(false, false);
+#undef XB_SWEEP
+  return check_launch("sweep kernel");
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise kernels (grid-stride over cells).
+inline dim3 ew_grid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65535 * 8) b = 65535 * 8;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+template <typename T>
+__global__ void k_prolong_add(int nx, int ny, T *__restrict__ f, const T *__restrict__ c, int cnx, int cny, int ix0,
+                              int ix1, int iy0, int iy1) {
+  const int nxi = ix1 - ix0, nyi = iy1 - iy0;
+  const int64_t n = (int64_t)nxi * nyi;
+  const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int xi = (int)(t / nyi), yi = (int)(t % nyi);
+    const size_t cell = (size_t)(ix0 + xi) * ny + (iy0 + yi);
+    const size_t cc = (size_t)(ix0 + (xi >> 1)) * cny + (iy0 + (yi >> 1));
+#pragma unroll
+    for (int d = 0; d < 8; ++d) f[d * N + cell] = f[d * N + cell] + c[d * Nc + cc];
+  }
+}
+
+template <typename T>
+__global__ void k_elast_macro(int64_t N, T cs, T cd, const T *__restrict__ f, const T *__restrict__ fa,
+                              const T *__restrict__ fb, T *__restrict__ eta, T *__restrict__ sig) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    T v[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) v[d] = f[d * N + i];
+    T e0 = v[0] - v[2] + v[4] - v[5] - v[6] + v[7];
+    T e1 = v[1] - v[3] + v[4] + v[5] - v[6] - v[7];
+    if (eta != nullptr) {
+      if (fa != nullptr) { e0 = e0 + T(0.5) * fa[i]; e1 = e1 + T(0.5) * fa[N + i]; }
+      if (fb != nullptr) { e0 = e0 + T(0.5) * fb[i]; e1 = e1 + T(0.5) * fb[N + i]; }
+      eta[i] = e0;
+      eta[N + i] = e1;
+    }
+    if (sig != nullptr) {
+      const T ms = (v[0] + v[1] + v[2] + v[3] + T(2) * (v[4] + v[5] + v[6] + v[7])) * cs;
+      const T md = (v[0] - v[1] + v[2] - v[3]) * cd;
+      const T m11 = v[4] - v[5] + v[6] - v[7];
+      sig[i] = ms + md;
+      sig[N + i] = (T(2) * cd) * m11;
+      sig[2 * N + i] = ms - md;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_elast_div(int64_t N, T k, const T *__restrict__ f, const T *__restrict__ fs, T *__restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    T v[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) v[d] = fs != nullptr ? f[d * N + i] - fs[d * N + i] : f[d * N + i];
+    out[i] = k * (v[0] + v[1] + v[2] + v[3] + T(2) * (v[4] + v[5] + v[6] + v[7]));
+  }
+}
+
+template <typename T>
+__global__ void k_elast_equilibrium(int64_t N, T cs, T cd, T h, T n, const T *__restrict__ eta,
+                                    const T *__restrict__ sig, const T *__restrict__ fa, const T *__restrict__ fb,
+                                    T *__restrict__ f) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    T e0 = eta[i], e1 = eta[N + i];
+    if (fa != nullptr) { e0 = e0 - T(0.5) * fa[i]; e1 = e1 - T(0.5) * fa[N + i]; }
+    if (fb != nullptr) { e0 = e0 - T(0.5) * fb[i]; e1 = e1 - T(0.5) * fb[N + i]; }
+    const T sxx = sig[i], sxy = sig[N + i], syy = sig[2 * N + i];
+    const T M2 = (sxx + syy) * cs, M3 = (sxx - syy) * cd, M4 = sxy * cd;
+    const T M5 = e0 / T(3), M6 = e1 / T(3);
+    const T hM2 = h * M2, nM2 = n * M2;
+    f[0 * N + i] = T(0.5) * e0 + hM2 + T(0.25) * M3 - T(0.5) * M5;
+    f[1 * N + i] = T(0.5) * e1 + hM2 - T(0.25) * M3 - T(0.5) * M6;
+    f[2 * N + i] = -T(0.5) * e0 + hM2 + T(0.25) * M3 + T(0.5) * M5;
+    f[3 * N + i] = -T(0.5) * e1 + hM2 - T(0.25) * M3 + T(0.5) * M6;
+    f[4 * N + i] = nM2 + T(0.25) * M4 + T(0.25) * M5 + T(0.25) * M6;
+    f[5 * N + i] = nM2 - T(0.25) * M4 - T(0.25) * M5 + T(0.25) * M6;
+    f[6 * N + i] = nM2 + T(0.25) * M4 - T(0.25) * M5 - T(0.25) * M6;
+    f[7 * N + i] = nM2 - T(0.25) * M4 + T(0.25) * M5 - T(0.25) * M6;
+  }
+}
+
+template <typename T>
+__global__ void k_darcy_macro(int64_t N, T m3om, const T *__restrict__ omf, const T *__restrict__ f,
+                              const T *__restrict__ sa, const T *__restrict__ sb, T *__restrict__ p,
+                              T *__restrict__ grad) {
+  using C = DarcyColl<T>;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    T v[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) v[d] = f[d * N + i];
+    if (p != nullptr) {
+      T rho = v[0];
+#pragma unroll
+      for (int d = 1; d < 9; ++d) rho = rho + v[d];
+      if (sa != nullptr && sb != nullptr) rho = rho + T(0.5) * (sa[i] + sb[i]);
+      else if (sa != nullptr) rho = rho + T(0.5) * sa[i];
+      else if (sb != nullptr) rho = rho + T(0.5) * sb[i];
+      p[i] = rho;
+    }
+    if (grad != nullptr) {
+      if (omf == nullptr) {
+        T g0 = T(0), g1 = T(0);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+          if (C::cx(d) != 0) g0 = g0 + v[d] * (C::cx(d) > 0 ? m3om : -m3om);
+          if (C::cy(d) != 0) g1 = g1 + v[d] * (C::cy(d) > 0 ? m3om : -m3om);
+        }
+        grad[i] = g0;
+        grad[N + i] = g1;
+      } else {
+        T g0 = T(0), g1 = T(0);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+          if (C::cx(d) != 0) g0 = C::cx(d) > 0 ? g0 + v[d] : g0 - v[d];
+          if (C::cy(d) != 0) g1 = C::cy(d) > 0 ? g1 + v[d] : g1 - v[d];
+        }
+        const T s = T(-3) * omf[i];
+        grad[i] = g0 * s;
+        grad[N + i] = g1 * s;
+      }
+    }
+  }
+}
+
+// out = a*base + b * sum_d qs_d * ((-3 scale) w_d) c_d, qs_d(x) = q(x - c_d) or the rule of the cell.
+template <typename T>
+__global__ void k_darcy_gradient(int nx, int ny, const T *__restrict__ q, T m3s, const BCDev<T> bc,
+                                 const T *__restrict__ base, T a, T b, T *__restrict__ out) {
+  using C = DarcyColl<T>;
+  const int64_t N = (int64_t)nx * ny;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    T qs[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) qs[d] = q[(size_t)wrapi(x - C::cx(d), nx) * ny + wrapi(y - C::cy(d), ny)];
+    if (bc.slot != nullptr) {
+      const int s = bc.slot[i];
+      if (s >= 0) {
+        T src[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) src[d] = qs[0];
+        apply_rules<T, 9>(qs, src, bc.code_ps + (size_t)s * 9, bc.const_ps + (size_t)s * 9);
+      }
+    }
+    T g0 = T(0), g1 = T(0);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const T cw = m3s * T(C::wd(d));
+      if (C::cx(d) != 0) g0 = g0 + qs[d] * (C::cx(d) > 0 ? cw : -cw);
+      if (C::cy(d) != 0) g1 = g1 + qs[d] * (C::cy(d) > 0 ? cw : -cw);
+    }
+    if (base != nullptr) {
+      out[i] = a * base[i] + b * g0;
+      out[N + i] = a * base[N + i] + b * g1;
+    } else {
+      out[i] = b * g0;
+      out[N + i] = b * g1;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_lincomb(int64_t n, T a, const T *__restrict__ x, T b, const T *__restrict__ y, T *__restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = y != nullptr ? a * x[i] + b * y[i] : a * x[i];
+}
+
+// One partial per block, fixed order inside the block (deterministic for a fixed grid).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sqdiff(int64_t n, const T *__restrict__ a, const T *__restrict__ b,
+                                                double *__restrict__ partials) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = b != nullptr ? (double)a[i] - (double)b[i] : (double)a[i];
+    s += d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void k_norm_finalize(const double *__restrict__ partials, int n, double *__restrict__ out) {
+  const int lane = threadIdx.x;
+  double s = 0.0;
+  for (int i = lane; i < n; i += WAVE) s += partials[i];
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+  if (lane == 0) {
+    out[0] = sqrt(s);
+    out[1] = (isfinite(s) ? 0.0 : 1.0);
+  }
+}
+
+template <typename T> BCDev<T> bcdev(const xb_bc_table *t) {
+  BCDev<T> b;
+  if (t != nullptr && t->cell_slot != nullptr) {
+    b.slot = t->cell_slot;
+    b.code_pc = t->code_pc;
+    b.code_ps = t->code_ps;
+    b.const_pc = static_cast<const T *>(t->const_pc);
+    b.const_ps = static_cast<const T *>(t->const_ps);
+  }
+  return b;
+}
+
+int check_dtype(int32_t dtype) {
+  if (dtype != XB_F64 && dtype != XB_F32) return fail(XB_EUNSUPPORTED, "unsupported dtype %d", dtype);
+  return XB_OK;
+}
+
+int check_bc(const xb_bc_table *t, bool need_ps) {
+  if (t == nullptr) return XB_OK;
+  if (t->nslots < 0) return fail(XB_EINVAL, "bc table: negative slot count");
+  if (t->nslots > 0 && t->cell_slot == nullptr) return fail(XB_EINVAL, "bc table: null cell_slot");
+  if (t->code_pc != nullptr && t->const_pc == nullptr) return fail(XB_EINVAL, "bc table: null const_pc");
+  if (t->code_ps != nullptr && t->const_ps == nullptr) return fail(XB_EINVAL, "bc table: null const_ps");
+  if (need_ps && t->cell_slot != nullptr && t->code_ps == nullptr) return fail(XB_EINVAL, "bc table: null code_ps");
+  return XB_OK;
+}
+
+int elast_geom(const xb_elast_args *a, SweepGeom &g) {
+  if (a->mode == XB_MODE_RESIDUAL)
+    g = make_geom(a->nx, a->ny, a->ix0, a->iy0, a->ix1 - a->ix0, a->iy1 - a->iy0, a->ix0, a->ix1, a->iy0, a->iy1);
+  else
+    g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, a->ix0, a->ix1, a->iy0, a->iy1);
+  return XB_OK;
+}
+
+template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st) {
+  ElastColl<T> c;
+  const double lm = a->lam + a->mu;
+  double g = 1.0 / (12.0 * lm - 4.0);
+  if (std::fabs(g) > 100.0) g = 0.0;
+  c.cs = T((3 * lm - 1) / (3 * lm + 1));
+  c.cd = T((6 * a->mu - 1) / (6 * a->mu + 1));
+  c.h = T(0.25 + 0.5 * g);
+  c.n = T(-0.25 * g);
+  c.fa = static_cast<const T *>(a->force_a);
+  c.fb = static_cast<const T *>(a->force_b);
+  c.N = (size_t)a->nx * a->ny;
+  SweepGeom geo;
+  elast_geom(a, geo);
+  switch (a->mode) {
+  case XB_MODE_STEP:
+    return launch_sweep<T, ElastColl<T>, XB_MODE_STEP>(geo, c, a->f_in, a->f_out, a->fpc_out, a->resid, a->bc, 1.0,
+                                                       nullptr, nullptr, st);
+  case XB_MODE_SMOOTH:
+    return launch_sweep<T, ElastColl<T>, XB_MODE_SMOOTH>(geo, c, a->f_in, a->f_out, nullptr, a->resid, a->bc,
+                                                         a->gamma, nullptr, nullptr, st);
+  default:
+    return launch_sweep<T, ElastColl<T>, XB_MODE_RESIDUAL>(geo, c, a->f_in, nullptr, nullptr, a->resid, a->bc, 1.0,
+                                                           a->coarse_out, a->norm_partials, st);
+  }
+}
+
+} // namespace
+
+// ============================================================================
+extern "C" {
+
+int xb_abi_version(void) { return XB_ABI_VERSION; }
+
+const char *xb_last_error(void) { return g_err; }
+
+int64_t xb_elast_partials_count(const xb_elast_args *a) {
+  if (a == nullptr) return -1;
+  SweepGeom g;
+  elast_geom(a, g);
+  return g.nitems;
+}
+
+int xb_elast_step(const xb_elast_args *a, void *stream) {
+  if (a == nullptr) return fail(XB_EINVAL, "xb_elast_step: null args");
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->nx <= 0 || a->ny <= 0) return fail(XB_EINVAL, "xb_elast_step: bad grid %d x %d", a->nx, a->ny);
+  if (a->f_in == nullptr) return fail(XB_EINVAL, "xb_elast_step: null f_in");
+  if (a->mode < XB_MODE_STEP || a->mode > XB_MODE_RESIDUAL) return fail(XB_EINVAL, "xb_elast_step: bad mode %d", a->mode);
+  if (a->ix0 < 0 || a->ix1 > a->nx || a->iy0 < 0 || a->iy1 > a->ny || a->ix0 >= a->ix1 || a->iy0 >= a->iy1)
+    return fail(XB_EINVAL, "xb_elast_step: bad interior box [%d,%d)x[%d,%d)", a->ix0, a->ix1, a->iy0, a->iy1);
+  if (a->mode != XB_MODE_RESIDUAL && a->f_out == nullptr) return fail(XB_EINVAL, "xb_elast_step: null f_out");
+  if (a->mode != XB_MODE_STEP && a->fpc_out != nullptr)
+    return fail(XB_EINVAL, "xb_elast_step: fpc_out is only produced by XB_MODE_STEP");
+  if (a->mode == XB_MODE_RESIDUAL) {
+    if (a->coarse_out == nullptr || a->norm_partials == nullptr)
+      return fail(XB_EINVAL, "xb_elast_step: residual mode needs coarse_out and norm_partials");
+    if (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1))
+      return fail(XB_EINVAL, "xb_elast_step: interior box must have even extents for restriction");
+  }
+  if (a->mode == XB_MODE_SMOOTH && !(a->gamma >= 0.0 && a->gamma <= 1.0))
+    return fail(XB_EINVAL, "xb_elast_step: gamma must be in [0,1]");
+  if (int r = check_bc(a->bc, false)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return a->dtype == XB_F64 ? elast_step_t<double>(a, st) : elast_step_t<float>(a, st);
+}
+
+int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const void *coarse, int32_t cnx, int32_t cny,
+                         int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (f == nullptr || coarse == nullptr) return fail(XB_EINVAL, "xb_elast_prolong_add: null pointer");
+  if (ix0 < 0 || ix1 > nx || iy0 < 0 || iy1 > ny || ix0 >= ix1 || iy0 >= iy1 || ((ix1 - ix0) & 1) || ((iy1 - iy0) & 1))
+    return fail(XB_EINVAL, "xb_elast_prolong_add: bad interior box");
+  if (cnx != ix0 + (ix1 - ix0) / 2 + (nx - ix1) || cny != iy0 + (iy1 - iy0) / 2 + (ny - iy1))
+    return fail(XB_EINVAL, "xb_elast_prolong_add: coarse grid %d x %d does not match the fine box", cnx, cny);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 gr = ew_grid((int64_t)(ix1 - ix0) * (iy1 - iy0));
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_prolong_add<double>, gr, dim3(256), 0, st, nx, ny, (double *)f, (const double *)coarse, cnx,
+                       cny, ix0, ix1, iy0, iy1);
+  else
+    hipLaunchKernelGGL(k_prolong_add<float>, gr, dim3(256), 0, st, nx, ny, (float *)f, (const float *)coarse, cnx, cny,
+                       ix0, ix1, iy0, iy1);
+  return check_launch("xb_elast_prolong_add");
+}
+
+int xb_elast_macro(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *f, const void *force_a,
+                   const void *force_b, void *eta_out, void *sigma_out, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || f == nullptr) return fail(XB_EINVAL, "xb_elast_macro: bad arguments");
+  const int64_t N = (int64_t)nx * ny;
+  const double cs = -1.5 * (lam + mu) / (3 * (lam + mu) + 1);
+  const double cd = -3 * mu / (6 * mu + 1);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_elast_macro<double>, ew_grid(N), dim3(256), 0, st, N, cs, cd, (const double *)f,
+                       (const double *)force_a, (const double *)force_b, (double *)eta_out, (double *)sigma_out);
+  else
+    hipLaunchKernelGGL(k_elast_macro<float>, ew_grid(N), dim3(256), 0, st, N, (float)cs, (float)cd, (const float *)f,
+                       (const float *)force_a, (const float *)force_b, (float *)eta_out, (float *)sigma_out);
+  return check_launch("xb_elast_macro");
+}
+
+int xb_elast_divergence(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *f, const void *f_sub,
+                        void *div_out, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || f == nullptr || div_out == nullptr) return fail(XB_EINVAL, "xb_elast_divergence: bad arguments");
+  const int64_t N = (int64_t)nx * ny;
+  const double k = -1.5 / (3 * (lam + mu) + 1);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_elast_div<double>, ew_grid(N), dim3(256), 0, st, N, k, (const double *)f,
+                       (const double *)f_sub, (double *)div_out);
+  else
+    hipLaunchKernelGGL(k_elast_div<float>, ew_grid(N), dim3(256), 0, st, N, (float)k, (const float *)f,
+                       (const float *)f_sub, (float *)div_out);
+  return check_launch("xb_elast_divergence");
+}
+
+int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *eta,
+                         const void *sigma, const void *force_a, const void *force_b, void *f_out, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || eta == nullptr || sigma == nullptr || f_out == nullptr)
+    return fail(XB_EINVAL, "xb_elast_equilibrium: bad arguments");
+  const int64_t N = (int64_t)nx * ny;
+  double g = 1.0 / (12.0 * (lam + mu) - 4.0);
+  if (std::fabs(g) > 100.0) g = 0.0;
+  const double cs = -1.0 - 1 / (3 * (lam + mu));
+  const double cd = -1.0 - 1 / (6 * mu);
+  const double h = 0.25 + 0.5 * g, n = -0.25 * g;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_elast_equilibrium<double>, ew_grid(N), dim3(256), 0, st, N, cs, cd, h, n, (const double *)eta,
+                       (const double *)sigma, (const double *)force_a, (const double *)force_b, (double *)f_out);
+  else
+    hipLaunchKernelGGL(k_elast_equilibrium<float>, ew_grid(N), dim3(256), 0, st, N, (float)cs, (float)cd, (float)h,
+                       (float)n, (const float *)eta, (const float *)sigma, (const float *)force_a,
+                       (const float *)force_b, (float *)f_out);
+  return check_launch("xb_elast_equilibrium");
+}
+
+int xb_darcy_step(const xb_darcy_args *a, void *stream) {
+  if (a == nullptr) return fail(XB_EINVAL, "xb_darcy_step: null args");
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->nx <= 0 || a->ny <= 0 || a->f_in == nullptr || a->f_out == nullptr)
+    return fail(XB_EINVAL, "xb_darcy_step: bad arguments");
+  if (int r = check_bc(a->bc, false)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny);
+  if (a->dtype == XB_F64) {
+    DarcyColl<double> c;
+    c.omega = a->omega;
+    c.sfac = 1 - 0.5 * a->omega;
+    c.om_field = static_cast<const double *>(a->omega_field);
+    c.sa = static_cast<const double *>(a->source_a);
+    c.sb = static_cast<const double *>(a->source_b);
+    return launch_sweep<double, DarcyColl<double>, XB_MODE_STEP>(g, c, a->f_in, a->f_out, a->fpc_out, nullptr, a->bc,
+                                                                 1.0, nullptr, nullptr, st);
+  }
+  DarcyColl<float> c;
+  c.omega = (float)a->omega;
+  c.sfac = (float)(1 - 0.5 * a->omega);
+  c.om_field = static_cast<const float *>(a->omega_field);
+  c.sa = static_cast<const float *>(a->source_a);
+  c.sb = static_cast<const float *>(a->source_b);
+  return launch_sweep<float, DarcyColl<float>, XB_MODE_STEP>(g, c, a->f_in, a->f_out, a->fpc_out, nullptr, a->bc, 1.0,
+                                                             nullptr, nullptr, st);
+}
+
+int xb_darcy_macro(int32_t nx, int32_t ny, int32_t dtype, double omega, const void *omega_field, const void *f,
+                   const void *source_a, const void *source_b, void *p_out, void *grad_out, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || f == nullptr) return fail(XB_EINVAL, "xb_darcy_macro: bad arguments");
+  const int64_t N = (int64_t)nx * ny;
+  const double m3om = -3 * omega;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_darcy_macro<double>, ew_grid(N), dim3(256), 0, st, N, m3om, (const double *)omega_field,
+                       (const double *)f, (const double *)source_a, (const double *)source_b, (double *)p_out,
+                       (double *)grad_out);
+  else
+    hipLaunchKernelGGL(k_darcy_macro<float>, ew_grid(N), dim3(256), 0, st, N, (float)m3om, (const float *)omega_field,
+                       (const float *)f, (const float *)source_a, (const float *)source_b, (float *)p_out,
+                       (float *)grad_out);
+  return check_launch("xb_darcy_macro");
+}
+
+int xb_darcy_gradient(int32_t nx, int32_t ny, int32_t dtype, const void *q, double scale, const xb_bc_table *bc,
+                      const void *base, double a, double b, void *out, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || q == nullptr || out == nullptr) return fail(XB_EINVAL, "xb_darcy_gradient: bad arguments");
+  if (int r = check_bc(bc, true)) return r;
+  const int64_t N = (int64_t)nx * ny;
+  const double m3s = -3 * scale;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_darcy_gradient<double>, ew_grid(N), dim3(256), 0, st, nx, ny, (const double *)q, m3s,
+                       bcdev<double>(bc), (const double *)base, a, b, (double *)out);
+  else
+    hipLaunchKernelGGL(k_darcy_gradient<float>, ew_grid(N), dim3(256), 0, st, nx, ny, (const float *)q, (float)m3s,
+                       bcdev<float>(bc), (const float *)base, (float)a, (float)b, (float *)out);
+  return check_launch("xb_darcy_gradient");
+}
+
+int xb_lincomb(int64_t n, int32_t dtype, double a, const void *x, double b, const void *y, void *out, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (n < 0 || x == nullptr || out == nullptr) return fail(XB_EINVAL, "xb_lincomb: bad arguments");
+  if (n == 0) return XB_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_lincomb<double>, ew_grid(n), dim3(256), 0, st, n, a, (const double *)x, b, (const double *)y,
+                       (double *)out);
+  else
+    hipLaunchKernelGGL(k_lincomb<float>, ew_grid(n), dim3(256), 0, st, n, (float)a, (const float *)x, (float)b,
+                       (const float *)y, (float *)out);
+  return check_launch("xb_lincomb");
+}
+
+int xb_sqdiff_partials(int64_t n, int32_t dtype, const void *a, const void *b, double *partials, int32_t max_partials,
+                       int32_t *count, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (n <= 0 || a == nullptr || partials == nullptr || max_partials <= 0 || count == nullptr)
+    return fail(XB_EINVAL, "xb_sqdiff_partials: bad arguments");
+  int64_t blocks = (n + 255) / 256;
+  blocks = std::min<int64_t>(blocks, max_partials);
+  blocks = std::min<int64_t>(blocks, 2048);
+  *count = (int32_t)blocks;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_sqdiff<double>, dim3((unsigned)blocks), dim3(256), 0, st, n, (const double *)a,
+                       (const double *)b, partials);
+  else
+    hipLaunchKernelGGL(k_sqdiff<float>, dim3((unsigned)blocks), dim3(256), 0, st, n, (const float *)a,
+                       (const float *)b, partials);
+  return check_launch("xb_sqdiff_partials");
+}
+
+int xb_norm_finalize(const double *partials, int32_t n, double *out, void *stream) {
+  if (partials == nullptr || out == nullptr || n <= 0) return fail(XB_EINVAL, "xb_norm_finalize: bad arguments");
+  hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(WAVE), 0, static_cast<hipStream_t>(stream), partials, n, out);
+  return check_launch("xb_norm_finalize");
+}
+
+} // extern "C"

@@ -1,0 +1,114 @@
+"""ctypes binding of ``libxlb_biot.so`` (declared in ``include/xlb_biot.h``).
+
+torch is imported first so that its HIP runtime (soname ``libamdhip64.so.7``) is
+the one the kernel library resolves against: one runtime per process.  There is
+no CPU fallback: a missing library or a missing GPU raises at first use.
+"""
+import ctypes
+import os
+
+import torch
+
+XB_F64, XB_F32 = 0, 1
+XB_MODE_STEP, XB_MODE_SMOOTH, XB_MODE_RESIDUAL = 0, 1, 2
+XB_RULE_ZERO = 100
+XB_EINVAL, XB_EUNSUPPORTED, XB_EHIP, XB_ENONFINITE = -1, -2, -3, -4
+
+LIB_PATH = os.path.normpath(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "lib", "libxlb_biot.so"))
+
+c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+
+class BCTable(ctypes.Structure):
+    _fields_ = [("nslots", c_int32), ("cell_slot", c_void_p), ("code_pc", c_void_p), ("code_ps", c_void_p),
+                ("const_pc", c_void_p), ("const_ps", c_void_p)]
+
+
+class ElastArgs(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("dtype", c_int32), ("mode", c_int32),
+                ("lam", c_double), ("mu", c_double), ("gamma", c_double),
+                ("f_in", c_void_p), ("f_out", c_void_p), ("fpc_out", c_void_p),
+                ("force_a", c_void_p), ("force_b", c_void_p), ("resid", c_void_p),
+                ("ix0", c_int32), ("ix1", c_int32), ("iy0", c_int32), ("iy1", c_int32),
+                ("bc", ctypes.POINTER(BCTable)), ("coarse_out", c_void_p), ("norm_partials", c_void_p)]
+
+
+class DarcyArgs(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("dtype", c_int32), ("pad_", c_int32),
+                ("omega", c_double), ("omega_field", c_void_p),
+                ("f_in", c_void_p), ("f_out", c_void_p), ("fpc_out", c_void_p),
+                ("source_a", c_void_p), ("source_b", c_void_p), ("bc", ctypes.POINTER(BCTable))]
+
+
+# name -> (restype, argtypes); every exported symbol of include/xlb_biot.h
+SIGNATURES = {
+    "xb_abi_version": (c_int32, []),
+    "xb_last_error": (ctypes.c_char_p, []),
+    "xb_elast_step": (c_int32, [ctypes.POINTER(ElastArgs), c_void_p]),
+    "xb_elast_partials_count": (c_int64, [ctypes.POINTER(ElastArgs)]),
+    "xb_elast_prolong_add": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_int32,
+                                       c_int32, c_int32, c_int32, c_int32, c_void_p]),
+    "xb_elast_macro": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p]),
+    "xb_elast_divergence": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
+                                      c_void_p, c_void_p]),
+    "xb_elast_equilibrium": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p]),
+    "xb_darcy_step": (c_int32, [ctypes.POINTER(DarcyArgs), c_void_p]),
+    "xb_darcy_macro": (c_int32, [c_int32, c_int32, c_int32, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p]),
+    "xb_darcy_gradient": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_double, ctypes.POINTER(BCTable),
+                                    c_void_p, c_double, c_double, c_void_p, c_void_p]),
+    "xb_lincomb": (c_int32, [c_int64, c_int32, c_double, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "xb_sqdiff_partials": (c_int32, [c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
+                                     ctypes.POINTER(c_int32), c_void_p]),
+    "xb_norm_finalize": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the kernel library (no GPU needed to load it)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"HIP kernel library not built: {path} (run __graft_entry__.build())")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def lib():
+    """The library, on a machine with a ROCm GPU (raises otherwise)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("xlb-biot kernels need a ROCm GPU; none is visible (there is no CPU fallback)")
+    return load_library()
+
+
+def check(status, what):
+    if status != 0:
+        msg = load_library().xb_last_error()
+        msg = msg.decode() if msg else ""
+        exc = ValueError if status in (XB_EINVAL, XB_EUNSUPPORTED) else RuntimeError
+        raise exc(f"{what} failed ({status}): {msg}")
+
+
+def ptr(t):
+    return None if t is None else c_void_p(t.data_ptr())
+
+
+def stream():
+    return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def dtype_code(dtype):
+    if dtype == torch.float64:
+        return XB_F64
+    if dtype == torch.float32:
+        return XB_F32
+    raise ValueError(f"unsupported dtype {dtype}")

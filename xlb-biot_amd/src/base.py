@@ -1,0 +1,254 @@
+"""Shared host machinery of the solvers: precision policy, field layout, LBMBase.
+
+Field layout.  Populations and fields live on the GPU as contiguous
+structure-of-arrays tensors ``(c, nx, ny)`` (y fastest).  What the API hands out
+and accepts are the reference-shaped ``(nx, ny, c)`` views of those buffers
+(``soa.permute(1, 2, 0)``), so user code indexes ``f[x, y, i]`` as with the
+reference's arrays, and passing a returned view back costs no copy.
+
+``LBMBase`` keeps the parts of the reference ``src/base.py`` that the Darcy
+solver uses (construction :44-146, boundary data :351-372,
+``distributed_array_init`` :391-411, grid mask :413-449, bounding box
+:451-485, precisions :487-510, ``assign_fields_sharded`` :528-559).
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .boundary_conditions import configure_bcs, create_grid_mask, build_rule_table
+
+_NP2TORCH = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+             np.dtype(np.float16): torch.float16, np.dtype(np.bool_): torch.bool,
+             np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64}
+
+
+def torch_dtype(dt):
+    if isinstance(dt, torch.dtype):
+        return dt
+    return _NP2TORCH[np.dtype(dt)]
+
+
+class PrecisionPolicy:
+    """``"compute/storage"`` precision (stand-in for the reference's ``jmp.Policy``).
+
+    ``compute_dtype`` / ``output_dtype`` are NumPy dtypes, so driver hooks that pass
+    them to NumPy keep working; ``torch_compute`` / ``torch_output`` are the torch ones.
+    This backend computes and stores in one precision: ``"f64/f64"`` or ``"f32/f32"``.
+    """
+    TABLE = {"f64/f64": (np.float64, np.float64), "f32/f32": (np.float32, np.float32),
+             "f32/f16": (np.float32, np.float16), "f16/f16": (np.float16, np.float16),
+             "f64/f32": (np.float64, np.float32), "f64/f16": (np.float64, np.float16)}
+
+    def __init__(self, precision):
+        # elasticity.py:606-629: unknown strings fall back to f32/f32
+        cdt, odt = self.TABLE.get(precision, (np.float32, np.float32))
+        if cdt is not odt:
+            raise NotImplementedError(
+                f"precision {precision!r}: mixed or half precision storage is not implemented on this backend "
+                "(use 'f64/f64' or 'f32/f32')")
+        self.compute_dtype = cdt
+        self.param_dtype = cdt
+        self.output_dtype = odt
+        self.torch_compute = torch_dtype(cdt)
+        self.torch_output = torch_dtype(odt)
+
+    def cast_to_compute(self, x):
+        return None if x is None else x.to(self.torch_compute)
+
+    def cast_to_output(self, x):
+        return None if x is None else x.to(self.torch_output)
+
+
+def default_device():
+    return torch.device("cuda", torch.cuda.current_device()) if torch.cuda.is_available() else torch.device("cpu")
+
+
+def as_soa(x, nx, ny, c, dtype, device):
+    """``(nx, ny, c)`` array-like -> contiguous ``(c, nx, ny)`` tensor on ``device``.
+
+    A view returned by this package (the permuted SoA buffer) is passed through
+    without a copy."""
+    if isinstance(x, torch.Tensor):
+        t = x
+        if t.dim() == 2:
+            t = t.unsqueeze(-1)
+        if tuple(t.shape) == (nx, ny, c) and t.device == device and t.dtype == dtype:
+            b = t.permute(2, 0, 1)
+            if b.is_contiguous():
+                return b
+        t = t.to(device=device, dtype=dtype)
+    else:
+        t = torch.as_tensor(np.asarray(x), dtype=dtype, device=device)
+        if t.dim() == 2:
+            t = t.unsqueeze(-1)
+    if tuple(t.shape) != (nx, ny, c):
+        t = t.expand(nx, ny, c)
+    return t.permute(2, 0, 1).contiguous()
+
+
+def view(soa):
+    """Reference-shaped ``(nx, ny, c)`` view of a ``(c, nx, ny)`` buffer."""
+    return soa.permute(1, 2, 0)
+
+
+class NormWorkspace:
+    """FP64 partial sums + the finalised (norm, non-finite flag) pair."""
+
+    def __init__(self, n, device):
+        self.partials = torch.zeros(max(int(n), 1), dtype=torch.float64, device=device)
+        self.count = nat.c_int32(0)
+
+    def finalize(self, n):
+        out = torch.empty(2, dtype=torch.float64, device=self.partials.device)
+        nat.check(nat.lib().xb_norm_finalize(nat.ptr(self.partials), int(n), nat.ptr(out), nat.stream()),
+                  "xb_norm_finalize")
+        return out
+
+
+class LBMBase(object):
+    """Base of the D2Q9 Darcy solver (reference src/base.py)."""
+
+    def __init__(self, **kwargs):
+        self.omega = kwargs.get("omega")
+        self.nz = kwargs.get("nz")
+        self.nx = kwargs.get("nx")
+        self.ny = kwargs.get("ny")
+        self.precision = kwargs.get("precision")
+        self.precisionPolicy = PrecisionPolicy(self.precision)
+        self.lattice = kwargs.get("lattice")
+        self.checkpointRate = kwargs.get("checkpoint_rate", 0)
+        self.checkpointDir = kwargs.get("checkpoint_dir", "./checkpoints")
+        self.downsamplingFactor = kwargs.get("downsampling_factor", 1)
+        self.printInfoRate = kwargs.get("print_info_rate", 100)
+        self.ioRate = kwargs.get("io_rate", 0)
+        self.returnFpost = kwargs.get("return_fpost", False)
+        self.computeMLUPS = kwargs.get("compute_MLUPS", False)
+        self.restore_checkpoint = kwargs.get("restore_checkpoint", False)
+        self.nDevices = 1
+        self.backend = "rocm"
+        self.device = default_device()
+        self.sharding = None
+        if self.computeMLUPS:
+            self.restore_checkpoint = False
+            self.ioRate = 0
+            self.checkpointRate = 0
+            self.printInfoRate = 0
+        self.c = self.lattice.c
+        self.q = self.lattice.q
+        self.w = self.lattice.w
+        self.dim = self.lattice.d
+        if None in {self.nx, self.ny, self.nz}:
+            raise ValueError("nx, ny, and nz must be provided. For 2D examples, nz must be set to 0.")
+        self.gridInfo = {"nx": self.nx, "ny": self.ny, "nz": self.nz, "dim": self.lattice.d, "lattice": self.lattice}
+        self.boundingBoxIndices = self.bounding_box_indices()
+        self._create_boundary_data()
+        self.force = self.get_force()
+
+    # ---- validated attributes (base.py:148-320)
+    @property
+    def lattice(self):
+        return self._lattice
+
+    @lattice.setter
+    def lattice(self, value):
+        if value is None:
+            raise ValueError("Lattice type must be provided.")
+        if self.nz == 0 and value.name not in ["D2Q9"]:
+            raise ValueError("For 2D simulations, lattice type must be LatticeD2Q9.")
+        if self.nz != 0:
+            raise ValueError("3D simulations are not supported on this path.")
+        self._lattice = value
+
+    @property
+    def omega(self):
+        return self._omega
+
+    @omega.setter
+    def omega(self, value):
+        if value is None:
+            raise ValueError("omega must be provided")
+        if not isinstance(value, float):
+            raise TypeError("omega must be a float")
+        self._omega = value
+
+    def _int_attr(name):
+        def getter(self):
+            return getattr(self, "_" + name)
+
+        def setter(self, value):
+            if value is None:
+                raise ValueError(f"{name} must be provided")
+            if not isinstance(value, int) or isinstance(value, bool):
+                raise TypeError(f"{name} must be an integer")
+            setattr(self, "_" + name, value)
+        return property(getter, setter)
+
+    nx = _int_attr("nx")
+    ny = _int_attr("ny")
+    nz = _int_attr("nz")
+
+    @property
+    def precision(self):
+        return self._precision
+
+    @precision.setter
+    def precision(self, value):
+        if not isinstance(value, str):
+            raise TypeError("precision must be a string")
+        self._precision = value
+
+    def set_precisions(self, precision):
+        return PrecisionPolicy.TABLE.get(precision, (np.float32, np.float32))
+
+    # ---- geometry and boundary data
+    def bounding_box_indices(self):
+        nx, ny = self.nx, self.ny
+        return {"bottom": np.array([[i, 0] for i in range(nx)], dtype=int),
+                "top": np.array([[i, ny - 1] for i in range(nx)], dtype=int),
+                "left": np.array([[0, i] for i in range(ny)], dtype=int),
+                "right": np.array([[nx - 1, i] for i in range(ny)], dtype=int)}
+
+    def create_grid_mask(self, solid_halo_voxels):
+        return create_grid_mask(self.nx, self.ny, self.lattice, solid_halo_voxels)
+
+    def _create_boundary_data(self):
+        self.BCs = []
+        self.set_boundary_conditions()
+        start = time.time()
+        configure_bcs(self.BCs, self.nx, self.ny, self.lattice)
+        self._bc_table = build_rule_table(self.BCs, self.nx, self.ny, self.q, self.precisionPolicy.torch_compute,
+                                          self.device)
+        self._bc_build_time = time.time() - start
+
+    def distributed_array_init(self, shape, type, init_val=0, sharding=None):
+        """``jnp.full(shape, init_val, dtype=type)`` on the solver's device (base.py:391-411)."""
+        dt = torch_dtype(type)
+        if isinstance(init_val, torch.Tensor):
+            v = init_val.to(device=self.device, dtype=dt)
+        else:
+            v = torch.as_tensor(np.asarray(init_val), dtype=dt, device=self.device)
+        return v.expand(shape).clone() if tuple(v.shape) != tuple(shape) else v.clone()
+
+    # ---- hooks
+    def set_boundary_conditions(self):
+        pass
+
+    def get_force(self):
+        pass
+
+    def initialize_macroscopic_fields(self):
+        print("WARNING: Default initial conditions assumed: density = 1, velocity = 0")
+        print("         To set explicit initial density and velocity, use self.initialize_macroscopic_fields.")
+        return None, None
+
+    def output_data(self, **kwargs):
+        pass
+
+    def show_simulation_parameters(self):
+        names = ["omega", "nx", "ny", "nz", "dim", "precision", "lattice", "ioRate", "computeMLUPS", "backend",
+                 "nDevices"]
+        print(f"**** Simulation Parameters for {self.__class__.__name__} ****")
+        for n in names:
+            print(f"{n:>30} | {getattr(self, n, 'Attribute not set')}")

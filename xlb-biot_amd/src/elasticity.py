@@ -1,0 +1,617 @@
+"""D2Q8 lattice-Boltzmann solver for static linear elasticity, with the multigrid V-cycle.
+
+Drop-in for the reference ``src/elasticity.py`` (``LBMElasticity`` :30-1501 and the
+elastic boundary conditions :1503-1750): same class names, keyword arguments,
+validation errors, hooks (``set_boundary_conditions``, ``get_force``,
+``initialize_macroscopic_fields``, ``output_data``) and ``step``/``run``
+signatures and return values.  Every step runs on the GPU through the C ABI of
+``include/xlb_biot.h``; arrays are torch tensors shaped like the reference's.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .base import NormWorkspace, PrecisionPolicy, as_soa, default_device, torch_dtype, view
+from .boundary_conditions import BoundaryCondition, build_rule_table, configure_bcs, create_grid_mask
+
+
+class LBMElasticity(object):
+    """Moment-based D2Q8 LBM for static linear elasticity (reference elasticity.py:30-148)."""
+
+    def __init__(self, **kwargs):
+        self.lamda = kwargs.get("lambda")
+        self.mu = kwargs.get("mu")
+        self.nx = kwargs.get("nx")
+        self.ny = kwargs.get("ny")
+        self.nz = kwargs.get("nz")
+        self.mg_BCs = kwargs.get("MG_BCs", {})
+        self.levels = kwargs.get("MG_levels", 1)
+        self.gamma = kwargs.get("MG_relaxation", 0.75)
+        self.k_smooth = kwargs.get("MG_n_smoothing", 2)
+        self.precision = kwargs.get("precision")
+        self.precisionPolicy = PrecisionPolicy(self.precision)
+        self.lattice = kwargs.get("lattice")
+        self.downsamplingFactor = kwargs.get("downsampling_factor", 1)
+        self.printInfoRate = kwargs.get("print_info_rate", 100)
+        self.ioRate = kwargs.get("io_rate", 0)
+        self.returnFpost = kwargs.get("return_fpost", False)
+        self.computeMLUPS = kwargs.get("compute_MLUPS", False)
+        self.nDevices = 1
+        self.backend = "rocm"
+        self.device = default_device()
+        self.sharding = None
+        if self.computeMLUPS:
+            self.ioRate = 0
+            self.printInfoRate = 0
+        self.c = self.lattice.c
+        self.q = self.lattice.q
+        self.w = self.lattice.w
+        self.dim = self.lattice.d
+        self._create_transfer_matrices()
+        self.gridInfo = {"nx": self.nx, "ny": self.ny, "nz": self.nz, "dim": self.lattice.d, "lattice": self.lattice}
+        self.boundingBoxIndices = self.bounding_box_indices()
+        self._create_boundary_data()
+        force = self.get_force()
+        self._force_soa = None if force is None else as_soa(force, self.nx, self.ny, 2, self._tdt, self.device)
+        self.force = None if self._force_soa is None else view(self._force_soa)
+        self._norm_ws = {}
+        self.mlups = None
+
+    @property
+    def _tdt(self):
+        return self.precisionPolicy.torch_compute
+
+    # ---- validated attributes (elasticity.py:150-354)
+    @property
+    def lattice(self):
+        return self._lattice
+
+    @lattice.setter
+    def lattice(self, value):
+        if value is None:
+            raise ValueError("Lattice type must be provided.")
+        if self.nz == 0 and value.name not in ["D2Q8"]:
+            raise ValueError("For 2D simulations, lattice type must be LatticeD2Q8.")
+        if self.nz != 0:
+            raise ValueError("3D simulations are not supported.")
+        self._lattice = value
+
+    @property
+    def lamda(self):
+        return self._lamda
+
+    @lamda.setter
+    def lamda(self, value):
+        if value is None:
+            raise ValueError("lambda must be provided")
+        if not isinstance(value, float):
+            raise TypeError("lambda must be a float")
+        self._lamda = value
+
+    @property
+    def mu(self):
+        return self._mu
+
+    @mu.setter
+    def mu(self, value):
+        if value is None:
+            raise ValueError("mu must be provided")
+        if not isinstance(value, float):
+            raise TypeError("mu must be a float")
+        self._mu = value
+
+    @property
+    def levels(self):
+        return self._levels
+
+    @levels.setter
+    def levels(self, value):
+        if value is None:
+            raise ValueError("levels must be provided")
+        if not isinstance(value, int) or value <= 0:
+            raise TypeError("levels must be a positive int")
+        den = 2 ** (value - 1)
+        if (self.nx - int(self.mg_BCs["left"]) - int(self.mg_BCs["right"])) % den != 0 \
+                or (self.ny - int(self.mg_BCs["bottom"]) - int(self.mg_BCs["top"])) % den != 0 \
+                or (self.nz - int(self.mg_BCs["front"]) - int(self.mg_BCs["back"])) % den != 0:
+            raise ValueError("nx,ny,nz (-BCs) must be divisible by 2**(levels-1)")
+        self._levels = value
+
+    @property
+    def gamma(self):
+        return self._gamma
+
+    @gamma.setter
+    def gamma(self, value):
+        if value is None:
+            raise ValueError("gamma must be provided")
+        if not isinstance(value, float):
+            raise TypeError("gamma must be a float")
+        if value < 0.0 or value > 1.0:
+            raise ValueError("gamma must be between 0 and 1")
+        self._gamma = value
+
+    @property
+    def k_smooth(self):
+        return self._k_smooth
+
+    @k_smooth.setter
+    def k_smooth(self, value):
+        if value is None:
+            raise ValueError("k_smooth must be provided")
+        if not isinstance(value, int):
+            raise TypeError("k_smooth must be an integer")
+        if value < 1:
+            raise ValueError("k_smooth must be positive")
+        self._k_smooth = value
+
+    @property
+    def mg_BCs(self):
+        return self._mg_BCs
+
+    @mg_BCs.setter
+    def mg_BCs(self, value):
+        if not isinstance(value, dict):
+            raise TypeError("mg_BCs must be a dictionary")
+        for key, bc in value.items():
+            if key not in ["left", "right", "bottom", "top", "front", "back"]:
+                raise ValueError(f"Invalid key '{key}' in mg_BCs. Valid keys are 'left', 'right', 'bottom', 'top', "
+                                 "'front', 'back'.")
+            if not isinstance(bc, bool):
+                raise TypeError("All values in mg_BCs must be boolean values")
+        self._mg_BCs = {"left": False, "right": False, "bottom": False, "top": False, "front": False, "back": False}
+        self._mg_BCs.update(value)
+
+    def _int_attr(name):
+        def getter(self):
+            return getattr(self, "_" + name)
+
+        def setter(self, value):
+            if value is None:
+                raise ValueError(f"{name} must be provided")
+            if not isinstance(value, int):
+                raise TypeError(f"{name} must be an integer")
+            setattr(self, "_" + name, value)
+        return property(getter, setter)
+
+    nx = _int_attr("nx")
+    ny = _int_attr("ny")
+    nz = _int_attr("nz")
+
+    def _typed(name, typ, msg):
+        def getter(self):
+            return getattr(self, "_" + name)
+
+        def setter(self, value):
+            if not isinstance(value, typ):
+                raise TypeError(msg)
+            setattr(self, "_" + name, value)
+        return property(getter, setter)
+
+    precision = _typed("precision", str, "precision must be a string")
+    downsamplingFactor = _typed("downsamplingFactor", int, "downsamplingFactor must be an integer")
+    printInfoRate = _typed("printInfoRate", int, "printInfoRate must be an integer")
+    ioRate = _typed("ioRate", int, "ioRate must be an integer")
+    returnFpost = _typed("returnFpost", bool, "returnFpost must be a boolean")
+    computeMLUPS = _typed("computeMLUPS", bool, "computeMLUPS must be a boolean")
+    nDevices = _typed("nDevices", int, "nDevices must be an integer")
+
+    def show_simulation_parameters(self):
+        names = ["lamda", "mu", "nx", "ny", "nz", "dim", "levels", "gamma", "k_smooth", "precision", "lattice",
+                 "downsamplingFactor", "printInfoRate", "ioRate", "computeMLUPS", "backend", "nDevices"]
+        print(f"**** Simulation Parameters for {self.__class__.__name__} ****")
+        for n in names:
+            print(f"{n:>30} | {getattr(self, n, 'Attribute not set')}")
+
+    # ---- geometry (elasticity.py:501-604)
+    def grid_size(self, level=0):
+        if level == 0:
+            return self.nx, self.ny
+        nx_bc = int(self.mg_BCs["left"]) + int(self.mg_BCs["right"])
+        ny_bc = int(self.mg_BCs["bottom"]) + int(self.mg_BCs["top"])
+        return (self.nx - nx_bc) // 2 ** level + nx_bc, (self.ny - ny_bc) // 2 ** level + ny_bc
+
+    def grid_info(self, level=0):
+        nx, ny = self.grid_size(level)
+        return {"nx": nx, "ny": ny, "nz": self.nz, "dim": self.lattice.d, "lattice": self.lattice}
+
+    def bounding_box_indices(self, level=0):
+        nx, ny = self.grid_size(level)
+        return {"bottom": np.array([[i, 0] for i in range(nx)], dtype=int),
+                "top": np.array([[i, ny - 1] for i in range(nx)], dtype=int),
+                "left": np.array([[0, i] for i in range(ny)], dtype=int),
+                "right": np.array([[nx - 1, i] for i in range(ny)], dtype=int)}
+
+    def interior_box(self, level=0):
+        """Cells outside the multigrid BC layers: ``(x0, x1, y0, y1)``."""
+        nx, ny = self.grid_size(level)
+        return (int(self.mg_BCs["left"]), nx - int(self.mg_BCs["right"]),
+                int(self.mg_BCs["bottom"]), ny - int(self.mg_BCs["top"]))
+
+    def set_precisions(self, precision):
+        return PrecisionPolicy.TABLE.get(precision, (np.float32, np.float32))
+
+    def create_grid_mask(self, solid_halo_voxels, level=0):
+        nx, ny = self.grid_size(level)
+        return create_grid_mask(nx, ny, self.lattice, solid_halo_voxels)
+
+    def _create_boundary_data(self):
+        # elasticity.py:396-436; then fold each level's BCs into a kernel rule table
+        self.BCs = []
+        self.BCs_MG = {level: [] for level in range(1, self.levels)}
+        self.set_boundary_conditions()
+        start = time.time()
+        self._tables = {}
+        for level in range(self.levels):
+            bcs = self.BCs if level == 0 else self.BCs_MG[level]
+            nx, ny = self.grid_size(level)
+            configure_bcs(bcs, nx, ny, self.lattice)
+            self._tables[level] = build_rule_table(bcs, nx, ny, self.q, self._tdt, self.device)
+        self._bc_build_time = time.time() - start
+
+    def distributed_array_init(self, shape, type, init_val=0, sharding=None):
+        dt = torch_dtype(type)
+        if isinstance(init_val, torch.Tensor):
+            v = init_val.to(device=self.device, dtype=dt)
+        else:
+            v = torch.as_tensor(np.asarray(init_val), dtype=dt, device=self.device)
+        return v.expand(shape).clone() if tuple(v.shape) != tuple(shape) else v.clone()
+
+    # ---- hooks (elasticity.py:631-645, 1458-1501)
+    def initialize_macroscopic_fields(self):
+        print("WARNING: Default initial conditions assumed: displacement = 0, stress = 0")
+        print("         To set explicit initial displacement and stress, use self.initialize_macroscopic_fields.")
+        return None, None
+
+    def output_data(self, **kwargs):
+        pass
+
+    def set_boundary_conditions(self):
+        pass
+
+    def get_force(self):
+        pass
+
+    # ---- constants (elasticity.py:805-848); host copies, the kernels form the same entries
+    def _create_transfer_matrices(self):
+        g = 1.0 / (12.0 * (self.lamda + self.mu) - 4.0)
+        if abs(g) > 100:
+            print("Warning: scaling value is too large. Reset last moment to default. Boundary conditions may fail.")
+            g = 0.0
+        self.f_to_m = np.array([[1, 0, -1, 0, 1, -1, -1, 1],
+                                [0, 1, 0, -1, 1, 1, -1, -1],
+                                [1, 1, 1, 1, 2, 2, 2, 2],
+                                [1, -1, 1, -1, 0, 0, 0, 0],
+                                [0, 0, 0, 0, 1, -1, 1, -1]], dtype=np.float64).T
+        h, n = 0.25 + 0.5 * g, -0.25 * g
+        self.m_to_f = np.array([[0.5, 0, h, 0.25, 0, -0.5, 0],
+                                [0, 0.5, h, -0.25, 0, 0, -0.5],
+                                [-0.5, 0, h, 0.25, 0, 0.5, 0],
+                                [0, -0.5, h, -0.25, 0, 0, 0.5],
+                                [0, 0, n, 0, 0.25, 0.25, 0.25],
+                                [0, 0, n, 0, -0.25, -0.25, 0.25],
+                                [0, 0, n, 0, 0.25, -0.25, -0.25],
+                                [0, 0, n, 0, -0.25, 0.25, -0.25]], dtype=np.float64).T
+
+    # ---- conversions
+    def _soa(self, x, c, level=0):
+        if x is None:
+            return None
+        nx, ny = self.grid_size(level)
+        return as_soa(x, nx, ny, c, self._tdt, self.device)
+
+    def _new(self, c, level=0):
+        nx, ny = self.grid_size(level)
+        return torch.empty((c, nx, ny), dtype=self._tdt, device=self.device)
+
+    # ---- kernels
+    def _launch(self, f_in, f_out, level, mode, ext=None, resid=None, fpc=None, coarse=None, partials=None,
+                use_force=None):
+        nx, ny = f_in.shape[1], f_in.shape[2]
+        x0, x1, y0, y1 = self.interior_box(level)
+        table = self._tables.get(level)
+        if use_force is None:
+            use_force = level == 0
+        fa = self._force_soa if use_force else None
+        a = nat.ElastArgs(nx, ny, nat.dtype_code(f_in.dtype), mode, self.lamda, self.mu, self.gamma,
+                          nat.ptr(f_in), nat.ptr(f_out), nat.ptr(fpc), nat.ptr(fa), nat.ptr(ext), nat.ptr(resid),
+                          x0, x1, y0, y1, table.ref() if table is not None else None,
+                          nat.ptr(coarse), nat.ptr(partials))
+        return a
+
+    def _run_step(self, f, ext=None, level=0, fpc=False):
+        """One ``_step`` on SoA buffers: returns ``(f_new, f_postcollision | None)``."""
+        out = torch.empty_like(f)
+        pc = torch.empty_like(f) if fpc else None
+        a = self._launch(f, out, level, nat.XB_MODE_STEP, ext=ext, fpc=pc)
+        nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step")
+        return out, pc
+
+    def _smooth(self, f, level, resid, ext):
+        out = torch.empty_like(f)
+        a = self._launch(f, out, level, nat.XB_MODE_SMOOTH, ext=ext, resid=resid)
+        nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step(smooth)")
+        return out
+
+    def _residual(self, f, level, resid, ext):
+        """Restricted residual (coarse interior, SoA) and the workspace holding its
+        squared-norm partials."""
+        x0, x1, y0, y1 = self.interior_box(level)
+        coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=f.dtype, device=f.device)
+        lib = nat.lib()
+        a = self._launch(f, None, level, nat.XB_MODE_RESIDUAL, ext=ext, resid=resid, coarse=coarse)
+        n = int(lib.xb_elast_partials_count(a))
+        ws = self._norm_ws.get(level)
+        if ws is None or ws.partials.numel() < n:
+            ws = self._norm_ws[level] = NormWorkspace(n, f.device)
+        a.norm_partials = nat.ptr(ws.partials)
+        nat.check(lib.xb_elast_step(a, nat.stream()), "xb_elast_step(residual)")
+        return coarse, ws, n
+
+    def _prolong_add(self, f, f_c, level):
+        nx, ny = self.grid_size(level)
+        cnx, cny = self.grid_size(level + 1)
+        x0, x1, y0, y1 = self.interior_box(level)
+        nat.check(nat.lib().xb_elast_prolong_add(nx, ny, nat.dtype_code(f.dtype), nat.ptr(f), nat.ptr(f_c), cnx,
+                                                 cny, x0, x1, y0, y1, nat.stream()), "xb_elast_prolong_add")
+
+    def _add(self, a, b):
+        out = torch.empty_like(a)
+        nat.check(nat.lib().xb_lincomb(a.numel(), nat.dtype_code(a.dtype), 1.0, nat.ptr(a), 1.0, nat.ptr(b),
+                                       nat.ptr(out), nat.stream()), "xb_lincomb")
+        return out
+
+    def _ext_soa(self, external_force, nonlinear_force, f, level):
+        """``external_force (+ nonlinear_force(f))`` as a SoA buffer (elasticity.py:1259-1265)."""
+        if not callable(nonlinear_force):
+            return external_force
+        fast = getattr(nonlinear_force, "soa_add", None)
+        if fast is not None:
+            return fast(f, external_force)
+        nl = self._soa(nonlinear_force(view(f)), 2, level)
+        return nl if external_force is None else self._add(external_force, nl)
+
+    # ---- public API (elasticity.py:908-1344)
+    def equilibrium(self, eta, sigma, cast_output=True, external_force=None):
+        eta_s = self._soa(eta, 2)
+        sig_s = self._soa(sigma, 3)
+        ext = self._soa(external_force, 2)
+        f = self._new(8)
+        nat.check(nat.lib().xb_elast_equilibrium(self.nx, self.ny, nat.dtype_code(self._tdt), self.lamda, self.mu,
+                                                 nat.ptr(eta_s), nat.ptr(sig_s), nat.ptr(self._force_soa),
+                                                 nat.ptr(ext), nat.ptr(f), nat.stream()), "xb_elast_equilibrium")
+        return view(f)
+
+    def update_macroscopic(self, f, external_force=None):
+        fs = self._soa(f, 8)
+        ext = self._soa(external_force, 2)
+        eta, sig = self._new(2), self._new(3)
+        nat.check(nat.lib().xb_elast_macro(self.nx, self.ny, nat.dtype_code(self._tdt), self.lamda, self.mu,
+                                           nat.ptr(fs), nat.ptr(self._force_soa), nat.ptr(ext), nat.ptr(eta),
+                                           nat.ptr(sig), nat.stream()), "xb_elast_macro")
+        return view(eta), view(sig)
+
+    def _divergence_soa(self, f, f_sub=None):
+        nx, ny = f.shape[1], f.shape[2]
+        out = torch.empty((1, nx, ny), dtype=f.dtype, device=f.device)
+        nat.check(nat.lib().xb_elast_divergence(nx, ny, nat.dtype_code(f.dtype), self.lamda, self.mu, nat.ptr(f),
+                                                nat.ptr(f_sub), nat.ptr(out), nat.stream()), "xb_elast_divergence")
+        return out
+
+    def update_divergence(self, f):
+        if isinstance(f, torch.Tensor) and f.dim() == 3 and f.shape[-1] == 8:
+            nx, ny = f.shape[0], f.shape[1]
+            return view(self._divergence_soa(as_soa(f, nx, ny, 8, self._tdt, self.device)))
+        raise ValueError("update_divergence expects populations shaped (nx, ny, 8)")
+
+    def collision(self, f, external_force, use_force=True):
+        """Post-collision populations without boundary rules (elasticity.py:959-1000)."""
+        fs = self._soa(f, 8)
+        out, pc = torch.empty_like(fs), torch.empty_like(fs)
+        a = self._launch(fs, out, 0, nat.XB_MODE_STEP, ext=self._soa(external_force, 2), fpc=pc,
+                         use_force=use_force)
+        a.bc = None
+        nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step(collision)")
+        return view(pc)
+
+    def step(self, f_poststreaming, timestep, return_fpost=False, external_force=None, nonlinear_force=None):
+        """elasticity.py:1128-1167."""
+        if self.levels == 1:
+            f = self._soa(f_poststreaming, 8)
+            ext = self._ext_soa(self._soa(external_force, 2), nonlinear_force, f, 0)
+            out, pc = self._run_step(f, ext, 0, return_fpost)
+            return view(out), (view(pc) if pc is not None else None)
+        return self._mg_step(f_poststreaming, timestep, external_force=external_force,
+                             nonlinear_force=nonlinear_force)
+
+    def _step_soa(self, f, external_force=None, nonlinear_force=None):
+        """One ``step`` on SoA buffers: ``(f_new, residual_norm | None)``."""
+        if self.levels == 1:
+            out, _ = self._run_step(f, self._ext_soa(external_force, nonlinear_force, f, 0), 0)
+            return out, None
+        return self._vcycle(f, 0, None, external_force, nonlinear_force)
+
+    def _step(self, f_poststreaming, timestep, return_fpost=False, external_force=None, level=0):
+        """elasticity.py:1169-1219."""
+        f = self._soa(f_poststreaming, 8, level)
+        out, pc = self._run_step(f, self._soa(external_force, 2, level), level, return_fpost)
+        return view(out), (view(pc) if pc is not None else None)
+
+    def _mg_step(self, f_poststreaming, timestep, level=0, residual=None, external_force=None, nonlinear_force=None):
+        """elasticity.py:1221-1312; ``residual`` is shaped like the level's interior."""
+        f = self._soa(f_poststreaming, 8, level)
+        r = None
+        if residual is not None:
+            x0, x1, y0, y1 = self.interior_box(level)
+            r = as_soa(residual, x1 - x0, y1 - y0, 8, self._tdt, self.device)
+        out = self._vcycle(f, level, r, self._soa(external_force, 2, level), nonlinear_force)
+        if level == 0:
+            return view(out[0]), out[1]
+        return view(out)
+
+    def _vcycle(self, f, level, residual, external_force, nonlinear_force):
+        for _ in range(self.k_smooth):
+            ext = self._ext_soa(external_force, nonlinear_force, f, level)
+            f = self._smooth(f, level, residual, ext)
+        norm = None
+        if level < self.levels - 1:
+            ext = self._ext_soa(external_force, nonlinear_force, f, level)
+            res_c, ws, n = self._residual(f, level, residual, ext)
+            if level == 0:
+                norm = ws.finalize(n)[0]
+            nxc, nyc = self.grid_size(level + 1)
+            f_c = torch.zeros((8, nxc, nyc), dtype=f.dtype, device=f.device)
+            f_c = self._vcycle(f_c, level + 1, res_c, None, nonlinear_force)
+            self._prolong_add(f, f_c, level)
+        for _ in range(self.k_smooth):
+            ext = self._ext_soa(external_force, nonlinear_force, f, level)
+            f = self._smooth(f, level, residual, ext)
+        if level == 0:
+            return f, norm
+        return f
+
+    # ---- driver (elasticity.py:647-699, 1346-1456)
+    def assign_fields_sharded(self):
+        eta0, sigma0 = self.initialize_macroscopic_fields()
+        if eta0 is None or sigma0 is None:
+            return view(torch.zeros((8, self.nx, self.ny), dtype=self._tdt, device=self.device))
+        return self.initialize_populations(eta0, sigma0)
+
+    def initialize_populations(self, eta0, sigma0):
+        return self.equilibrium(eta0, sigma0)
+
+    def run(self, t_max):
+        f = self.assign_fields_sharded()
+        start = time.time()
+        for timestep in range(0, t_max + 1):
+            io_flag = self.ioRate > 0 and (timestep % self.ioRate == 0 or timestep == t_max)
+            print_iter_flag = self.printInfoRate > 0 and timestep % self.printInfoRate == 0
+            if io_flag:
+                eta_prev, sigma_prev = (x.cpu() for x in self.update_macroscopic(f))
+            f, fstar = self.step(f, timestep, return_fpost=self.returnFpost)
+            if print_iter_flag:
+                print(f"Timestep {timestep} of {t_max} completed")
+            if io_flag:
+                eta, sigma = (x.cpu() for x in self.update_macroscopic(f))
+                self.handle_io_timestep(timestep, f, fstar, eta, sigma, eta_prev, sigma_prev)
+            if self.computeMLUPS and timestep == 1:
+                torch.cuda.synchronize()
+                start = time.time()
+        if self.computeMLUPS:
+            torch.cuda.synchronize()
+            end = time.time()
+            self.mlups = self.nx * self.ny * t_max / (end - start) / 1e6
+            print(f"Domain: {self.nx} x {self.ny}")
+            print(f"Number of voxels: {self.nx * self.ny}")
+            print(f"MLUPS: {self.mlups}")
+        return f
+
+    def handle_io_timestep(self, timestep, f, fstar, eta, sigma, eta_prev, sigma_prev):
+        self.output_data(timestep=timestep, eta=eta, eta_prev=eta_prev, sigma=sigma, sigma_prev=sigma_prev,
+                         f_poststreaming=f, f_postcollision=fstar)
+
+
+# ---------------------------------------------------------------------------
+# Elastic boundary conditions (elasticity.py:1503-1750).  Construction follows the
+# reference; ``apply_rules`` is the symbolic form of each ``apply``.
+
+class ElasticityBounceBackFullway(BoundaryCondition):
+    """Full-way bounce-back (Dirichlet), post-collision at the wall cells:
+    ``f*_i = f_opp(i) + 6 w_i c_i . eta`` (elasticity.py:1503-1550)."""
+
+    def __init__(self, indices, gridInfo, precision_policy, eta=None):
+        super().__init__(indices, gridInfo, precision_policy)
+        self.name = "ElasticityBounceBackFullway"
+        self.implementationStep = "PostCollision"
+        self.eta = None if eta is None else np.asarray(eta, dtype=np.float64)
+
+    def apply_rules(self, state):
+        opp = np.asarray(self.lattice.opp_indices, dtype=np.int64)
+        state.code[:, :] = opp[None, :] + 1
+        state.const[:, :] = 0.0
+        if self.eta is not None:
+            state.const[:, :] = np.broadcast_to(self.cu_rules(self.eta), state.const.shape)
+
+
+class ElasticityBounceBack(BoundaryCondition):
+    """Halfway bounce-back (Dirichlet), post-streaming (elasticity.py:1552-1646)."""
+
+    def __init__(self, indices, gridInfo, precision_policy, eta=None):
+        super().__init__(indices, gridInfo, precision_policy)
+        self.name = "ElasticityBounceBack"
+        self.implementationStep = "PostStreaming"
+        self.needsExtraConfiguration = True
+        self.isSolid = True
+        self.eta = None if eta is None else np.asarray(eta, dtype=np.float64)
+
+    def configure(self, boundaryMask):
+        nbd_orig = len(self.indices[0])
+        indices_new, array_indices = self._shift_to_fluid(boundaryMask)
+        self.indices = tuple(indices_new.T)
+        nbd_modified = len(self.indices[0])
+        if self.eta is not None:
+            if self.eta.shape == (self.lattice.d,):
+                self.eta = np.tile(self.eta, (nbd_modified, 1))
+            elif nbd_orig != nbd_modified:
+                print("WARNING: Assuming a correct displacement vector to impose at the BC cells!")
+                self.eta = self.eta[array_indices]
+
+    def apply_rules(self, state):
+        state.set_from_fin(self.imissing, self.iknown, +1)
+        if self.eta is not None:
+            cu = np.broadcast_to(self.cu_rules(self.eta), state.const.shape)
+            im = np.asarray(self.imissing, dtype=np.int64)
+            state.add(im, np.take_along_axis(cu, im, 1))
+
+
+class ElasticityAntiBounceBack(BoundaryCondition):
+    """Halfway anti-bounce-back (Neumann / traction), post-streaming
+    (elasticity.py:1648-1750).  Missing and known directions pair in row-major mask order."""
+
+    def __init__(self, indices, gridInfo, precision_policy, T=None):
+        super().__init__(indices, gridInfo, precision_policy)
+        self.name = "ElasticityAntiBounceBack"
+        self.implementationStep = "PostStreaming"
+        self.needsExtraConfiguration = True
+        self.isSolid = True
+        self.T = None if T is None else np.asarray(T, dtype=np.float64)
+
+    def configure(self, boundaryMask):
+        nbd_orig = len(self.indices[0])
+        indices_new, array_indices = self._shift_to_fluid(boundaryMask)
+        self.indices = tuple(indices_new.T)
+        nbd_modified = len(self.indices[0])
+        if self.T is not None:
+            if self.T.shape == (self.lattice.d,):
+                self.T = np.tile(self.T, (nbd_modified, 1))
+            elif nbd_orig != nbd_modified:
+                print("WARNING: Assuming a correct normal stress vector to impose at the BC cells!")
+                self.T = self.T[array_indices]
+
+    def get_missing_mask(self, boundaryMask):
+        if self.T is not None:
+            normals = self.get_normals(boundaryMask)
+            T = self.T
+            if np.all(normals[:, 0] == -1):  # left
+                self.i_update = (0, 4, 7)
+                self.T_update = np.stack((T[:, 0], 0.5 * T[:, 1], -0.5 * T[:, 1]), axis=-1)
+            elif np.all(normals[:, 0] == 1):  # right
+                self.i_update = (2, 5, 6)
+                self.T_update = np.stack((-T[:, 0], 0.5 * T[:, 1], -0.5 * T[:, 1]), axis=-1)
+            elif np.all(normals[:, 1] == -1):  # bottom
+                self.i_update = (1, 4, 5)
+                self.T_update = np.stack((T[:, 1], 0.5 * T[:, 0], -0.5 * T[:, 0]), axis=-1)
+            elif np.all(normals[:, 1] == 1):  # top
+                self.i_update = (3, 6, 7)
+                self.T_update = np.stack((-T[:, 1], -0.5 * T[:, 0], 0.5 * T[:, 0]), axis=-1)
+            else:
+                raise NotImplementedError
+        return super().get_missing_mask(boundaryMask)
+
+    def apply_rules(self, state):
+        state.set_masked_from_fin(self.imissingMask, self.iknownMask, -1)
+        if self.T is not None:
+            cols = np.broadcast_to(np.asarray(self.i_update, dtype=np.int64), self.T_update.shape)
+            state.add(cols, self.T_update)
