@@ -1,0 +1,169 @@
+#!/usr/bin/env python
+"""Benchmark of the lattice-Boltzmann Darcy / elasticity path on one MI355X.
+
+Workload (the headline line): the plain D2Q9 Darcy solver on a 512 x 512 domain
+(plus one ghost layer per wall: 514 x 514 cells), FP64, log-normal permeability
+K = exp(g) with g a seeded Gaussian random field of variance 1 and exponential
+covariance of correlation length 16 cells, p = 1 on the left and p = 0 on the
+right (anti-bounce-back), no flux at the bottom and top (bounce-back).  One
+"step" is one Darcy LBM update of the whole grid.  ``value`` is MLUPS in the
+reference convention (cells x steps / s / 1e6), with the populations resident in
+HBM; the steps replay captured HIP graphs of 64 updates.
+
+``roofline`` prices the dominant kernel (the fused Darcy sweep) at its
+algorithmic bytes, 19 values per cell (9 populations read, 9 written, the per-cell
+relaxation rate read) x 8 B, over its average launch duration measured with HIP
+events on the launch stream.  ``cpu_baseline`` times the CPU restatement of the
+reference algorithm (torch CPU, float64, all host cores) on a bounded sample of
+the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "xlb-biot_amd"), os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "MLUPS & HBM %peak, plain vs batched; adjoint/forward time at residual 1e-8"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def event_pair_time(torch, fn, reps):
+    """Average duration (s) of ``fn()`` launches, each bracketed by its own HIP events."""
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    for i in range(reps):
+        starts[i].record()
+        fn()
+        ends[i].record()
+    torch.cuda.synchronize()
+    return sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / reps / 1e3
+
+
+def darcy_workload(n, precision):
+    import problems as P
+    return P.darcy_c2("gpu", n=n, precision=precision, heterogeneous=True, seed=0, variance=1.0, corr=16.0)
+
+
+def bench_darcy(torch, n, precision, steps, warmup, chunk=64):
+    d = darcy_workload(n, precision)
+    f = d.assign_fields_sharded()
+    torch.cuda.synchronize()
+    f = d.run_steps(f, warmup, chunk)
+    torch.cuda.synchronize()
+    # timed region: `steps` updates, bracketed by synchronisations
+    t0 = time.perf_counter()
+    f = d.run_steps(f, steps, chunk)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    cells = d.nx * d.ny
+    # per-launch duration of the sweep kernel, on the stream it is launched on
+    fs = d._soa(f, 9)
+    out = torch.empty_like(fs)
+    kt = event_pair_time(torch, lambda: d._step_soa(fs, None, False, out=out), 200)
+    S = 8 if precision == "f64/f64" else 4
+    alg_bytes = cells * 19 * S
+    return dict(solver=d, f=f, elapsed=el, cells=cells, kernel_s=kt, bytes=alg_bytes)
+
+
+def residual_solve(torch, d, tol, max_steps, check_every=1000, chunk=64):
+    """Steps until ||f_{k+1}-f_k|| / ||f_1-f_0|| < tol; returns (steps, seconds)."""
+    f = d.assign_fields_sharded()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f1 = d.run_steps(f, 1)
+    r0 = torch.linalg.norm(f1 - f).item()
+    f = f1
+    k = 1
+    while k < max_steps:
+        f = d.run_steps(f, check_every - 1, chunk)
+        fn = d.run_steps(f, 1)
+        r = torch.linalg.norm(fn - f).item()
+        f = fn
+        k += check_every
+        if r / r0 < tol:
+            break
+    torch.cuda.synchronize()
+    return k, time.perf_counter() - t0, r / r0
+
+
+def cpu_baseline(n, precision, seconds):
+    import torch
+    import problems as P
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    o = P.darcy_c2("oracle", n=n, precision=precision, heterogeneous=True, seed=0, variance=1.0, corr=16.0)
+    f = o.assign_fields_sharded()
+    f, _ = o.step(f, 0)  # warm-up (the reference starts its clock after step 1)
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < seconds:
+        f, _ = o.step(f, k + 1)
+        k += 1
+    el = time.perf_counter() - t0
+    return dict(value=o.nx * o.ny * k / el / 1e6, unit="MLUPS", cores=threads, kind="port",
+                sample=f"{k} Darcy steps of the {o.nx}x{o.ny} FP64 workload on the CPU restatement "
+                       f"(torch CPU, {threads} threads), {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the path shards by x-slabs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(int(os.environ.get("LOCAL_RANK", "0")))
+
+    log(f"darcy C2 {args.n}x{args.n} fp64: warmup {args.warmup}, steps {args.steps}")
+    r = bench_darcy(torch, args.n, "f64/f64", args.steps, args.warmup)
+    value = r["cells"] * args.steps / r["elapsed"] / 1e6
+    achieved = r["bytes"] / r["kernel_s"] / 1e9
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["elapsed"] / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded log-normal permeability (variance 1, correlation 16 cells, seed 0)",
+        "config": {"workload": f"C2 plain Darcy D2Q9 {args.n}x{args.n} (+ghost layers: {args.n + 2}x{args.n + 2}), "
+                               "FP64, p=1 left / p=0 right, no-flux top/bottom",
+                   "nx": args.n + 2, "ny": args.n + 2, "precision": "f64/f64", "graph_chunk": 64},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_sweep<double, DarcyColl, STEP, bc>", "kernel_us": round(r["kernel_s"] * 1e6, 3),
+                     "bytes_per_cell": 152, "cells": r["cells"]},
+    }
+    extras = {}
+    if not args.no_extras:
+        log("darcy C2 fp32")
+        r32 = bench_darcy(torch, args.n, "f32/f32", args.steps // 2, args.warmup // 2)
+        extras["c2_fp32_mlups"] = round(r32["cells"] * (args.steps // 2) / r32["elapsed"] / 1e6, 2)
+        extras["c2_fp32_kernel_gbs"] = round(r32["bytes"] / r32["kernel_s"] / 1e9, 1)
+    line["extra"] = extras
+    if not args.no_cpu:
+        log(f"cpu baseline ({args.cpu_seconds:.0f} s)")
+        line["cpu_baseline"] = cpu_baseline(args.n, "f64/f64", args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,83 @@
+"""The C-ABI library builds, loads without a GPU and exports exactly what
+``include/xlb_biot.h`` declares; host-side validation of the solver API."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from src import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "xlb_biot.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(xb_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for n in ("xb_elast_step", "xb_darcy_step", "xb_darcy_gradient", "xb_elast_prolong_add", "xb_norm_finalize"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nat.load_library()
+    for n in declared_functions():
+        assert hasattr(lib, n), n
+    assert set(declared_functions()) == set(nat.SIGNATURES), "ctypes bindings and header disagree"
+    assert lib.xb_abi_version() == 1
+
+
+def test_struct_layouts_match_header():
+    # the ctypes mirrors of the argument structs follow the header field by field
+    text = open(os.path.join(ROOT, "include", "xlb_biot.h")).read()
+    for struct, cls in (("xb_elast_args", nat.ElastArgs), ("xb_darcy_args", nat.DarcyArgs),
+                        ("xb_bc_table", nat.BCTable)):
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, flags=re.S).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            names = decl.split()[-1] if "," not in decl else decl
+            for part in re.split(r",", decl):
+                fields.append(re.findall(r"[A-Za-z_][A-Za-z_0-9]*", part)[-1])
+        assert fields == [f[0] for f in cls._fields_], (struct, fields)
+
+
+def test_solver_validation_errors_match_reference():
+    import problems as P
+    from src.elasticity import LBMElasticity
+    from src.lattice import LatticeD2Q8, LatticeD2Q9
+    kw = {"lambda": 1.0, "mu": 1.0, "nx": 16, "ny": 16, "nz": 0, "precision": "f64/f64",
+          "lattice": LatticeD2Q8("f64/f64"), "print_info_rate": 0}
+    LBMElasticity(**kw)
+    with pytest.raises(TypeError):
+        LBMElasticity(**{**kw, "lambda": 1})
+    with pytest.raises(ValueError):
+        LBMElasticity(**{**kw, "MG_levels": 6})
+    with pytest.raises(ValueError):
+        LBMElasticity(**{**kw, "MG_relaxation": 1.5})
+    with pytest.raises(TypeError):
+        LBMElasticity(**{**kw, "MG_BCs": {"left": 1}})
+    with pytest.raises(ValueError):
+        LBMElasticity(**{**kw, "lattice": LatticeD2Q9("f64/f64")})
+    with pytest.raises(NotImplementedError):
+        LBMElasticity(**{**kw, "precision": "f64/f32"})
+    with pytest.raises(TypeError):
+        P.biot_periodic("gpu", nx=16, coupling="sideways")
+
+
+def test_kernels_refuse_to_run_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    import problems as P
+    el = P.periodic_elasticity("gpu", nx=16)
+    f = el.assign_fields_sharded()
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        el.step(f, 0)

@@ -93,6 +93,26 @@ def view(soa):
     return soa.permute(1, 2, 0)
 
 
+class GraphedSteps:
+    """``n`` (even) ping-pong steps ``launch(src, dst)`` captured once into a HIP graph
+    (through ``torch.cuda.CUDAGraph``) and replayed: one launch for the whole chunk.
+    The state lives in ``buf_a`` before and after each replay."""
+
+    def __init__(self, launch, buf_a, buf_b, n):
+        if n < 2 or n % 2:
+            raise ValueError("graphed chunks hold an even number of steps")
+        self.n = n
+        self.buf_a, self.buf_b = buf_a, buf_b
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for _ in range(n // 2):
+                launch(buf_a, buf_b)
+                launch(buf_b, buf_a)
+
+    def replay(self):
+        self.graph.replay()
+
+
 class NormWorkspace:
     """FP64 partial sums + the finalised (norm, non-finite flag) pair."""
 

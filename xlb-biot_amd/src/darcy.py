@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .base import LBMBase, as_soa, view
+from .base import GraphedSteps, LBMBase, as_soa, view
 from .boundary_conditions import BoundaryCondition, BounceBackHalfway, RuleTable, build_rule_table, compile_rules
 
 
@@ -133,6 +133,23 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         pc = torch.empty_like(f) if fpc else None
         nat.check(nat.lib().xb_darcy_step(self._args(f, out, ext_source, pc), nat.stream()), "xb_darcy_step")
         return out, pc
+
+    def run_steps(self, f, n, chunk=64):
+        """``n`` steps without I/O (the body of ``run`` when nothing happens between
+        steps): chunks of ``chunk`` steps replay one captured HIP graph."""
+        fs = self._soa(f, 9)
+        a = fs.clone()
+        b = torch.empty_like(a)
+        done = 0
+        if n >= chunk and chunk >= 2:
+            gs = GraphedSteps(lambda s, d: self._step_soa(s, None, False, out=d), a, b, chunk)
+            while n - done >= chunk:
+                gs.replay()
+                done += chunk
+        while done < n:
+            a, _ = self._step_soa(a)
+            done += 1
+        return view(a)
 
     def collision(self, f, external_source=None):
         fs = self._soa(f, 9)
