@@ -47,7 +47,7 @@ class OraclePoroelasticity:
         f_darcy = da.assign_fields_sharded()
         source = (-self.alpha / self.epsilon) * dt_div_eta
         if self.useDynamicSource:
-            source = source + self.get_darcy_source(0)
+            source = source + torch.as_tensor(self.get_darcy_source(0), dtype=source.dtype)
         f_darcy = f_darcy - 0.5 * source * torch.as_tensor(da.w, dtype=source.dtype)
         pressure = da.update_macroscopic(f_darcy, source)
         f_el = el.assign_fields_sharded()
@@ -58,7 +58,7 @@ class OraclePoroelasticity:
             pressure_grad = pressure_grad + da.gradient(source, 0.5)
             force = (-self.alpha * self.epsilon) * pressure_grad
         if self.useDynamicForce:
-            force = force + self.get_elastic_force(0)
+            force = force + torch.as_tensor(self.get_elastic_force(0), dtype=force.dtype)
         eta, sigma = el.initialize_macroscopic_fields()
         if eta is not None and sigma is not None:
             f_el = el.equilibrium(eta, sigma, external_force=force)
@@ -71,7 +71,7 @@ class OraclePoroelasticity:
         fprev = f_elasticity.clone()
         source_base = ((1.0 - th) * -a / eps) * dt_div_eta
         if self.useDynamicSource:
-            source_base = source_base + self.get_darcy_source(timestep)
+            source_base = source_base + torch.as_tensor(self.get_darcy_source(timestep), dtype=source_base.dtype)
         if self.alternative_gradient:
             pressure_grad_base = da.gradient(da.update_macroscopic(f_darcy, source_base))
         else:
@@ -79,7 +79,7 @@ class OraclePoroelasticity:
             pressure_grad_base = pressure_grad_base + da.gradient(source_base, 0.5)
         force = (-a * eps) * pressure_grad_base - da.gradient(el.update_divergence(fprev), 0.5 * th * a ** 2)
         if self.useDynamicForce:
-            force = force + self.get_elastic_force(timestep)
+            force = force + torch.as_tensor(self.get_elastic_force(timestep), dtype=force.dtype)
 
         def nonlinear_force(f_el):
             return da.gradient(el.update_divergence(f_el), 0.5 * th * a ** 2)

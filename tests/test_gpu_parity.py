@@ -13,11 +13,13 @@ pytestmark = pytest.mark.gpu
 TOL = {"f64/f64": 1e-12, "f32/f32": 1e-5}
 
 
-def rel(a, b):
+def rel(a, b, floor=0.0):
+    """max|a-b| / max(max|b|, floor): ``floor`` is the scale below which a field is
+    round-off (fields that are identically zero in exact arithmetic)."""
     a = torch.as_tensor(a).detach().to("cpu", torch.float64)
     b = torch.as_tensor(b).detach().to("cpu", torch.float64)
     assert a.shape == b.shape, (a.shape, b.shape)
-    den = b.abs().max().item()
+    den = max(b.abs().max().item(), floor)
     return (a - b).abs().max().item() / (den if den > 0 else 1.0)
 
 
@@ -161,14 +163,14 @@ def test_biot_periodic_steps(levels, coupling, alt):
     sg = g.initial_state()
     so = o.initial_state()
     for a, b in zip(sg, so):
-        assert rel(a.permute(1, 2, 0), b) < 1e-12
+        assert rel(a.permute(1, 2, 0), b, 1e-3) < 1e-12
     fe, fd, dde = (t.permute(1, 2, 0) for t in sg)
     oe, od, odde = so
     for t in range(3):
         outg = g.step(fe, fd, dde, t)
         outo = o.step(oe, od, odde, t)
         for k, (a, b) in enumerate(zip(outg, outo)):
-            assert rel(a, b) < 1e-11, (t, k, rel(a, b))
+            assert rel(a, b, 1e-3) < 1e-11, (t, k, rel(a, b, 1e-3))
         fe, fd, dde = outg[0], outg[1], outg[4]
         oe, od, odde = outo[0], outo[1], outo[4]
 
@@ -185,7 +187,7 @@ def test_biot_consolidation_mg_walls():
         outg = g.step(fe, fd, dde, t)
         outo = o.step(oe, od, odde, t)
         for k, (a, b) in enumerate(zip(outg, outo)):
-            assert rel(a, b) < 1e-11, (t, k, rel(a, b))
+            assert rel(a, b, 1e-3) < 1e-11, (t, k, rel(a, b, 1e-3))
         fe, fd, dde = outg[0], outg[1], outg[4]
         oe, od, odde = outo[0], outo[1], outo[4]
 

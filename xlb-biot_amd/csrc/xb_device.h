@@ -77,7 +77,19 @@ template <typename T> struct ElastColl {
   const T *fb = nullptr; // external force (2, N)
   size_t N = 0;
 
-  __device__ __forceinline__ void collide(const T (&f)[8], T (&o)[8], size_t cell) const {
+  // Per-cell operands besides the populations, loaded ahead of the collision so that
+  // the sweep can keep them in flight with the populations of the same row.
+  struct Aux {
+    T a0, a1, b0, b1;
+  };
+  __device__ __forceinline__ Aux load_aux(size_t cell) const {
+    Aux x{T(0), T(0), T(0), T(0)};
+    if (fa != nullptr) { x.a0 = fa[cell]; x.a1 = fa[N + cell]; }
+    if (fb != nullptr) { x.b0 = fb[cell]; x.b1 = fb[N + cell]; }
+    return x;
+  }
+
+  __device__ __forceinline__ void collide(const T (&f)[8], const Aux &x, T (&o)[8]) const {
     const T m10 = f[0] - f[2] + f[4] - f[5] - f[6] + f[7];
     const T m01 = f[1] - f[3] + f[4] + f[5] - f[6] - f[7];
     const T ms = f[0] + f[1] + f[2] + f[3] + T(2) * (f[4] + f[5] + f[6] + f[7]);
@@ -85,15 +97,9 @@ template <typename T> struct ElastColl {
     const T m11 = f[4] - f[5] + f[6] - f[7];
     T M0 = m10, M1 = m01, M5 = m10 / T(3), M6 = m01 / T(3);
     if (fa != nullptr || fb != nullptr) {
-      T F0, F1;
-      if (fa != nullptr && fb != nullptr) {
-        F0 = fa[cell] + fb[cell];
-        F1 = fa[N + cell] + fb[N + cell];
-      } else {
-        const T *F = fa != nullptr ? fa : fb;
-        F0 = F[cell];
-        F1 = F[N + cell];
-      }
+      // reference order: body force + external force
+      const T F0 = (fa != nullptr && fb != nullptr) ? x.a0 + x.b0 : (fa != nullptr ? x.a0 : x.b0);
+      const T F1 = (fa != nullptr && fb != nullptr) ? x.a1 + x.b1 : (fa != nullptr ? x.a1 : x.b1);
       M0 = m10 + F0;
       M1 = m01 + F1;
       M5 = m10 / T(3) + F0 / T(6);
@@ -129,17 +135,28 @@ template <typename T> struct DarcyColl {
   const T *sa = nullptr;       // static source (N)
   const T *sb = nullptr;       // external source (N)
 
-  __device__ __forceinline__ void collide(const T (&f)[9], T (&o)[9], size_t cell) const {
+  struct Aux {
+    T sa, sb, om;
+  };
+  __device__ __forceinline__ Aux load_aux(size_t cell) const {
+    Aux x{T(0), T(0), omega};
+    if (sa != nullptr) x.sa = sa[cell];
+    if (sb != nullptr) x.sb = sb[cell];
+    if (om_field != nullptr) x.om = om_field[cell];
+    return x;
+  }
+
+  __device__ __forceinline__ void collide(const T (&f)[9], const Aux &x, T (&o)[9]) const {
     T rho = f[0];
 #pragma unroll
     for (int d = 1; d < 9; ++d) rho = rho + f[d];
     const bool has_s = sa != nullptr || sb != nullptr;
     T S = T(0);
     if (has_s) {
-      S = (sa != nullptr && sb != nullptr) ? sa[cell] + sb[cell] : (sa != nullptr ? sa[cell] : sb[cell]);
+      S = (sa != nullptr && sb != nullptr) ? x.sa + x.sb : (sa != nullptr ? x.sa : x.sb);
       rho = rho + T(0.5) * S;
     }
-    const T om = om_field != nullptr ? om_field[cell] : omega;
+    const T om = x.om;
     const T sf = om_field != nullptr ? T(1) - T(0.5) * om : sfac;
 #pragma unroll
     for (int d = 0; d < 9; ++d) {

@@ -89,36 +89,54 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
   const size_t Ni = (size_t)(g.ix1 - g.ix0) * (size_t)nyi;
   const bool y_in_box = y >= g.iy0 && y < g.iy1;
 
-  // Load row x, subtract the residual where it lives, collide, post-collision rules.
-  auto load_collide = [&](int x, T(&fr)[Q], T(&pc)[Q], int &slot) {
+  using Aux = typename Coll::Aux;
+  // Everything one row needs from memory.  Rows are loaded one step ahead of their
+  // collision (two rows in flight per wavefront) so that the load latency of row
+  // x+2 overlaps the collision of row x+1 and the stores of row x.
+  struct Row {
+    T f[Q];
+    T r[Q];
+    Aux aux;
+    int slot;
+    bool has_r;
+  };
+  auto load_row = [&](int x, Row &rw) {
     const size_t cell = (size_t)x * g.ny + y;
-    T pre[Q];
 #pragma unroll
-    for (int d = 0; d < Q; ++d) {
-      fr[d] = fin[d * N + cell];
-      pre[d] = fr[d];
-    }
+    for (int d = 0; d < Q; ++d) rw.f[d] = fin[d * N + cell];
+    rw.has_r = false;
     if constexpr (HAS_RES) {
-      if (y_in_box && x >= g.ix0 && x < g.ix1) {
+      rw.has_r = y_in_box && x >= g.ix0 && x < g.ix1;
+      if (rw.has_r) {
         const size_t ci = (size_t)(x - g.ix0) * nyi + (size_t)(y - g.iy0);
 #pragma unroll
-        for (int d = 0; d < Q; ++d) pre[d] = pre[d] - res[d * Ni + ci];
+        for (int d = 0; d < Q; ++d) rw.r[d] = res[d * Ni + ci];
       }
     }
-    coll.collide(pre, pc, cell);
-    slot = -1;
+    rw.aux = coll.load_aux(cell);
+    rw.slot = -1;
+    if constexpr (HAS_BC) rw.slot = bc.slot[cell];
+  };
+  // Subtract the residual where it lives, collide, post-collision rules.
+  auto finish_row = [&](const Row &rw, T(&pc)[Q]) {
+    T pre[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) pre[d] = rw.f[d];
+    if constexpr (HAS_RES) {
+      if (rw.has_r) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) pre[d] = pre[d] - rw.r[d];
+      }
+    }
+    coll.collide(pre, rw.aux, pc);
     if constexpr (HAS_BC) {
-      slot = bc.slot[cell];
-      if (slot >= 0 && bc.code_pc != nullptr)
-        apply_rules<T, Q>(pc, pre, bc.code_pc + (size_t)slot * Q, bc.const_pc + (size_t)slot * Q);
+      if (rw.slot >= 0 && bc.code_pc != nullptr)
+        apply_rules<T, Q>(pc, pre, bc.code_pc + (size_t)rw.slot * Q, bc.const_pc + (size_t)rw.slot * Q);
     }
   };
 
-  T R0[Q], R1[Q], R2[Q], F1[Q], F2[Q];
-  int S1 = -1, S2 = -1;
-  load_collide(wrapi(xs - 1, g.nx), F2, R0, S2);
-  load_collide(xs, F1, R1, S1);
-
+  T R0[Q], R1[Q], R2[Q], F1[Q];
+  int S1 = -1;
   double nrm = 0.0;
   T acc[Q];
 #pragma unroll
@@ -127,9 +145,8 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
   const size_t Nc = (size_t)(g.ex >> 1) * (size_t)ncy;
   const bool first_of_pair = out_lane && (((yv - g.oy) & 1) == 0);
 
-  for (int x = xs; x < xe; ++x) {
-    load_collide(wrapi(x + 1, g.nx), F2, R2, S2);
-
+  // Output row x from the ring (R0, R1, R2) = post-collision rows (x-1, x, x+1).
+  auto emit = [&](int x) {
     T v[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
@@ -142,7 +159,6 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
       if (S1 >= 0 && bc.code_ps != nullptr)
         apply_rules<T, Q>(v, R1, bc.code_ps + (size_t)S1 * Q, bc.const_ps + (size_t)S1 * Q);
     }
-
     const size_t cell = (size_t)x * g.ny + y;
     if constexpr (MODE == XB_MODE_STEP) {
       if (out_lane) {
@@ -176,14 +192,37 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
         for (int d = 0; d < Q; ++d) coarse[d * Nc + cc] = acc[d] + pair[d];
       }
     }
-
+  };
+  // Finish row x+1 from `cur`, emit row x, rotate the ring.
+  auto advance = [&](int x, const Row &cur) {
+    finish_row(cur, R2);
+    emit(x);
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       R0[d] = R1[d];
       R1[d] = R2[d];
-      F1[d] = F2[d];
+      F1[d] = cur.f[d];
     }
-    S1 = S2;
+    S1 = cur.slot;
+  };
+
+  // rows xs-1 .. xe are read (xe: the halo row after the last output row)
+  Row A, B;
+  load_row(wrapi(xs - 1, g.nx), A);
+  load_row(xs, B);
+  finish_row(A, R0);
+  load_row(wrapi(xs + 1, g.nx), A);
+  finish_row(B, R1);
+#pragma unroll
+  for (int d = 0; d < Q; ++d) F1[d] = B.f[d];
+  S1 = B.slot;
+  // A holds row x+1 at even steps, B at odd steps; the other buffer takes row x+2
+  for (int x = xs; x < xe; x += 2) {
+    if (x + 2 <= xe) load_row(wrapi(x + 2, g.nx), B);
+    advance(x, A);
+    if (x + 1 >= xe) break;
+    if (x + 3 <= xe) load_row(wrapi(x + 3, g.nx), A);
+    advance(x + 1, B);
   }
 
   if constexpr (MODE == XB_MODE_RESIDUAL) {

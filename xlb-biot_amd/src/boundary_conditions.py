@@ -19,6 +19,8 @@ XLA:CPU: ``.at[rows, cols].set`` applies updates in row-major order (last write
 wins), ``.add`` accumulates every update.  This matters because the reference's
 ``imissing`` holds index 0 for every non-missing direction.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -221,9 +223,11 @@ class RuleTable:
                                   nat.ptr(self.code_pc), nat.ptr(self.code_ps),
                                   nat.ptr(self.const_pc), nat.ptr(self.const_ps))
 
+        self._ptr = ctypes.pointer(self.struct)
+
     def ref(self):
-        import ctypes
-        return ctypes.byref(self.struct) if self.nslots else None
+        """``xb_bc_table*`` for the C ABI (None when no cell carries a rule)."""
+        return self._ptr if self.nslots else None
 
 
 def compile_rules(bcs, nx, ny, q, stages=("PostCollision", "PostStreaming"), bounds=None):

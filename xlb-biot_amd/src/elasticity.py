@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .base import NormWorkspace, PrecisionPolicy, as_soa, default_device, torch_dtype, view
+from .base import GraphedSteps, NormWorkspace, PrecisionPolicy, as_soa, default_device, torch_dtype, view
 from .boundary_conditions import BoundaryCondition, build_rule_table, configure_bcs, create_grid_mask
 
 
@@ -328,6 +328,30 @@ class LBMElasticity(object):
         a = self._launch(f, out, level, nat.XB_MODE_STEP, ext=ext, fpc=pc)
         nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step")
         return out, pc
+
+    def _step_into(self, f, out, ext=None, level=0):
+        a = self._launch(f, out, level, nat.XB_MODE_STEP, ext=ext)
+        nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step")
+        return out
+
+    def run_steps(self, f, n, chunk=64, external_force=None):
+        """``n`` single-grid steps with a fixed external force and no I/O: chunks of
+        ``chunk`` steps replay one captured HIP graph."""
+        if self.levels != 1:
+            raise ValueError("run_steps is the single-grid loop; multigrid uses step() / run()")
+        ext = self._soa(external_force, 2)
+        a = self._soa(f, 8).clone()
+        b = torch.empty_like(a)
+        done = 0
+        if n >= chunk >= 2:
+            gs = GraphedSteps(lambda s, d: self._step_into(s, d, ext), a, b, chunk)
+            while n - done >= chunk:
+                gs.replay()
+                done += chunk
+        while done < n:
+            a = self._step_into(a, torch.empty_like(a), ext)
+            done += 1
+        return view(a)
 
     def _smooth(self, f, level, resid, ext):
         out = torch.empty_like(f)
