@@ -66,8 +66,10 @@ def timed_steps(torch, run, f, steps):
     return f, time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3
 
 
-def bench_darcy(torch, n, precision, steps, warmup, chunk=64):
+def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True):
     d = darcy_workload(n, precision)
+    if tune:
+        d.autotune()
     f = d.assign_fields_sharded()
     f = d.run_steps(f, warmup, chunk)
     f, wall, gpu = timed_steps(torch, lambda f, k: d.run_steps(f, k, chunk), f, steps)
@@ -81,6 +83,7 @@ def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=3
     """Periodic D2Q8 step on an n x n grid with a body force (18 values per cell)."""
     import problems as P
     el = P.periodic_elasticity("gpu", nx=n, precision=precision)
+    el.autotune()
     f = el.assign_fields_sharded()
     f = el.run_steps(f, warmup, chunk)
     f, wall, gpu = timed_steps(torch, lambda f, k: el.run_steps(f, k, chunk), f, steps)
@@ -129,6 +132,25 @@ def cpu_baseline(n, precision, seconds):
                        f"(torch CPU, {threads} threads), {el:.1f} s")
 
 
+def only(torch, args):
+    """Counter-profile entry: one kernel family, ``--steps`` launches."""
+    n = 4096
+    if args.only == "copy":
+        # plain device copy of a known size: 1 GiB read + 1 GiB written per launch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy: {args.steps} launches of {a.numel() * 8} B")
+    elif args.only == "large_elast":
+        r = bench_large_elasticity(torch, n, args.steps, 2)
+        log(f"elasticity {n}^2: {r['kernel_s'] * 1e6:.1f} us/launch, {r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
+    else:
+        r = bench_darcy(torch, n - 2, "f64/f64", args.steps, 2, chunk=16)
+        log(f"darcy {n}^2: {r['kernel_s'] * 1e6:.1f} us/launch, {r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +160,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy"],
+                    help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -149,6 +173,8 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(int(os.environ.get("LOCAL_RANK", "0")))
+    if args.only:
+        return only(torch, args)
 
     log(f"darcy C2 {args.n}x{args.n} fp64: warmup {args.warmup}, steps {args.steps}")
     r = bench_darcy(torch, args.n, "f64/f64", args.steps, args.warmup)

@@ -85,6 +85,9 @@ typedef struct xb_elast_args {
   const xb_bc_table *bc;    /* optional                                             */
   void *coarse_out;         /* RESIDUAL: (8, (ix1-ix0)/2, (iy1-iy0)/2)              */
   double *norm_partials;    /* RESIDUAL: xb_elast_partials_count() doubles          */
+  int32_t rows;             /* launch shape: rows per wavefront (power of 2 in
+                               [2, 64]) or 0 for the built-in choice                */
+  int32_t pad_;
 } xb_elast_args;
 
 typedef struct xb_darcy_args {
@@ -99,6 +102,8 @@ typedef struct xb_darcy_args {
   const void *source_a;     /* optional (nx, ny): static source                     */
   const void *source_b;     /* optional (nx, ny): external source                   */
   const xb_bc_table *bc;    /* optional                                             */
+  int32_t rows;             /* launch shape, as in xb_elast_args                    */
+  int32_t pad2_;
 } xb_darcy_args;
 
 /* ---- library ---------------------------------------------------------------- */
@@ -168,6 +173,9 @@ int xb_darcy_gradient(int32_t nx, int32_t ny, int32_t dtype, const void *q, doub
 /* out = a*x + b*y over n elements (y may be null). */
 int xb_lincomb(int64_t n, int32_t dtype, double a, const void *x, double b, const void *y,
                void *out, void *stream);
+
+/* out[0..n) = value.  A kernel, not a memset, so that it may sit in a captured graph. */
+int xb_fill(int64_t n, int32_t dtype, double value, void *out, void *stream);
 
 /* partials[k] = sum over a slice of (a - b)^2 (b may be null) in FP64; returns the
  * number of partials written through *count.  Used by convergence checks. */

@@ -51,14 +51,19 @@ struct SweepGeom {
 
 // Rows per wavefront: enough items to give every SIMD several waves, never fewer
 // than 2 (residual restriction pairs rows) and at most 64.
-SweepGeom make_geom(int nx, int ny, int ox, int oy, int ex, int ey, int ix0, int ix1, int iy0, int iy1) {
+SweepGeom make_geom(int nx, int ny, int ox, int oy, int ex, int ey, int ix0, int ix1, int iy0, int iy1,
+                    int rows_hint) {
   SweepGeom g{};
   g.nx = nx; g.ny = ny; g.ox = ox; g.oy = oy; g.ex = ex; g.ey = ey;
   g.ix0 = ix0; g.ix1 = ix1; g.iy0 = iy0; g.iy1 = iy1;
   g.nstrips = cdiv(ey, STRIP);
-  const int target = 8192;
   int rows = 64;
-  while (rows > 2 && (int64_t)g.nstrips * cdiv(ex, rows) < target) rows >>= 1;
+  if (rows_hint >= 2 && rows_hint <= 64 && (rows_hint & (rows_hint - 1)) == 0) {
+    rows = rows_hint;
+  } else {
+    const int target = 8192;
+    while (rows > 2 && (int64_t)g.nstrips * cdiv(ex, rows) < target) rows >>= 1;
+  }
   g.rows = rows;
   g.nitems = g.nstrips * cdiv(ex, rows);
   return g;
@@ -431,6 +436,11 @@ __global__ void k_lincomb(int64_t n, T a, const T *__restrict__ x, T b, const T 
     out[i] = y != nullptr ? a * x[i] + b * y[i] : a * x[i];
 }
 
+template <typename T> __global__ void k_fill(int64_t n, T v, T *__restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = v;
+}
+
 // One partial per block, fixed order inside the block (deterministic for a fixed grid).
 template <typename T>
 __global__ void __launch_bounds__(256) k_sqdiff(int64_t n, const T *__restrict__ a, const T *__restrict__ b,
@@ -491,9 +501,10 @@ int check_bc(const xb_bc_table *t, bool need_ps) {
 
 int elast_geom(const xb_elast_args *a, SweepGeom &g) {
   if (a->mode == XB_MODE_RESIDUAL)
-    g = make_geom(a->nx, a->ny, a->ix0, a->iy0, a->ix1 - a->ix0, a->iy1 - a->iy0, a->ix0, a->ix1, a->iy0, a->iy1);
+    g = make_geom(a->nx, a->ny, a->ix0, a->iy0, a->ix1 - a->ix0, a->iy1 - a->iy0, a->ix0, a->ix1, a->iy0, a->iy1,
+                  a->rows);
   else
-    g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, a->ix0, a->ix1, a->iy0, a->iy1);
+    g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, a->ix0, a->ix1, a->iy0, a->iy1, a->rows);
   return XB_OK;
 }
 
@@ -557,6 +568,8 @@ int xb_elast_step(const xb_elast_args *a, void *stream) {
     if (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1))
       return fail(XB_EINVAL, "xb_elast_step: interior box must have even extents for restriction");
   }
+  if (a->rows != 0 && (a->rows < 2 || a->rows > 64 || (a->rows & (a->rows - 1))))
+    return fail(XB_EINVAL, "xb_elast_step: rows must be 0 or a power of two in [2, 64]");
   if (a->mode == XB_MODE_SMOOTH && !(a->gamma >= 0.0 && a->gamma <= 1.0))
     return fail(XB_EINVAL, "xb_elast_step: gamma must be in [0,1]");
   if (int r = check_bc(a->bc, false)) return r;
@@ -644,8 +657,10 @@ int xb_darcy_step(const xb_darcy_args *a, void *stream) {
   if (a->nx <= 0 || a->ny <= 0 || a->f_in == nullptr || a->f_out == nullptr)
     return fail(XB_EINVAL, "xb_darcy_step: bad arguments");
   if (int r = check_bc(a->bc, false)) return r;
+  if (a->rows != 0 && (a->rows < 2 || a->rows > 64 || (a->rows & (a->rows - 1))))
+    return fail(XB_EINVAL, "xb_darcy_step: rows must be 0 or a power of two in [2, 64]");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny);
+  const SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny, a->rows);
   if (a->dtype == XB_F64) {
     DarcyColl<double> c;
     c.omega = a->omega;
@@ -713,6 +728,18 @@ int xb_lincomb(int64_t n, int32_t dtype, double a, const void *x, double b, cons
     hipLaunchKernelGGL(k_lincomb<float>, ew_grid(n), dim3(256), 0, st, n, (float)a, (const float *)x, (float)b,
                        (const float *)y, (float *)out);
   return check_launch("xb_lincomb");
+}
+
+int xb_fill(int64_t n, int32_t dtype, double value, void *out, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (n < 0 || (n > 0 && out == nullptr)) return fail(XB_EINVAL, "xb_fill: bad arguments");
+  if (n == 0) return XB_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_fill<double>, ew_grid(n), dim3(256), 0, st, n, value, (double *)out);
+  else
+    hipLaunchKernelGGL(k_fill<float>, ew_grid(n), dim3(256), 0, st, n, (float)value, (float *)out);
+  return check_launch("xb_fill");
 }
 
 int xb_sqdiff_partials(int64_t n, int32_t dtype, const void *a, const void *b, double *partials, int32_t max_partials,

@@ -30,14 +30,16 @@ class ElastArgs(ctypes.Structure):
                 ("f_in", c_void_p), ("f_out", c_void_p), ("fpc_out", c_void_p),
                 ("force_a", c_void_p), ("force_b", c_void_p), ("resid", c_void_p),
                 ("ix0", c_int32), ("ix1", c_int32), ("iy0", c_int32), ("iy1", c_int32),
-                ("bc", ctypes.POINTER(BCTable)), ("coarse_out", c_void_p), ("norm_partials", c_void_p)]
+                ("bc", ctypes.POINTER(BCTable)), ("coarse_out", c_void_p), ("norm_partials", c_void_p),
+                ("rows", c_int32), ("pad_", c_int32)]
 
 
 class DarcyArgs(ctypes.Structure):
     _fields_ = [("nx", c_int32), ("ny", c_int32), ("dtype", c_int32), ("pad_", c_int32),
                 ("omega", c_double), ("omega_field", c_void_p),
                 ("f_in", c_void_p), ("f_out", c_void_p), ("fpc_out", c_void_p),
-                ("source_a", c_void_p), ("source_b", c_void_p), ("bc", ctypes.POINTER(BCTable))]
+                ("source_a", c_void_p), ("source_b", c_void_p), ("bc", ctypes.POINTER(BCTable)),
+                ("rows", c_int32), ("pad2_", c_int32)]
 
 
 # name -> (restype, argtypes); every exported symbol of include/xlb_biot.h
@@ -60,6 +62,7 @@ SIGNATURES = {
     "xb_darcy_gradient": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_double, ctypes.POINTER(BCTable),
                                     c_void_p, c_double, c_double, c_void_p, c_void_p]),
     "xb_lincomb": (c_int32, [c_int64, c_int32, c_double, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "xb_fill": (c_int32, [c_int64, c_int32, c_double, c_void_p, c_void_p]),
     "xb_sqdiff_partials": (c_int32, [c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
                                      ctypes.POINTER(c_int32), c_void_p]),
     "xb_norm_finalize": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
@@ -104,6 +107,12 @@ def ptr(t):
 
 def stream():
     return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def fill(t, value):
+    """``t[...] = value`` with the library's fill kernel (graph-capture safe)."""
+    check(lib().xb_fill(t.numel(), dtype_code(t.dtype), float(value), ptr(t), stream()), "xb_fill")
+    return t
 
 
 def dtype_code(dtype):

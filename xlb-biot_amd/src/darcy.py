@@ -14,6 +14,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import autotune
 from .base import GraphedSteps, LBMBase, as_soa, view
 from .boundary_conditions import BoundaryCondition, BounceBackHalfway, RuleTable, build_rule_table, compile_rules
 
@@ -122,11 +123,28 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         return view(self._gradient_soa(self._soa(q, 1, nx, ny), scale))
 
     # ---- darcy.py:106-171
+    def _tune_key(self):
+        return (f"darcy/{self.precision}/{self.nx}x{self.ny}/bc{int(self._bc_table is not None)}"
+                f"/omf{int(self._omega_field is not None)}")
+
     def _args(self, f, out, ext_source, fpc):
         table = self._bc_table
         return nat.DarcyArgs(self.nx, self.ny, nat.dtype_code(f.dtype), 0, self.omega, nat.ptr(self._omega_field),
                              nat.ptr(f), nat.ptr(out), nat.ptr(fpc), nat.ptr(self._source_soa), nat.ptr(ext_source),
-                             table.ref() if table is not None else None)
+                             table.ref() if table is not None else None, autotune.lookup(self._tune_key()), 0)
+
+    def autotune(self):
+        """Measure and cache the best sweep launch shape of this solver's step."""
+        lib = nat.lib()
+        f = torch.zeros((9, self.nx, self.ny), dtype=self._tdt, device=self.device)
+        o = torch.empty_like(f)
+        a = self._args(f, o, None, None)
+
+        def run(rows):
+            a.rows = rows
+            nat.check(lib.xb_darcy_step(a, nat.stream()), "xb_darcy_step(tune)")
+        key = self._tune_key()
+        return {key: autotune.tune(key, run)}
 
     def _step_soa(self, f, ext_source=None, fpc=False, out=None):
         out = torch.empty_like(f) if out is None else out

@@ -13,6 +13,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import autotune
 from .base import GraphedSteps, NormWorkspace, PrecisionPolicy, as_soa, default_device, torch_dtype, view
 from .boundary_conditions import BoundaryCondition, build_rule_table, configure_bcs, create_grid_mask
 
@@ -307,6 +308,11 @@ class LBMElasticity(object):
         return torch.empty((c, nx, ny), dtype=self._tdt, device=self.device)
 
     # ---- kernels
+    def _tune_key(self, level, mode, has_ext, has_res):
+        nx, ny = self.grid_size(level)
+        return (f"elast/{self.precision}/{nx}x{ny}/mode{mode}/bc{int(self._tables.get(level) is not None)}"
+                f"/ext{int(has_ext)}/res{int(has_res)}")
+
     def _launch(self, f_in, f_out, level, mode, ext=None, resid=None, fpc=None, coarse=None, partials=None,
                 use_force=None):
         nx, ny = f_in.shape[1], f_in.shape[2]
@@ -318,8 +324,37 @@ class LBMElasticity(object):
         a = nat.ElastArgs(nx, ny, nat.dtype_code(f_in.dtype), mode, self.lamda, self.mu, self.gamma,
                           nat.ptr(f_in), nat.ptr(f_out), nat.ptr(fpc), nat.ptr(fa), nat.ptr(ext), nat.ptr(resid),
                           x0, x1, y0, y1, table.ref() if table is not None else None,
-                          nat.ptr(coarse), nat.ptr(partials))
+                          nat.ptr(coarse), nat.ptr(partials),
+                          autotune.lookup(self._tune_key(level, mode, ext is not None, resid is not None)), 0)
         return a
+
+    def autotune(self, levels=None):
+        """Measure and cache the best sweep launch shape for the step / smoothing /
+        residual kernels of every level of this solver."""
+        lib = nat.lib()
+        out = {}
+        for level in (range(self.levels) if levels is None else levels):
+            nx, ny = self.grid_size(level)
+            f = torch.zeros((8, nx, ny), dtype=self._tdt, device=self.device)
+            o = torch.empty_like(f)
+            x0, x1, y0, y1 = self.interior_box(level)
+            r = torch.zeros((8, x1 - x0, y1 - y0), dtype=self._tdt, device=self.device) if level > 0 else None
+            modes = [nat.XB_MODE_STEP] if self.levels == 1 else [nat.XB_MODE_SMOOTH]
+            if level < self.levels - 1:
+                modes.append(nat.XB_MODE_RESIDUAL)
+            for mode in modes:
+                coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=self._tdt, device=self.device)
+                a = self._launch(f, o, level, mode, resid=r, coarse=coarse)
+                a.rows = 2  # the most wavefronts, hence the most partial sums
+                ws = NormWorkspace(lib.xb_elast_partials_count(a), self.device)
+                a.norm_partials = nat.ptr(ws.partials)
+
+                def run(rows, a=a):
+                    a.rows = rows
+                    nat.check(lib.xb_elast_step(a, nat.stream()), "xb_elast_step(tune)")
+                key = self._tune_key(level, mode, False, r is not None)
+                out[key] = autotune.tune(key, run)
+        return out
 
     def _run_step(self, f, ext=None, level=0, fpc=False):
         """One ``_step`` on SoA buffers: returns ``(f_new, f_postcollision | None)``."""
@@ -353,8 +388,8 @@ class LBMElasticity(object):
             done += 1
         return view(a)
 
-    def _smooth(self, f, level, resid, ext):
-        out = torch.empty_like(f)
+    def _smooth(self, f, level, resid, ext, out=None):
+        out = torch.empty_like(f) if out is None else out
         a = self._launch(f, out, level, nat.XB_MODE_SMOOTH, ext=ext, resid=resid)
         nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step(smooth)")
         return out
@@ -475,7 +510,9 @@ class LBMElasticity(object):
             return view(out[0]), out[1]
         return view(out)
 
-    def _vcycle(self, f, level, residual, external_force, nonlinear_force):
+    def _vcycle(self, f, level, residual, external_force, nonlinear_force, out=None):
+        """One V-cycle from ``level`` down (elasticity.py:1221-1312) on SoA buffers; the
+        last smoothing step of the finest level writes into ``out`` when given."""
         for _ in range(self.k_smooth):
             ext = self._ext_soa(external_force, nonlinear_force, f, level)
             f = self._smooth(f, level, residual, ext)
@@ -486,15 +523,19 @@ class LBMElasticity(object):
             if level == 0:
                 norm = ws.finalize(n)[0]
             nxc, nyc = self.grid_size(level + 1)
-            f_c = torch.zeros((8, nxc, nyc), dtype=f.dtype, device=f.device)
+            f_c = nat.fill(torch.empty((8, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
             f_c = self._vcycle(f_c, level + 1, res_c, None, nonlinear_force)
             self._prolong_add(f, f_c, level)
-        for _ in range(self.k_smooth):
+        for k in range(self.k_smooth):
             ext = self._ext_soa(external_force, nonlinear_force, f, level)
-            f = self._smooth(f, level, residual, ext)
+            f = self._smooth(f, level, residual, ext, out if k == self.k_smooth - 1 else None)
         if level == 0:
             return f, norm
         return f
+
+    def vcycle_graph(self, external_force=None, nonlinear_force=None):
+        """A V-cycle captured as HIP graphs: see :class:`VCycleGraph`."""
+        return VCycleGraph(self, external_force, nonlinear_force)
 
     # ---- driver (elasticity.py:647-699, 1346-1456)
     def assign_fields_sharded(self):
@@ -535,6 +576,72 @@ class LBMElasticity(object):
     def handle_io_timestep(self, timestep, f, fstar, eta, sigma, eta_prev, sigma_prev):
         self.output_data(timestep=timestep, eta=eta, eta_prev=eta_prev, sigma=sigma, sigma_prev=sigma_prev,
                          f_poststreaming=f, f_postcollision=fstar)
+
+
+class VCycleGraph:
+    """V-cycles replayed from two captured HIP graphs (ping-pong state buffers).
+
+    The external force is a static buffer: :meth:`set_force` copies a new force in
+    before the next replays (the Biot coupler changes it once per physical step).
+    Each replay advances the state by one V-cycle and leaves the level-0 residual
+    norm of that cycle in :attr:`norm` (a device scalar; reading it synchronises)."""
+
+    def __init__(self, el, external_force=None, nonlinear_force=None):
+        if el.levels < 2:
+            raise ValueError("V-cycle graphs need MG_levels >= 2")
+        self.el = el
+        shape = (8, el.nx, el.ny)
+        self.buf = [torch.zeros(shape, dtype=el._tdt, device=el.device) for _ in range(2)]
+        self.ext = None if external_force is None else external_force.clone()
+        nlf = nonlinear_force
+        # eager warm-up: rule tables, norm workspaces and gradient tables exist before capture
+        el._vcycle(self.buf[0], 0, None, self.ext, nlf, out=self.buf[1])
+        torch.cuda.synchronize()
+        self.graphs, self.norms = [], []
+        for src in (0, 1):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                _, norm = el._vcycle(self.buf[src], 0, None, self.ext, nlf, out=self.buf[1 - src])
+            self.graphs.append(g)
+            self.norms.append(norm)
+        self.cur = 0
+        self.norm = None
+
+    @property
+    def state(self):
+        """Current populations (SoA buffer owned by the graph)."""
+        return self.buf[self.cur]
+
+    def load(self, f_soa):
+        self.buf[self.cur].copy_(f_soa)
+
+    def set_force(self, force_soa):
+        if self.ext is None:
+            raise ValueError("this V-cycle graph was captured without an external force")
+        self.ext.copy_(force_soa)
+
+    def replay(self):
+        g = self.graphs[self.cur]
+        self.norm = self.norms[self.cur]
+        g.replay()
+        self.cur = 1 - self.cur
+        return self.norm
+
+    def run(self, n):
+        for _ in range(n):
+            self.replay()
+
+    def run_to(self, tol, max_cycles):
+        """Replay until the cycle's residual norm is below ``tol`` times the first
+        cycle's (one device-to-host read per cycle); returns the cycle count."""
+        r0 = None
+        for k in range(max_cycles):
+            self.replay()
+            r = self.norm.item()
+            r0 = r if r0 is None else r0
+            if r <= tol * r0:
+                return k + 1
+        return max_cycles
 
 
 # ---------------------------------------------------------------------------

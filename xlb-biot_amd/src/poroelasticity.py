@@ -63,6 +63,12 @@ class LBMPoroelasticity(object):
             raise ValueError("elasticity and Darcy must use the same precision")
         self.useDynamicForce = self.get_elastic_force(0) is not None
         self.useDynamicSource = self.get_darcy_source(0) is not None
+        # extension: residual-driven inner loop (None keeps the reference's fixed count)
+        self.elasticityTol = kwargs.get("elasticity_tol", None)
+        self.elasticityMaxSteps = kwargs.get("elasticity_max_steps", 10000)
+        self.useGraphs = kwargs.get("use_graphs", True)
+        self._inner = None
+        self.inner_counts = []
         self.mlups = None
         self.last_residual = None
 
@@ -223,10 +229,7 @@ class LBMPoroelasticity(object):
         if self.useDynamicForce:
             force = self._lincomb(1.0, force, 1.0, self._soa(self.get_elastic_force(timestep), 2))
         nlf = BiotNonlinearForce(el, da, s_nl)
-        for _ in range(self.elasticitySteps):
-            fe, aux = el._step_soa(fe, force, nlf)
-            if aux is not None:
-                self.last_residual = aux
+        fe = self._inner_solve(fe, force, nlf)
         ext = nlf.soa_add(fe, force)
         eta = el._new(2)
         sigma = el._new(3)
@@ -242,6 +245,43 @@ class LBMPoroelasticity(object):
             pressure_grad = da._gradient_soa(dde, 0.5 * th * a / eps, base=pgb, a=1.0, b=-1.0)
         fd, _ = da._step_soa(fd, source)
         return fe, fd, eta, sigma, dde, pressure, pressure_grad
+
+    def _inner_solve(self, fe, force, nlf):
+        """The elasticity iterations of one physical step (poroelasticity.py:325-326):
+        ``elasticity_steps`` steps / V-cycles, or, with ``elasticity_tol``, V-cycles
+        until the residual norm falls below ``tol`` times that of the first cycle."""
+        el = self.elasticity
+        tol = self.elasticityTol
+        if tol is not None and el.levels < 2:
+            raise ValueError("elasticity_tol needs a multigrid elasticity solver (elasticity_MG_levels >= 2)")
+        if not self.useGraphs:
+            n = self.elasticitySteps if tol is None else self.elasticityMaxSteps
+            r0 = None
+            for k in range(n):
+                fe, aux = el._step_soa(fe, force, nlf)
+                if aux is not None:
+                    self.last_residual = aux
+                    if tol is not None:
+                        r = aux.item()
+                        r0 = r if r0 is None else r0
+                        if r <= tol * r0:
+                            break
+            self.inner_counts.append(k + 1)
+            return fe
+        if self._inner is None or self._inner_scale != nlf.scale:
+            self._inner = (el.vcycle_graph(force, nlf) if el.levels > 1 else _PlainInnerGraph(el, force, nlf))
+            self._inner_scale = nlf.scale
+        inner = self._inner
+        inner.set_force(force)
+        inner.load(fe)
+        if tol is None:
+            inner.run(self.elasticitySteps)
+            count = self.elasticitySteps
+        else:
+            count = inner.run_to(tol, self.elasticityMaxSteps)
+        self.inner_counts.append(count)
+        self.last_residual = getattr(inner, "norm", None)
+        return inner.state.clone()
 
     def step(self, f_elasticity, f_darcy, dt_div_eta, timestep):
         fe = self.elasticity._soa(f_elasticity, 8)
@@ -294,3 +334,41 @@ class LBMPoroelasticity(object):
         print("WARNING: Default initial conditions assumed: dt_div_eta = 0")
         print("         To set explicit initial dt_div_eta, use self.initialize_macroscopic_fields.")
         return None
+
+
+class _PlainInnerGraph:
+    """Single-grid inner iterations (nonlinear force + step) replayed from a captured
+    HIP graph of ``chunk`` steps; the remainder runs eagerly."""
+
+    def __init__(self, el, force, nlf, chunk=16):
+        self.el, self.nlf, self.chunk = el, nlf, chunk
+        self.ext = force.clone()
+        self.a = torch.zeros((8, el.nx, el.ny), dtype=el._tdt, device=el.device)
+        self.b = torch.empty_like(self.a)
+        launch = lambda s, d: el._step_into(s, d, nlf.soa_add(s, self.ext))
+        launch(self.a, self.b)
+        torch.cuda.synchronize()
+        from .base import GraphedSteps
+        self.graph = GraphedSteps(launch, self.a, self.b, chunk)
+
+    @property
+    def state(self):
+        return self.a
+
+    def load(self, f):
+        self.a.copy_(f)
+
+    def set_force(self, force):
+        self.ext.copy_(force)
+
+    def run(self, n):
+        done = 0
+        while n - done >= self.chunk:
+            self.graph.replay()
+            done += self.chunk
+        while done < n:
+            out = torch.empty_like(self.a)
+            self.el._step_into(self.a, out, self.nlf.soa_add(self.a, self.ext))
+            self.a.copy_(out)
+            done += 1
+
