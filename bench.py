@@ -113,6 +113,67 @@ def residual_solve(torch, d, tol, max_steps, check_every=1000, chunk=64):
     return k, time.perf_counter() - t0, r / r0
 
 
+def bench_c1(torch, tol=1e-10, check_every=10, max_steps=20000):
+    """C1: periodic 64^2 elasticity, FP64, plain steps until the relative displacement
+    residual ||f_{k+1}-f_k|| / ||f_1-f_0|| < tol; error to the closed form."""
+    import numpy as np
+    import problems as P
+    el = P.periodic_elasticity("gpu", nx=64)
+    f0 = el.assign_fields_sharded()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f1 = el.run_steps(f0, 1)
+    r0 = torch.linalg.norm(f1 - f0).item()
+    f, k, r = f1, 1, r0
+    while k < max_steps:
+        f = el.run_steps(f, check_every - 1, chunk=check_every - 1 if check_every % 2 else check_every - 2)
+        fn = el.run_steps(f, 1)
+        r = torch.linalg.norm(fn - f).item()
+        f, k = fn, k + check_every
+        if r / r0 < tol:
+            break
+    torch.cuda.synchronize()
+    el_s = time.perf_counter() - t0
+    eta, _ = el.update_macroscopic(f)
+    ue = P.manufactured_c1(64) / (8 * np.pi ** 2)
+    err = float(np.linalg.norm(eta.cpu().numpy() - ue) / np.linalg.norm(ue))
+    return {"steps": k, "seconds": round(el_s, 4), "mlups": round(64 * 64 * k / el_s / 1e6, 1),
+            "rel_residual": r / r0, "rel_error_closed_form": round(err, 6)}
+
+
+def bench_c3(torch, n=256, levels=4, steps=20, tol=1e-8):
+    """C3: periodic Biot (manufactured forcing), FP64, elasticity V-cycles (4 levels,
+    k=2) to an inner relative residual ``tol`` in every physical step."""
+    import problems as P
+    b = P.biot_periodic("gpu", nx=n, levels=levels, steps=1)
+    b.elasticityTol = tol
+    b.elasticityMaxSteps = 200
+    fe, fd, dde = b.initial_state()
+    out = b._step_soa(fe, fd, dde, 0)  # warm-up step: captures the V-cycle graphs
+    fe, fd, dde = out[0], out[1], out[4]
+    b.inner_counts.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(1, steps + 1):
+        out = b._step_soa(fe, fd, dde, t)
+        fe, fd, dde = out[0], out[1], out[4]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    cycles = sum(b.inner_counts)
+    # time of one captured V-cycle (with the nonlinear Biot force), HIP events on the stream
+    vg = b._inner
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        vg.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    vc_s = e0.elapsed_time(e1) / 1e3 / 20
+    return {"physical_steps": steps, "seconds": round(el, 4), "vcycles": cycles,
+            "vcycles_per_step": round(cycles / steps, 2), "mlups_reference_convention": round(n * n * steps / el / 1e6, 3),
+            "vcycle_us": round(vc_s * 1e6, 2), "fine_cell_vcycles_per_s": round(n * n / vc_s / 1e6, 1)}
+
+
 def cpu_baseline(n, precision, seconds):
     import torch
     import problems as P
@@ -199,6 +260,10 @@ def main():
         r32 = bench_darcy(torch, args.n, "f32/f32", args.steps // 2, args.warmup // 2)
         extras["c2_fp32_mlups"] = round(r32["cells"] * (args.steps // 2) / r32["elapsed"] / 1e6, 1)
         extras["c2_fp32_kernel_gbs"] = round(r32["bytes"] / r32["kernel_s"] / 1e9, 1)
+        log("C1 64^2 to 1e-10")
+        extras["c1_elasticity_64_fp64"] = bench_c1(torch)
+        log("C3 Biot 256^2 MG, inner 1e-8")
+        extras["c3_biot_256_mg4_fp64"] = bench_c3(torch)
         for n in (4096,):
             log(f"elasticity {n}^2 fp64 (HBM-resident)")
             rl = bench_large_elasticity(torch, n, 200, 20)

@@ -88,6 +88,14 @@ typedef struct xb_elast_args {
   int32_t rows;             /* launch shape: rows per wavefront (power of 2 in
                                [2, 64]) or 0 for the built-in choice                */
   int32_t pad_;
+  /* Fused Biot coupling force (all optional; active when div_in is set):
+   * force_b is replaced by force_b + G(div_in), G(q) = sum_d qs_d (-3 nl_scale w_d c_d)
+   * over the D2Q9 stencil, qs_d(x) = q(x - c_d) or the Darcy wall rule of the cell
+   * (grad_bc); STEP and SMOOTH also write div_out = div(f_out). */
+  const void *div_in;       /* (nx, ny) divergence of f_in                          */
+  void *div_out;            /* (nx, ny) divergence of f_out (STEP / SMOOTH)         */
+  double nl_scale;
+  const xb_bc_table *grad_bc;
 } xb_elast_args;
 
 typedef struct xb_darcy_args {
@@ -131,6 +139,13 @@ int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f,
                          const void *coarse, int32_t cnx, int32_t cny,
                          int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1,
                          void *stream);
+
+/* As xb_elast_prolong_add, and div[interior] = div(f) of the updated populations
+ * (the divergence carried by the fused Biot coupling force). */
+int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, void *f,
+                             const void *coarse, int32_t cnx, int32_t cny,
+                             int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1,
+                             void *div, void *stream);
 
 /* eta (2,nx,ny), sigma (3,nx,ny) from f and the optional forces.
  * Replaces update_macroscopic (src/elasticity.py:1002-1038). Either output may be null. */

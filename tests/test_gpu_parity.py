@@ -212,3 +212,74 @@ def test_c1_converges_to_closed_form():
     err = np.linalg.norm(eta.cpu().numpy() - ue) / np.linalg.norm(ue)
     # second order: 0.0256 / 0.0064 / 0.0016 at 16 / 32 / 64 cells (oracle)
     assert 1.4e-3 < err < 1.8e-3, err
+
+
+def test_vcycle_graph_replays_match_eager():
+    say("V-cycle graph vs eager")
+    g = P.neumann_elasticity("gpu", n=32, levels=3)
+    f = g.assign_fields_sharded()
+    vg = g.vcycle_graph()
+    vg.load(g._soa(f, 8))
+    for t in range(4):
+        n_graph = vg.replay().item()
+        f, n_eager = g.step(f, t)
+        torch.cuda.synchronize()
+        assert rel(vg.state.permute(1, 2, 0), f) < 1e-15
+        assert abs(n_graph - n_eager.item()) <= 1e-15 * abs(n_eager.item())
+
+
+def test_biot_residual_driven_inner_loop_matches_oracle():
+    say("biot inner loop to tolerance")
+    g = P.biot_periodic("gpu", nx=128, levels=3, steps=1)
+    g.elasticityTol = 1e-2
+    g.elasticityMaxSteps = 100
+    o = P.biot_periodic("oracle", nx=128, levels=3, steps=1)
+    sg, so = g.initial_state(), o.initial_state()
+    fe, fd, dde = (t.permute(1, 2, 0) for t in sg)
+    oe, od, odde = so
+    for t in range(2):
+        outg = g.step(fe, fd, dde, t)
+        o.elasticitySteps = g.inner_counts[-1]
+        outo = o.step(oe, od, odde, t)
+        assert g.inner_counts[-1] > 1
+        for k, (a, b) in enumerate(zip(outg, outo)):
+            assert rel(a, b, 1e-3) < 1e-11, (t, k)
+        fe, fd, dde = outg[0], outg[1], outg[4]
+        oe, od, odde = outo[0], outo[1], outo[4]
+
+
+@pytest.mark.parametrize("levels", [1, 3])
+def test_run_loop_matches_oracle(levels):
+    # run(t_max) performs t_max + 1 steps (elasticity.py:1372); without I/O the loop
+    # replays captured graphs
+    say("elasticity run()", levels)
+    # 64^2 with 3 levels: a contracting V-cycle (coarsest grid 16^2)
+    g = P.periodic_elasticity("gpu", nx=64, levels=levels)
+    o = P.periodic_elasticity("oracle", nx=64, levels=levels)
+    g.ioRate = 0
+    g.printInfoRate = 0
+    fg = g.run(20)
+    fo = o.run(20)
+    assert rel(fg, fo) < 1e-12
+
+
+def test_darcy_run_loop_matches_oracle():
+    say("darcy run()")
+    g = P.darcy_c2("gpu", n=30)
+    o = P.darcy_c2("oracle", n=30)
+    g.printInfoRate = 0
+    assert rel(g.run(150), o.run(150)) < 1e-12
+
+
+def test_nonfinite_residual_stops_the_inner_loop():
+    # the V-cycle of the reference has no exact coarse solve; at 64^2 with 3 levels the
+    # periodic Biot problem diverges (in the oracle too): the loop must stop and say so
+    say("biot divergence detection")
+    g = P.biot_periodic("gpu", nx=64, levels=3, steps=1)
+    g.elasticityTol = 1e-30
+    g.elasticityMaxSteps = 5000
+    fe, fd, dde = g.initial_state()
+    with pytest.raises(FloatingPointError):
+        for t in range(50):
+            out = g._step_soa(fe, fd, dde, t)
+            fe, fd, dde = out[0], out[1], out[4]

@@ -76,6 +76,7 @@ template <typename T> struct ElastColl {
   const T *fa = nullptr; // body force (2, N)
   const T *fb = nullptr; // external force (2, N)
   size_t N = 0;
+  bool has_b = false;    // an external force takes part (given, or formed in the sweep)
 
   // Per-cell operands besides the populations, loaded ahead of the collision so that
   // the sweep can keep them in flight with the populations of the same row.
@@ -96,10 +97,10 @@ template <typename T> struct ElastColl {
     const T md = f[0] - f[1] + f[2] - f[3];
     const T m11 = f[4] - f[5] + f[6] - f[7];
     T M0 = m10, M1 = m01, M5 = m10 / T(3), M6 = m01 / T(3);
-    if (fa != nullptr || fb != nullptr) {
+    if (fa != nullptr || has_b) {
       // reference order: body force + external force
-      const T F0 = (fa != nullptr && fb != nullptr) ? x.a0 + x.b0 : (fa != nullptr ? x.a0 : x.b0);
-      const T F1 = (fa != nullptr && fb != nullptr) ? x.a1 + x.b1 : (fa != nullptr ? x.a1 : x.b1);
+      const T F0 = (fa != nullptr && has_b) ? x.a0 + x.b0 : (fa != nullptr ? x.a0 : x.b0);
+      const T F1 = (fa != nullptr && has_b) ? x.a1 + x.b1 : (fa != nullptr ? x.a1 : x.b1);
       M0 = m10 + F0;
       M1 = m01 + F1;
       M5 = m10 / T(3) + F0 / T(6);
@@ -167,6 +168,39 @@ template <typename T> struct DarcyColl {
       o[d] = v;
     }
   }
+};
+
+// ---------------------------------------------------------------------------
+// Shared by the stand-alone kernels and the fused sweep so both paths do the same
+// arithmetic in the same order.
+
+// divergence of D2Q8 populations: k * m_s (reference elasticity.py:1040-1058)
+template <typename T> __device__ __forceinline__ T elast_div_of(const T (&v)[8], T k) {
+  return k * (v[0] + v[1] + v[2] + v[3] + T(2) * (v[4] + v[5] + v[6] + v[7]));
+}
+
+// lattice gradient sum_d qs_d ((-3 scale) w_d) c_d over the D2Q9 stencil
+// (reference darcy.py:75-104, the final matmul)
+template <typename T> __device__ __forceinline__ void stencil_grad(const T (&qs)[9], T m3s, T &g0, T &g1) {
+  using C = DarcyColl<T>;
+  g0 = T(0);
+  g1 = T(0);
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    const T cw = m3s * T(C::wd(d));
+    if (C::cx(d) != 0) g0 = g0 + qs[d] * (C::cx(d) > 0 ? cw : -cw);
+    if (C::cy(d) != 0) g1 = g1 + qs[d] * (C::cy(d) > 0 ? cw : -cw);
+  }
+}
+
+// Fused Biot coupling force of the elasticity sweep
+template <typename T> struct NLDev {
+  const T *div_in = nullptr;
+  T *div_out = nullptr;
+  T m3s = T(0);   // -3 * scale
+  T kdiv = T(0);  // -1.5 / (3 (lam + mu) + 1)
+  BCDev<T> gbc;   // Darcy wall rules of the gradient
+  const T *base = nullptr;
 };
 
 } // namespace xb

@@ -70,11 +70,11 @@ SweepGeom make_geom(int nx, int ny, int ox, int oy, int ex, int ey, int ix0, int
 }
 
 // ---------------------------------------------------------------------------
-template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES>
+template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES, bool HAS_NL>
 __global__ void __launch_bounds__(BLOCK)
 k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
         T *__restrict__ fpc, const T *__restrict__ res, const BCDev<T> bc, const T omg, const T gamma,
-        T *__restrict__ coarse, double *__restrict__ partials) {
+        T *__restrict__ coarse, double *__restrict__ partials, const NLDev<T> nl) {
   constexpr int Q = Coll::Q;
   const int lane = threadIdx.x & (WAVE - 1);
   const int item = xcd_block_id() * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
@@ -101,8 +101,10 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
   struct Row {
     T f[Q];
     T r[Q];
+    T dv[9];   // HAS_NL: divergence on the D2Q9 stencil around the cell
     Aux aux;
     int slot;
+    int gslot; // HAS_NL: Darcy wall rule slot of the cell
     bool has_r;
   };
   auto load_row = [&](int x, Row &rw) {
@@ -121,6 +123,13 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
     rw.aux = coll.load_aux(cell);
     rw.slot = -1;
     if constexpr (HAS_BC) rw.slot = bc.slot[cell];
+    if constexpr (HAS_NL) {
+      using D = DarcyColl<T>;
+#pragma unroll
+      for (int d = 0; d < 9; ++d)
+        rw.dv[d] = nl.div_in[(size_t)wrapi(x - D::cx(d), g.nx) * g.ny + wrapi(y - D::cy(d), g.ny)];
+      rw.gslot = nl.gbc.slot != nullptr ? nl.gbc.slot[cell] : -1;
+    }
   };
   // Subtract the residual where it lives, collide, post-collision rules.
   auto finish_row = [&](const Row &rw, T(&pc)[Q]) {
@@ -133,7 +142,26 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
         for (int d = 0; d < Q; ++d) pre[d] = pre[d] - rw.r[d];
       }
     }
-    coll.collide(pre, rw.aux, pc);
+    if constexpr (HAS_NL) {
+      // external force + grad(div f) (reference poroelasticity.py:324, elasticity.py:1259-1265)
+      T qs[9];
+#pragma unroll
+      for (int d = 0; d < 9; ++d) qs[d] = rw.dv[d];
+      if (rw.gslot >= 0) {
+        T src[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) src[d] = rw.dv[0];
+        apply_rules<T, 9>(qs, src, nl.gbc.code_ps + (size_t)rw.gslot * 9, nl.gbc.const_ps + (size_t)rw.gslot * 9);
+      }
+      T g0, g1;
+      stencil_grad<T>(qs, nl.m3s, g0, g1);
+      Aux ax = rw.aux;
+      ax.b0 = nl.base != nullptr ? ax.b0 + g0 : g0;
+      ax.b1 = nl.base != nullptr ? ax.b1 + g1 : g1;
+      coll.collide(pre, ax, pc);
+    } else {
+      coll.collide(pre, rw.aux, pc);
+    }
     if constexpr (HAS_BC) {
       if (rw.slot >= 0 && bc.code_pc != nullptr)
         apply_rules<T, Q>(pc, pre, bc.code_pc + (size_t)rw.slot * Q, bc.const_pc + (size_t)rw.slot * Q);
@@ -173,11 +201,21 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
 #pragma unroll
           for (int d = 0; d < Q; ++d) fpc[d * N + cell] = R1[d];
         }
+        if constexpr (HAS_NL && Q == 8) {
+          if (nl.div_out != nullptr) nl.div_out[cell] = elast_div_of<T>(v, nl.kdiv);
+        }
       }
     } else if constexpr (MODE == XB_MODE_SMOOTH) {
       if (out_lane) {
+        T o[Q];
 #pragma unroll
-        for (int d = 0; d < Q; ++d) fout[d * N + cell] = omg * F1[d] + gamma * v[d];
+        for (int d = 0; d < Q; ++d) {
+          o[d] = omg * F1[d] + gamma * v[d];
+          fout[d * N + cell] = o[d];
+        }
+        if constexpr (HAS_NL && Q == 8) {
+          if (nl.div_out != nullptr) nl.div_out[cell] = elast_div_of<T>(o, nl.kdiv);
+        }
       }
     } else {
       // residual r = f - S(f - r_in) on the interior box, 2x2 sum, squared norm
@@ -239,7 +277,8 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
 
 template <typename T, class Coll, int MODE>
 int launch_sweep(const SweepGeom &g, const Coll &coll, const void *fin, void *fout, void *fpc, const void *res,
-                 const xb_bc_table *bct, double gamma, void *coarse, double *partials, hipStream_t st) {
+                 const xb_bc_table *bct, double gamma, void *coarse, double *partials, hipStream_t st,
+                 const NLDev<T> *nlp = nullptr) {
   BCDev<T> bc;
   const bool has_bc = bct != nullptr && bct->cell_slot != nullptr;
   if (has_bc) {
@@ -257,12 +296,23 @@ int launch_sweep(const SweepGeom &g, const Coll &coll, const void *fin, void *fo
   T *fp = static_cast<T *>(fpc);
   const T *rs = static_cast<const T *>(res);
   T *co = static_cast<T *>(coarse);
-#define XB_SWEEP(BC, RES) \
-  hipLaunchKernelGGL((k_sweep<T, Coll, MODE, BC, RES>), grid, block, 0, st, g, coll, fi, fo, fp, rs, bc, omg, gm, co, partials)
-  if (has_bc && has_res) XB_SWEEP(true, true);
-  else if (has_bc) XB_SWEEP(true, false);
-  else if (has_res) XB_SWEEP(false, true);
-  else XB_SWEEP(false, false);
+  const NLDev<T> nl = nlp != nullptr ? *nlp : NLDev<T>();
+#define XB_SWEEP(BC, RES, NL) \
+  hipLaunchKernelGGL((k_sweep<T, Coll, MODE, BC, RES, NL>), grid, block, 0, st, g, coll, fi, fo, fp, rs, bc, omg, gm, \
+                     co, partials, nl)
+  if constexpr (Coll::Q == 8) {
+    if (nlp != nullptr) {
+      if (has_bc && has_res) XB_SWEEP(true, true, true);
+      else if (has_bc) XB_SWEEP(true, false, true);
+      else if (has_res) XB_SWEEP(false, true, true);
+      else XB_SWEEP(false, false, true);
+      return check_launch("sweep kernel");
+    }
+  }
+  if (has_bc && has_res) XB_SWEEP(true, true, false);
+  else if (has_bc) XB_SWEEP(true, false, false);
+  else if (has_res) XB_SWEEP(false, true, false);
+  else XB_SWEEP(false, false, false);
 #undef XB_SWEEP
   return check_launch("sweep kernel");
 }
@@ -288,6 +338,26 @@ __global__ void k_prolong_add(int nx, int ny, T *__restrict__ f, const T *__rest
     const size_t cc = (size_t)(ix0 + (xi >> 1)) * cny + (iy0 + (yi >> 1));
 #pragma unroll
     for (int d = 0; d < 8; ++d) f[d * N + cell] = f[d * N + cell] + c[d * Nc + cc];
+  }
+}
+
+template <typename T>
+__global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__restrict__ c, int cnx, int cny,
+                                  int ix0, int ix1, int iy0, int iy1, T k, T *__restrict__ div) {
+  const int nxi = ix1 - ix0, nyi = iy1 - iy0;
+  const int64_t n = (int64_t)nxi * nyi;
+  const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int xi = (int)(t / nyi), yi = (int)(t % nyi);
+    const size_t cell = (size_t)(ix0 + xi) * ny + (iy0 + yi);
+    const size_t cc = (size_t)(ix0 + (xi >> 1)) * cny + (iy0 + (yi >> 1));
+    T v[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      v[d] = f[d * N + cell] + c[d * Nc + cc];
+      f[d * N + cell] = v[d];
+    }
+    div[cell] = elast_div_of<T>(v, k);
   }
 }
 
@@ -323,7 +393,7 @@ __global__ void k_elast_div(int64_t N, T k, const T *__restrict__ f, const T *__
     T v[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) v[d] = fs != nullptr ? f[d * N + i] - fs[d * N + i] : f[d * N + i];
-    out[i] = k * (v[0] + v[1] + v[2] + v[3] + T(2) * (v[4] + v[5] + v[6] + v[7]));
+    out[i] = elast_div_of<T>(v, k);
   }
 }
 
@@ -413,13 +483,8 @@ __global__ void k_darcy_gradient(int nx, int ny, const T *__restrict__ q, T m3s,
         apply_rules<T, 9>(qs, src, bc.code_ps + (size_t)s * 9, bc.const_ps + (size_t)s * 9);
       }
     }
-    T g0 = T(0), g1 = T(0);
-#pragma unroll
-    for (int d = 0; d < 9; ++d) {
-      const T cw = m3s * T(C::wd(d));
-      if (C::cx(d) != 0) g0 = g0 + qs[d] * (C::cx(d) > 0 ? cw : -cw);
-      if (C::cy(d) != 0) g1 = g1 + qs[d] * (C::cy(d) > 0 ? cw : -cw);
-    }
+    T g0, g1;
+    stencil_grad<T>(qs, m3s, g0, g1);
     if (base != nullptr) {
       out[i] = a * base[i] + b * g0;
       out[N + i] = a * base[N + i] + b * g1;
@@ -520,18 +585,30 @@ template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st) {
   c.fa = static_cast<const T *>(a->force_a);
   c.fb = static_cast<const T *>(a->force_b);
   c.N = (size_t)a->nx * a->ny;
+  c.has_b = a->force_b != nullptr || a->div_in != nullptr;
+  NLDev<T> nl;
+  const NLDev<T> *nlp = nullptr;
+  if (a->div_in != nullptr) {
+    nl.div_in = static_cast<const T *>(a->div_in);
+    nl.div_out = static_cast<T *>(a->div_out);
+    nl.m3s = T(-3 * a->nl_scale);
+    nl.kdiv = T(-1.5 / (3 * (a->lam + a->mu) + 1));
+    nl.gbc = bcdev<T>(a->grad_bc);
+    nl.base = static_cast<const T *>(a->force_b);
+    nlp = &nl;
+  }
   SweepGeom geo;
   elast_geom(a, geo);
   switch (a->mode) {
   case XB_MODE_STEP:
     return launch_sweep<T, ElastColl<T>, XB_MODE_STEP>(geo, c, a->f_in, a->f_out, a->fpc_out, a->resid, a->bc, 1.0,
-                                                       nullptr, nullptr, st);
+                                                       nullptr, nullptr, st, nlp);
   case XB_MODE_SMOOTH:
     return launch_sweep<T, ElastColl<T>, XB_MODE_SMOOTH>(geo, c, a->f_in, a->f_out, nullptr, a->resid, a->bc,
-                                                         a->gamma, nullptr, nullptr, st);
+                                                         a->gamma, nullptr, nullptr, st, nlp);
   default:
     return launch_sweep<T, ElastColl<T>, XB_MODE_RESIDUAL>(geo, c, a->f_in, nullptr, nullptr, a->resid, a->bc, 1.0,
-                                                           a->coarse_out, a->norm_partials, st);
+                                                           a->coarse_out, a->norm_partials, st, nlp);
   }
 }
 
@@ -573,6 +650,11 @@ int xb_elast_step(const xb_elast_args *a, void *stream) {
   if (a->mode == XB_MODE_SMOOTH && !(a->gamma >= 0.0 && a->gamma <= 1.0))
     return fail(XB_EINVAL, "xb_elast_step: gamma must be in [0,1]");
   if (int r = check_bc(a->bc, false)) return r;
+  if (a->div_in == nullptr && (a->div_out != nullptr || a->grad_bc != nullptr))
+    return fail(XB_EINVAL, "xb_elast_step: div_out / grad_bc without div_in");
+  if (a->div_in != nullptr) {
+    if (int r = check_bc(a->grad_bc, true)) return r;
+  }
   hipStream_t st = static_cast<hipStream_t>(stream);
   return a->dtype == XB_F64 ? elast_step_t<double>(a, st) : elast_step_t<float>(a, st);
 }
@@ -594,6 +676,28 @@ int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const v
     hipLaunchKernelGGL(k_prolong_add<float>, gr, dim3(256), 0, st, nx, ny, (float *)f, (const float *)coarse, cnx, cny,
                        ix0, ix1, iy0, iy1);
   return check_launch("xb_elast_prolong_add");
+}
+
+int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, void *f,
+                             const void *coarse, int32_t cnx, int32_t cny, int32_t ix0, int32_t ix1, int32_t iy0,
+                             int32_t iy1, void *div, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (f == nullptr || coarse == nullptr || div == nullptr)
+    return fail(XB_EINVAL, "xb_elast_prolong_add_div: null pointer");
+  if (ix0 < 0 || ix1 > nx || iy0 < 0 || iy1 > ny || ix0 >= ix1 || iy0 >= iy1 || ((ix1 - ix0) & 1) || ((iy1 - iy0) & 1))
+    return fail(XB_EINVAL, "xb_elast_prolong_add_div: bad interior box");
+  if (cnx != ix0 + (ix1 - ix0) / 2 + (nx - ix1) || cny != iy0 + (iy1 - iy0) / 2 + (ny - iy1))
+    return fail(XB_EINVAL, "xb_elast_prolong_add_div: coarse grid %d x %d does not match the fine box", cnx, cny);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 gr = ew_grid((int64_t)(ix1 - ix0) * (iy1 - iy0));
+  const double k = -1.5 / (3 * (lam + mu) + 1);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_prolong_add_div<double>, gr, dim3(256), 0, st, nx, ny, (double *)f, (const double *)coarse,
+                       cnx, cny, ix0, ix1, iy0, iy1, k, (double *)div);
+  else
+    hipLaunchKernelGGL(k_prolong_add_div<float>, gr, dim3(256), 0, st, nx, ny, (float *)f, (const float *)coarse, cnx,
+                       cny, ix0, ix1, iy0, iy1, (float)k, (float *)div);
+  return check_launch("xb_elast_prolong_add_div");
 }
 
 int xb_elast_macro(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *f, const void *force_a,

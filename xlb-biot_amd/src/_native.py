@@ -31,7 +31,9 @@ class ElastArgs(ctypes.Structure):
                 ("force_a", c_void_p), ("force_b", c_void_p), ("resid", c_void_p),
                 ("ix0", c_int32), ("ix1", c_int32), ("iy0", c_int32), ("iy1", c_int32),
                 ("bc", ctypes.POINTER(BCTable)), ("coarse_out", c_void_p), ("norm_partials", c_void_p),
-                ("rows", c_int32), ("pad_", c_int32)]
+                ("rows", c_int32), ("pad_", c_int32),
+                ("div_in", c_void_p), ("div_out", c_void_p), ("nl_scale", c_double),
+                ("grad_bc", ctypes.POINTER(BCTable))]
 
 
 class DarcyArgs(ctypes.Structure):
@@ -50,6 +52,9 @@ SIGNATURES = {
     "xb_elast_partials_count": (c_int64, [ctypes.POINTER(ElastArgs)]),
     "xb_elast_prolong_add": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_int32,
                                        c_int32, c_int32, c_int32, c_int32, c_void_p]),
+    "xb_elast_prolong_add_div": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
+                                           c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, c_void_p,
+                                           c_void_p]),
     "xb_elast_macro": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     "xb_elast_divergence": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,

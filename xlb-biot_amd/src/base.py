@@ -113,6 +113,38 @@ class GraphedSteps:
         self.graph.replay()
 
 
+class StepRunner:
+    """Persistent ping-pong buffers and one captured graph of ``chunk`` steps of
+    ``launch(src, dst)``; :meth:`run` advances a state by any number of steps
+    (graph replays, then single launches) and returns a fresh copy of the result."""
+
+    def __init__(self, launch, like, chunk):
+        self.launch = launch
+        self.a = torch.empty_like(like)
+        self.b = torch.empty_like(like)
+        self.a.copy_(like)
+        launch(self.a, self.b)  # warm-up outside the capture
+        torch.cuda.synchronize()
+        self.chunk = chunk
+        self.graph = GraphedSteps(launch, self.a, self.b, chunk) if chunk >= 2 else None
+
+    def run(self, f, n):
+        self.a.copy_(f)
+        done = 0
+        if self.graph is not None:
+            while n - done >= self.chunk:
+                self.graph.replay()
+                done += self.chunk
+        while done < n:
+            self.launch(self.a, self.b)
+            self.a, self.b = self.b, self.a
+            done += 1
+        if self.graph is not None and self.a is not self.graph.buf_a:
+            self.a, self.b = self.b, self.a
+            self.a.copy_(self.b)
+        return self.a.clone()
+
+
 class NormWorkspace:
     """FP64 partial sums + the finalised (norm, non-finite flag) pair."""
 

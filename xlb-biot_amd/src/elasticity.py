@@ -14,7 +14,7 @@ import torch
 
 from . import _native as nat
 from . import autotune
-from .base import GraphedSteps, NormWorkspace, PrecisionPolicy, as_soa, default_device, torch_dtype, view
+from .base import NormWorkspace, StepRunner, PrecisionPolicy, as_soa, default_device, torch_dtype, view
 from .boundary_conditions import BoundaryCondition, build_rule_table, configure_bcs, create_grid_mask
 
 
@@ -58,7 +58,9 @@ class LBMElasticity(object):
         self._force_soa = None if force is None else as_soa(force, self.nx, self.ny, 2, self._tdt, self.device)
         self.force = None if self._force_soa is None else view(self._force_soa)
         self._norm_ws = {}
+        self._runners = {}
         self.mlups = None
+        self.last_residual = None
 
     @property
     def _tdt(self):
@@ -370,23 +372,19 @@ class LBMElasticity(object):
         return out
 
     def run_steps(self, f, n, chunk=64, external_force=None):
-        """``n`` single-grid steps with a fixed external force and no I/O: chunks of
-        ``chunk`` steps replay one captured HIP graph."""
+        """``n`` single-grid steps with a fixed external force and no I/O, replayed
+        from a captured HIP graph of ``chunk`` steps."""
         if self.levels != 1:
             raise ValueError("run_steps is the single-grid loop; multigrid uses step() / run()")
-        ext = self._soa(external_force, 2)
-        a = self._soa(f, 8).clone()
-        b = torch.empty_like(a)
-        done = 0
-        if n >= chunk >= 2:
-            gs = GraphedSteps(lambda s, d: self._step_into(s, d, ext), a, b, chunk)
-            while n - done >= chunk:
-                gs.replay()
-                done += chunk
-        while done < n:
-            a = self._step_into(a, torch.empty_like(a), ext)
-            done += 1
-        return view(a)
+        fs = self._soa(f, 8)
+        key = ("steps", chunk, external_force is None)
+        runner = self._runners.get(key)
+        if runner is None or external_force is not None:
+            ext = self._soa(external_force, 2)
+            runner = StepRunner(lambda s, d: self._step_into(s, d, ext), fs, chunk)
+            if external_force is None:
+                self._runners[key] = runner
+        return view(runner.run(fs, n))
 
     def _smooth(self, f, level, resid, ext, out=None):
         out = torch.empty_like(f) if out is None else out
@@ -550,6 +548,22 @@ class LBMElasticity(object):
     def run(self, t_max):
         f = self.assign_fields_sharded()
         start = time.time()
+        if self.ioRate == 0 and self.printInfoRate == 0 and not self.returnFpost and t_max >= 2:
+            # nothing happens between steps: steps 0 and 1 eagerly (the MLUPS clock
+            # starts after step 1), the remaining t_max - 1 from captured graphs
+            for timestep in (0, 1):
+                f, _ = self.step(f, timestep)
+            torch.cuda.synchronize()
+            start = time.time()
+            if self.levels == 1:
+                f = self.run_steps(f, t_max - 1)
+            else:
+                vg = self.vcycle_graph()
+                vg.load(self._soa(f, 8))
+                vg.run(t_max - 1)
+                self.last_residual = vg.norm
+                f = view(vg.state.clone())
+            return self._finish_mlups(f, start, t_max)
         for timestep in range(0, t_max + 1):
             io_flag = self.ioRate > 0 and (timestep % self.ioRate == 0 or timestep == t_max)
             print_iter_flag = self.printInfoRate > 0 and timestep % self.printInfoRate == 0
@@ -564,6 +578,9 @@ class LBMElasticity(object):
             if self.computeMLUPS and timestep == 1:
                 torch.cuda.synchronize()
                 start = time.time()
+        return self._finish_mlups(f, start, t_max)
+
+    def _finish_mlups(self, f, start, t_max):
         if self.computeMLUPS:
             torch.cuda.synchronize()
             end = time.time()
@@ -638,6 +655,8 @@ class VCycleGraph:
         for k in range(max_cycles):
             self.replay()
             r = self.norm.item()
+            if not np.isfinite(r):
+                raise FloatingPointError(f"V-cycle residual is not finite after {k + 1} cycles")
             r0 = r if r0 is None else r0
             if r <= tol * r0:
                 return k + 1

@@ -263,6 +263,8 @@ class LBMPoroelasticity(object):
                     self.last_residual = aux
                     if tol is not None:
                         r = aux.item()
+                        if not np.isfinite(r):
+                            raise FloatingPointError(f"V-cycle residual is not finite after {k + 1} cycles")
                         r0 = r if r0 is None else r0
                         if r <= tol * r0:
                             break
