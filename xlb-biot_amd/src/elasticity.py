@@ -483,8 +483,86 @@ class LBMElasticity(object):
         return self._mg_step(f_poststreaming, timestep, external_force=external_force,
                              nonlinear_force=nonlinear_force)
 
+    # ---- fused Biot coupling force: the sweep forms base + grad(div f) itself from a
+    # carried divergence field (every sweep writes div of its output)
+    @staticmethod
+    def _fusable(nonlinear_force):
+        return getattr(nonlinear_force, "fused", False)
+
+    def _nl_args(self, a, div_in, div_out, nlf, level):
+        nx, ny = self.grid_size(level)
+        gt = nlf.darcy._grad_table(nx, ny)
+        a.div_in = nat.ptr(div_in)
+        a.div_out = nat.ptr(div_out)
+        a.nl_scale = float(nlf.scale)
+        a.grad_bc = gt.ref() if gt is not None else None
+        return a
+
+    def _step_nl(self, f, div, level, base, nlf, out=None, out_div=None):
+        out = torch.empty_like(f) if out is None else out
+        out_div = torch.empty_like(div) if out_div is None else out_div
+        a = self._nl_args(self._launch(f, out, level, nat.XB_MODE_STEP, ext=base), div, out_div, nlf, level)
+        nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step(coupled)")
+        return out, out_div
+
+    def _smooth_nl(self, f, div, level, resid, base, nlf, out=None, out_div=None):
+        out = torch.empty_like(f) if out is None else out
+        out_div = torch.empty_like(div) if out_div is None else out_div
+        a = self._nl_args(self._launch(f, out, level, nat.XB_MODE_SMOOTH, ext=base, resid=resid), div, out_div,
+                          nlf, level)
+        nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step(coupled smooth)")
+        return out, out_div
+
+    def _residual_nl(self, f, div, level, resid, base, nlf):
+        x0, x1, y0, y1 = self.interior_box(level)
+        coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=f.dtype, device=f.device)
+        lib = nat.lib()
+        a = self._nl_args(self._launch(f, None, level, nat.XB_MODE_RESIDUAL, ext=base, resid=resid,
+                                       coarse=coarse), div, None, nlf, level)
+        n = int(lib.xb_elast_partials_count(a))
+        ws = self._norm_ws.get(level)
+        if ws is None or ws.partials.numel() < n:
+            ws = self._norm_ws[level] = NormWorkspace(n, f.device)
+        a.norm_partials = nat.ptr(ws.partials)
+        nat.check(lib.xb_elast_step(a, nat.stream()), "xb_elast_step(coupled residual)")
+        return coarse, ws, n
+
+    def _vcycle_nl(self, f, div, level, residual, base, nlf, out=None, out_div=None):
+        """``_vcycle`` with the Biot coupling force fused into the sweeps; ``div`` is
+        the divergence of ``f`` and is carried along.  Returns ``(f, div[, norm])``."""
+        for _ in range(self.k_smooth):
+            f, div = self._smooth_nl(f, div, level, residual, base, nlf)
+        norm = None
+        if level < self.levels - 1:
+            res_c, ws, n = self._residual_nl(f, div, level, residual, base, nlf)
+            if level == 0:
+                norm = ws.finalize(n)[0]
+            nxc, nyc = self.grid_size(level + 1)
+            f_c = nat.fill(torch.empty((8, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
+            d_c = nat.fill(torch.empty((1, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
+            f_c, d_c = self._vcycle_nl(f_c, d_c, level + 1, res_c, None, nlf)
+            nx, ny = self.grid_size(level)
+            x0, x1, y0, y1 = self.interior_box(level)
+            nat.check(nat.lib().xb_elast_prolong_add_div(nx, ny, nat.dtype_code(f.dtype), self.lamda, self.mu,
+                                                         nat.ptr(f), nat.ptr(f_c), nxc, nyc, x0, x1, y0, y1,
+                                                         nat.ptr(div), nat.stream()), "xb_elast_prolong_add_div")
+        for k in range(self.k_smooth):
+            last = k == self.k_smooth - 1
+            f, div = self._smooth_nl(f, div, level, residual, base, nlf, out if last else None,
+                                     out_div if last else None)
+        if level == 0:
+            return f, div, norm
+        return f, div
+
     def _step_soa(self, f, external_force=None, nonlinear_force=None):
         """One ``step`` on SoA buffers: ``(f_new, residual_norm | None)``."""
+        if self._fusable(nonlinear_force):
+            div = self._divergence_soa(f)
+            if self.levels == 1:
+                out, _ = self._step_nl(f, div, 0, external_force, nonlinear_force)
+                return out, None
+            f, _, norm = self._vcycle_nl(f, div, 0, None, external_force, nonlinear_force)
+            return f, norm
         if self.levels == 1:
             out, _ = self._run_step(f, self._ext_soa(external_force, nonlinear_force, f, 0), 0)
             return out, None
@@ -611,14 +689,24 @@ class VCycleGraph:
         self.buf = [torch.zeros(shape, dtype=el._tdt, device=el.device) for _ in range(2)]
         self.ext = None if external_force is None else external_force.clone()
         nlf = nonlinear_force
+        self.fused = el._fusable(nlf)
+        self.div = [torch.zeros((1, el.nx, el.ny), dtype=el._tdt, device=el.device) for _ in range(2)]
+
+        def cycle(src):
+            if self.fused:
+                _, _, norm = el._vcycle_nl(self.buf[src], self.div[src], 0, None, self.ext, nlf,
+                                           out=self.buf[1 - src], out_div=self.div[1 - src])
+            else:
+                _, norm = el._vcycle(self.buf[src], 0, None, self.ext, nlf, out=self.buf[1 - src])
+            return norm
         # eager warm-up: rule tables, norm workspaces and gradient tables exist before capture
-        el._vcycle(self.buf[0], 0, None, self.ext, nlf, out=self.buf[1])
+        cycle(0)
         torch.cuda.synchronize()
         self.graphs, self.norms = [], []
         for src in (0, 1):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                _, norm = el._vcycle(self.buf[src], 0, None, self.ext, nlf, out=self.buf[1 - src])
+                norm = cycle(src)
             self.graphs.append(g)
             self.norms.append(norm)
         self.cur = 0
@@ -631,6 +719,11 @@ class VCycleGraph:
 
     def load(self, f_soa):
         self.buf[self.cur].copy_(f_soa)
+        if self.fused:
+            el = self.el
+            nat.check(nat.lib().xb_elast_divergence(el.nx, el.ny, nat.dtype_code(f_soa.dtype), el.lamda, el.mu,
+                                                    nat.ptr(self.buf[self.cur]), None, nat.ptr(self.div[self.cur]),
+                                                    nat.stream()), "xb_elast_divergence")
 
     def set_force(self, force_soa):
         if self.ext is None:
@@ -648,10 +741,13 @@ class VCycleGraph:
         for _ in range(n):
             self.replay()
 
-    def run_to(self, tol, max_cycles):
+    def run_to(self, tol, max_cycles, patience=None):
         """Replay until the cycle's residual norm is below ``tol`` times the first
-        cycle's (one device-to-host read per cycle); returns the cycle count."""
+        cycle's (one device-to-host read per cycle).  With ``patience``, also stop once
+        the residual has not improved for that many cycles, restoring the state of the
+        best cycle.  Returns ``(cycles, best relative residual, reached)``."""
         r0 = None
+        best, best_k, best_state = None, 0, None
         for k in range(max_cycles):
             self.replay()
             r = self.norm.item()
@@ -659,8 +755,17 @@ class VCycleGraph:
                 raise FloatingPointError(f"V-cycle residual is not finite after {k + 1} cycles")
             r0 = r if r0 is None else r0
             if r <= tol * r0:
-                return k + 1
-        return max_cycles
+                return k + 1, r / r0, True
+            if patience is not None:
+                if best is None or r < best:
+                    best, best_k = r, k
+                    # the norm is that of the state entering this cycle: keep that state
+                    best_state = self.buf[1 - self.cur].clone() if k > 0 else None
+                elif k - best_k >= patience:
+                    if best_state is not None:
+                        self.buf[self.cur].copy_(best_state)
+                    return k + 1, best / r0, False
+        return max_cycles, (best if best is not None else r) / r0, False
 
 
 # ---------------------------------------------------------------------------

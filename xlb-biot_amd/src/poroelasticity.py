@@ -21,10 +21,12 @@ class BiotNonlinearForce:
     """``f_el -> darcy.gradient(elasticity.update_divergence(f_el), scale)``
     (poroelasticity.py:324), with a fused SoA form used inside the elasticity steps."""
 
-    def __init__(self, elasticity, darcy, scale):
+    def __init__(self, elasticity, darcy, scale, fused=True):
         self.elasticity = elasticity
         self.darcy = darcy
         self.scale = scale
+        # formed inside the elasticity sweeps from a carried divergence field
+        self.fused = fused
 
     def __call__(self, f_el):
         return self.darcy.gradient(self.elasticity.update_divergence(f_el), self.scale)
@@ -66,7 +68,12 @@ class LBMPoroelasticity(object):
         # extension: residual-driven inner loop (None keeps the reference's fixed count)
         self.elasticityTol = kwargs.get("elasticity_tol", None)
         self.elasticityMaxSteps = kwargs.get("elasticity_max_steps", 10000)
+        # stop the residual-driven loop when the residual has not improved for this many
+        # cycles (the reference V-cycle has no exact coarse solve and can stall or grow)
+        self.elasticityPatience = kwargs.get("elasticity_patience", None)
+        self.inner_residuals = []
         self.useGraphs = kwargs.get("use_graphs", True)
+        self.fusedCoupling = kwargs.get("fused_coupling", True)
         self._inner = None
         self.inner_counts = []
         self.mlups = None
@@ -228,7 +235,7 @@ class LBMPoroelasticity(object):
         force = da._gradient_soa(el._divergence_soa(fprev), s_nl, base=pgb, a=-a * eps, b=-1.0)
         if self.useDynamicForce:
             force = self._lincomb(1.0, force, 1.0, self._soa(self.get_elastic_force(timestep), 2))
-        nlf = BiotNonlinearForce(el, da, s_nl)
+        nlf = BiotNonlinearForce(el, da, s_nl, fused=self.fusedCoupling)
         fe = self._inner_solve(fe, force, nlf)
         ext = nlf.soa_add(fe, force)
         eta = el._new(2)
@@ -270,9 +277,9 @@ class LBMPoroelasticity(object):
                             break
             self.inner_counts.append(k + 1)
             return fe
-        if self._inner is None or self._inner_scale != nlf.scale:
+        if self._inner is None or self._inner_key != (nlf.scale, nlf.fused):
             self._inner = (el.vcycle_graph(force, nlf) if el.levels > 1 else _PlainInnerGraph(el, force, nlf))
-            self._inner_scale = nlf.scale
+            self._inner_key = (nlf.scale, nlf.fused)
         inner = self._inner
         inner.set_force(force)
         inner.load(fe)
@@ -280,7 +287,8 @@ class LBMPoroelasticity(object):
             inner.run(self.elasticitySteps)
             count = self.elasticitySteps
         else:
-            count = inner.run_to(tol, self.elasticityMaxSteps)
+            count, rel, _ = inner.run_to(tol, self.elasticityMaxSteps, self.elasticityPatience)
+            self.inner_residuals.append(rel)
         self.inner_counts.append(count)
         self.last_residual = getattr(inner, "norm", None)
         return inner.state.clone()
@@ -339,19 +347,31 @@ class LBMPoroelasticity(object):
 
 
 class _PlainInnerGraph:
-    """Single-grid inner iterations (nonlinear force + step) replayed from a captured
-    HIP graph of ``chunk`` steps; the remainder runs eagerly."""
+    """Single-grid inner iterations (coupling force + step) replayed from a captured
+    HIP graph of ``chunk`` steps; the remainder runs eagerly.  With the fused coupling
+    force the divergence of the state is carried in ``da`` / ``db``."""
 
     def __init__(self, el, force, nlf, chunk=16):
         self.el, self.nlf, self.chunk = el, nlf, chunk
+        self.fused = el._fusable(nlf)
         self.ext = force.clone()
         self.a = torch.zeros((8, el.nx, el.ny), dtype=el._tdt, device=el.device)
         self.b = torch.empty_like(self.a)
-        launch = lambda s, d: el._step_into(s, d, nlf.soa_add(s, self.ext))
-        launch(self.a, self.b)
+        self.da = torch.zeros((1, el.nx, el.ny), dtype=el._tdt, device=el.device)
+        self.db = torch.empty_like(self.da)
+        self._one(self.a, self.da, self.b, self.db)
         torch.cuda.synchronize()
-        from .base import GraphedSteps
-        self.graph = GraphedSteps(launch, self.a, self.b, chunk)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for _ in range(chunk // 2):
+                self._one(self.a, self.da, self.b, self.db)
+                self._one(self.b, self.db, self.a, self.da)
+
+    def _one(self, s, ds, d, dd):
+        if self.fused:
+            self.el._step_nl(s, ds, 0, self.ext, self.nlf, out=d, out_div=dd)
+        else:
+            self.el._step_into(s, d, self.nlf.soa_add(s, self.ext))
 
     @property
     def state(self):
@@ -359,6 +379,11 @@ class _PlainInnerGraph:
 
     def load(self, f):
         self.a.copy_(f)
+        if self.fused:
+            el = self.el
+            nat.check(nat.lib().xb_elast_divergence(el.nx, el.ny, nat.dtype_code(f.dtype), el.lamda, el.mu,
+                                                    nat.ptr(self.a), None, nat.ptr(self.da), nat.stream()),
+                      "xb_elast_divergence")
 
     def set_force(self, force):
         self.ext.copy_(force)
@@ -369,8 +394,7 @@ class _PlainInnerGraph:
             self.graph.replay()
             done += self.chunk
         while done < n:
-            out = torch.empty_like(self.a)
-            self.el._step_into(self.a, out, self.nlf.soa_add(self.a, self.ext))
-            self.a.copy_(out)
+            self._one(self.a, self.da, self.b, self.db)
+            self.a.copy_(self.b)
+            self.da.copy_(self.db)
             done += 1
-
