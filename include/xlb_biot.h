@@ -200,6 +200,83 @@ int xb_sqdiff_partials(int64_t n, int32_t dtype, const void *a, const void *b,
 /* out[0] = sqrt(sum partials[0..n)) in a fixed order; out[1] = 1 if non-finite. */
 int xb_norm_finalize(const double *partials, int32_t n, double *out, void *stream);
 
+/* ---- batches: many independent grids per launch -------------------------------------
+ *
+ * The reference solves one problem per solver object; its parameter sweeps run the
+ * solvers one after another (the run scripts of the examples).  A batch puts
+ * one operation of many instances (grids of any sizes and parameters) into one
+ * launch.  A plan is prepared once (host-side job table copied into a device
+ * workspace the caller owns; this call synchronises the stream and must not be
+ * captured) and launched any number of times (capturable in a HIP graph).  Every
+ * launch takes an optional device array `active` of int32 flags indexed by the
+ * instance numbers of the jobs: the jobs of an instance whose flag is 0 do nothing. */
+enum {
+  XB_BATCH_ELAST_STEP = 1,  /* jobs: xb_elast_args[] (one xb_elast_step each)             */
+  XB_BATCH_PROLONG_DIV = 2, /* jobs: xb_prolong_job[] (xb_elast_prolong_add[_div] each)   */
+  XB_BATCH_FILL = 3,        /* jobs: xb_fill_job[] (xb_fill each)                         */
+  XB_BATCH_NORM = 4         /* jobs: xb_norm_job[] (for xb_batch_converge)                */
+};
+
+typedef struct xb_prolong_job {
+  int32_t nx, ny, cnx, cny;
+  int32_t ix0, ix1, iy0, iy1;
+  double lam, mu;           /* divergence factor of div                              */
+  void *f;                  /* (8, nx, ny)                                           */
+  const void *coarse;       /* (8, cnx, cny)                                         */
+  void *div;                /* optional (nx, ny): div(f) on the interior afterwards  */
+} xb_prolong_job;
+
+typedef struct xb_fill_job {
+  int64_t n;
+  void *out;
+  double value;
+} xb_fill_job;
+
+typedef struct xb_norm_job {
+  const double *partials;   /* residual partials of the instance's finest level      */
+  int32_t count;            /* xb_elast_partials_count() of that residual sweep      */
+  int32_t pad_;
+} xb_norm_job;
+
+typedef struct xb_batch_plan { /* filled by xb_batch_prepare                          */
+  int32_t kind, dtype, mode, flags;
+  int32_t njobs, nblocks;
+  void *table;
+  int64_t table_bytes;
+  int64_t jobs_offset, blocks_offset;
+} xb_batch_plan;
+
+/* Per-instance convergence state (device arrays of ninst entries, nactive one int).
+ * Before a solve: r0 < 0, cycles 0, active 1 (or 0 to leave an instance out). */
+typedef struct xb_converge_state {
+  double *r0;               /* residual norm of the first cycle                      */
+  double *rlast;            /* residual norm of the last cycle                       */
+  int32_t *cycles;          /* cycles done                                           */
+  int32_t *active;          /* 1 while iterating                                     */
+  int32_t *status;          /* 0 running, 1 converged, 2 out of cycles, 3 non-finite */
+  int32_t *nactive;         /* number of active instances after the update           */
+} xb_converge_state;
+
+/* Bytes of device workspace a plan of these jobs needs (-1 on bad arguments). */
+int64_t xb_batch_table_bytes(int32_t kind, int32_t dtype, const void *jobs, int32_t njobs);
+
+/* Validate the jobs (as the single-grid calls do) and upload their table.  inst[k]
+ * is job k's instance number (null: k).  Elasticity jobs must share dtype, mode and
+ * the presence of bc, resid and div_in. */
+int xb_batch_prepare(int32_t kind, int32_t dtype, const void *jobs, const int32_t *inst, int32_t njobs,
+                     void *table, int64_t table_bytes, xb_batch_plan *plan, void *stream);
+
+/* One launch of every job of the plan (kinds ELAST_STEP, PROLONG_DIV, FILL). */
+int xb_batch_launch(const xb_batch_plan *plan, const int32_t *active, void *stream);
+
+/* Convergence update after one V-cycle of every active instance (plan of kind NORM):
+ * r = sqrt(sum of its partials) (same order as xb_norm_finalize); cycles += 1;
+ * the first cycle sets r0; the instance stops when r <= tol * r0 (status 1), when
+ * cycles reaches max_cycles (2), or when r is not finite (3). Replaces the
+ * host-side test of the residual-driven inner loop of the Biot step. */
+int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converge_state *state, double tol,
+                      int32_t max_cycles, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

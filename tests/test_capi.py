@@ -1,5 +1,6 @@
 """The C-ABI library builds, loads without a GPU and exports exactly what
 ``include/xlb_biot.h`` declares; host-side validation of the solver API."""
+import ctypes
 import os
 import re
 
@@ -31,11 +32,39 @@ def test_library_exports_every_declared_symbol():
     assert lib.xb_abi_version() == 1
 
 
+STRUCTS = (("xb_elast_args", nat.ElastArgs), ("xb_darcy_args", nat.DarcyArgs), ("xb_bc_table", nat.BCTable),
+           ("xb_prolong_job", nat.ProlongJob), ("xb_fill_job", nat.FillJob), ("xb_norm_job", nat.NormJob),
+           ("xb_batch_plan", nat.BatchPlan), ("xb_converge_state", nat.ConvergeState))
+
+
+def test_struct_sizes_and_offsets_match_the_c_compiler(tmp_path):
+    # sizeof / offsetof of every struct as gcc lays it out, against the ctypes mirrors
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "xlb_biot.h"', "int main(void) {"]
+    for struct, cls in STRUCTS:
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (struct, struct))
+        for name, _ in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (struct, name, struct, name))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for struct, cls in STRUCTS:
+        assert int(out[struct]) == ctypes.sizeof(cls), struct
+        for name, _ in cls._fields_:
+            assert int(out[f"{struct}.{name}"]) == getattr(cls, name).offset, (struct, name)
+
+
 def test_struct_layouts_match_header():
     # the ctypes mirrors of the argument structs follow the header field by field
     text = open(os.path.join(ROOT, "include", "xlb_biot.h")).read()
-    for struct, cls in (("xb_elast_args", nat.ElastArgs), ("xb_darcy_args", nat.DarcyArgs),
-                        ("xb_bc_table", nat.BCTable)):
+    for struct, cls in STRUCTS:
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, flags=re.S).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         fields = []

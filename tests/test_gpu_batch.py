@@ -1,0 +1,110 @@
+"""Batched solves against the same instances solved one after another.
+
+A batched V-cycle runs, per instance, the kernels of the single-instance V-cycle
+with the same operands and launch shapes, and the device-side convergence test
+sums the same partials in the same order as the host-side loop; the states are
+therefore compared bit for bit, and the cycle counts exactly."""
+import numpy as np
+import pytest
+import torch
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _elastic(n, lam, mu, levels):
+    return P.periodic_elasticity("gpu", nx=n, lam=lam, mu=mu, levels=levels)
+
+
+CASES = [(32, 0.5, 0.5, 2), (64, 0.5, 0.5, 3), (48, 2.0, 1.0, 2), (32, 1.0, 1.0, 2)]
+
+
+def _sequential(els, tol, max_cycles):
+    out, cycles, reached = [], [], []
+    for el in els:
+        vg = el.vcycle_graph()
+        vg.load(torch.zeros((8, el.nx, el.ny), dtype=torch.float64, device="cuda"))
+        c, _, ok = vg.run_to(tol, max_cycles)
+        out.append(vg.state.clone())
+        cycles.append(c)
+        reached.append(ok)
+    return out, cycles, reached
+
+
+def test_elasticity_batch_matches_sequential_bitwise():
+    from src.batch import ElasticityBatch
+    say("elasticity batch vs sequential", CASES)
+    els = [_elastic(*c) for c in CASES]
+    ref, ref_cycles, reached = _sequential(els, 1e-6, 400)
+    assert all(reached)
+    eb = ElasticityBatch([_elastic(*c) for c in CASES], tol=1e-6, max_cycles=400, poll=3, external_force=False)
+    zeros = [torch.zeros((8, c[0], c[0]), dtype=torch.float64, device="cuda") for c in CASES]
+    got = eb.solve(zeros)
+    assert list(eb.last_cycles) == ref_cycles
+    assert list(eb.last_status) == [1 if ok else 2 for ok in reached]
+    assert all(r <= 1e-6 for r in eb.last_rel)
+    for i, (a, b) in enumerate(zip(got, ref)):
+        assert torch.equal(a, b), (i, (a - b).abs().max().item())
+    # a second solve from the converged states reuses the captured graphs
+    got2 = eb.solve([g.clone() for g in got])
+    ref2 = []
+    for el, r in zip(els, ref):
+        vg = el.vcycle_graph()
+        vg.load(r)
+        vg.run_to(1e-6, 400)
+        ref2.append(vg.state.clone())
+    for i, (a, b) in enumerate(zip(got2, ref2)):
+        assert torch.equal(a, b), ("second solve", i)
+
+
+def test_elasticity_batch_masks_and_cycle_limit():
+    from src.batch import ElasticityBatch
+    say("elasticity batch: inactive instance and cycle limit")
+    cases = CASES[:3]
+    eb = ElasticityBatch([_elastic(*c) for c in cases], tol=0.0, max_cycles=5, poll=2, external_force=False)
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    init = [1e-3 * torch.randn((8, c[0], c[0]), generator=gen, dtype=torch.float64).cuda() for c in cases]
+    got = eb.solve([t.clone() for t in init], active=[1, 0, 1])
+    assert list(eb.last_cycles) == [5, 0, 5]
+    assert list(eb.last_status) == [2, 0, 2]
+    assert torch.equal(got[1], init[1])
+    for i in (0, 2):
+        el = _elastic(*cases[i])
+        vg = el.vcycle_graph()
+        vg.load(init[i])
+        vg.run(5)
+        assert torch.equal(got[i], vg.state), i
+
+
+def _biot(n, lam, mu, levels, K=1.0):
+    p = P.biot_periodic("gpu", nx=n, levels=levels, lam=lam, mu=mu, K=K)
+    p.elasticityTol = 1e-6
+    p.elasticityMaxSteps = 400
+    return p
+
+
+BIOT = [(32, 0.5, 0.5, 2, 1.0), (64, 0.2444, 0.0306, 3, 0.1), (32, 2.0, 1.0, 2, 0.5)]
+
+
+def test_biot_batch_matches_sequential_bitwise():
+    from src.batch import BiotBatch
+    say("biot batch vs sequential", BIOT)
+    seq = [_biot(*c) for c in BIOT]
+    states = [p.initial_state() for p in seq]
+    bb = BiotBatch([_biot(*c) for c in BIOT], poll=3)
+    bstates = bb.initial_state()
+    for t in range(2):
+        ref = [p._step_soa(fe, fd, dde, t) for p, (fe, fd, dde) in zip(seq, states)]
+        got = bb.step(bstates, t)
+        for i in range(len(BIOT)):
+            assert bb.problems[i].inner_counts[-1] == seq[i].inner_counts[-1], (t, i)
+            assert seq[i].inner_counts[-1] > 1
+            for k, (a, b) in enumerate(zip(got[i], ref[i])):
+                assert torch.equal(a, b), (t, i, k, (a - b).abs().max().item())
+        states = [(o[0], o[1], o[4]) for o in ref]
+        bstates = [(o[0], o[1], o[4]) for o in got]

@@ -16,6 +16,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <type_traits>
+#include <vector>
 
 #include "xb_device.h"
 
@@ -70,15 +72,16 @@ SweepGeom make_geom(int nx, int ny, int ox, int oy, int ex, int ey, int ix0, int
 }
 
 // ---------------------------------------------------------------------------
+// One wavefront item of the sweep: a strip of 62 cells along y walked down `g.rows`
+// rows.  Shared by the single-grid kernel and the batched kernel, so both do the
+// same arithmetic in the same order.
 template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES, bool HAS_NL>
-__global__ void __launch_bounds__(BLOCK)
-k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
-        T *__restrict__ fpc, const T *__restrict__ res, const BCDev<T> bc, const T omg, const T gamma,
-        T *__restrict__ coarse, double *__restrict__ partials, const NLDev<T> nl) {
+__device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll, const T *__restrict__ fin,
+                                           T *__restrict__ fout, T *__restrict__ fpc, const T *__restrict__ res,
+                                           const BCDev<T> &bc, const T omg, const T gamma, T *__restrict__ coarse,
+                                           double *__restrict__ partials, const NLDev<T> &nl, const int item) {
   constexpr int Q = Coll::Q;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int item = xcd_block_id() * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
-  if (item >= g.nitems) return; // uniform per wavefront
 
   const int strip = item % g.nstrips;
   const int rb = item / g.nstrips;
@@ -273,6 +276,54 @@ k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__rest
     for (int off = WAVE / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off, WAVE);
     if (lane == 0) partials[item] = nrm;
   }
+}
+
+template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES, bool HAS_NL>
+__global__ void __launch_bounds__(BLOCK)
+k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
+        T *__restrict__ fpc, const T *__restrict__ res, const BCDev<T> bc, const T omg, const T gamma,
+        T *__restrict__ coarse, double *__restrict__ partials, const NLDev<T> nl) {
+  const int item = xcd_block_id() * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
+  if (item >= g.nitems) return; // uniform per wavefront
+  sweep_item<T, Coll, MODE, HAS_BC, HAS_RES, HAS_NL>(g, coll, fin, fout, fpc, res, bc, omg, gamma, coarse, partials,
+                                                     nl, item);
+}
+
+// ---------------------------------------------------------------------------
+// Batched sweeps: one launch advances many independent grids.  Every grid (a
+// "job") owns a contiguous range of blocks; block_job maps a block to its job
+// (-1 for the padding that rounds the grid up to a multiple of the 8 XCDs).  A job
+// whose instance is inactive (active[inst] == 0) returns at once, so converged
+// instances cost one scalar load per block.
+template <typename T, class Coll> struct SweepJob {
+  SweepGeom g;
+  Coll coll;
+  BCDev<T> bc;
+  NLDev<T> nl;
+  const T *fin;
+  T *fout;
+  T *fpc;
+  const T *res;
+  T *coarse;
+  double *partials;
+  T omg, gm;
+  int32_t inst;   // index of the instance's activity flag
+  int32_t block0; // first block of this job in the launch
+};
+
+template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES, bool HAS_NL>
+__global__ void __launch_bounds__(BLOCK)
+k_sweep_batch(const SweepJob<T, Coll> *__restrict__ jobs, const int32_t *__restrict__ block_job,
+              const int32_t *__restrict__ active) {
+  const int b = xcd_block_id();
+  const int j = block_job[b];
+  if (j < 0) return;
+  const SweepJob<T, Coll> J = jobs[j];
+  if (active != nullptr && active[J.inst] == 0) return;
+  const int item = (b - J.block0) * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
+  if (item >= J.g.nitems) return; // uniform per wavefront
+  sweep_item<T, Coll, MODE, HAS_BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
+                                                     J.coarse, J.partials, J.nl, item);
 }
 
 template <typename T, class Coll, int MODE>
@@ -525,12 +576,18 @@ __global__ void __launch_bounds__(256) k_sqdiff(int64_t n, const T *__restrict__
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-__global__ void k_norm_finalize(const double *__restrict__ partials, int n, double *__restrict__ out) {
-  const int lane = threadIdx.x;
+// Sum of p[0..n) by one wavefront in a fixed order (lane-strided, then a butterfly).
+__device__ __forceinline__ double wave_sum(const double *__restrict__ p, int n, int lane) {
   double s = 0.0;
-  for (int i = lane; i < n; i += WAVE) s += partials[i];
+  for (int i = lane; i < n; i += WAVE) s += p[i];
 #pragma unroll
   for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+  return s;
+}
+
+__global__ void k_norm_finalize(const double *__restrict__ partials, int n, double *__restrict__ out) {
+  const int lane = threadIdx.x;
+  const double s = wave_sum(partials, n, lane);
   if (lane == 0) {
     out[0] = sqrt(s);
     out[1] = (isfinite(s) ? 0.0 : 1.0);
@@ -573,8 +630,7 @@ int elast_geom(const xb_elast_args *a, SweepGeom &g) {
   return XB_OK;
 }
 
-template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st) {
-  ElastColl<T> c;
+template <typename T> void elast_coll(const xb_elast_args *a, ElastColl<T> &c, NLDev<T> &nl) {
   const double lm = a->lam + a->mu;
   double g = 1.0 / (12.0 * lm - 4.0);
   if (std::fabs(g) > 100.0) g = 0.0;
@@ -586,8 +642,7 @@ template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st) {
   c.fb = static_cast<const T *>(a->force_b);
   c.N = (size_t)a->nx * a->ny;
   c.has_b = a->force_b != nullptr || a->div_in != nullptr;
-  NLDev<T> nl;
-  const NLDev<T> *nlp = nullptr;
+  nl = NLDev<T>();
   if (a->div_in != nullptr) {
     nl.div_in = static_cast<const T *>(a->div_in);
     nl.div_out = static_cast<T *>(a->div_out);
@@ -595,8 +650,14 @@ template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st) {
     nl.kdiv = T(-1.5 / (3 * (a->lam + a->mu) + 1));
     nl.gbc = bcdev<T>(a->grad_bc);
     nl.base = static_cast<const T *>(a->force_b);
-    nlp = &nl;
   }
+}
+
+template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st) {
+  ElastColl<T> c;
+  NLDev<T> nl;
+  elast_coll<T>(a, c, nl);
+  const NLDev<T> *nlp = a->div_in != nullptr ? &nl : nullptr;
   SweepGeom geo;
   elast_geom(a, geo);
   switch (a->mode) {
@@ -609,6 +670,355 @@ template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st) {
   default:
     return launch_sweep<T, ElastColl<T>, XB_MODE_RESIDUAL>(geo, c, a->f_in, nullptr, nullptr, a->resid, a->bc, 1.0,
                                                            a->coarse_out, a->norm_partials, st, nlp);
+  }
+}
+
+// Argument checks of one elasticity sweep (xb_elast_step and every job of a batch).
+int validate_elast(const xb_elast_args *a, const char *who) {
+  if (a == nullptr) return fail(XB_EINVAL, "%s: null args", who);
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->nx <= 0 || a->ny <= 0) return fail(XB_EINVAL, "%s: bad grid %d x %d", who, a->nx, a->ny);
+  if (a->f_in == nullptr) return fail(XB_EINVAL, "%s: null f_in", who);
+  if (a->mode < XB_MODE_STEP || a->mode > XB_MODE_RESIDUAL) return fail(XB_EINVAL, "%s: bad mode %d", who, a->mode);
+  if (a->ix0 < 0 || a->ix1 > a->nx || a->iy0 < 0 || a->iy1 > a->ny || a->ix0 >= a->ix1 || a->iy0 >= a->iy1)
+    return fail(XB_EINVAL, "%s: bad interior box [%d,%d)x[%d,%d)", who, a->ix0, a->ix1, a->iy0, a->iy1);
+  if (a->mode != XB_MODE_RESIDUAL && a->f_out == nullptr) return fail(XB_EINVAL, "%s: null f_out", who);
+  if (a->mode != XB_MODE_STEP && a->fpc_out != nullptr)
+    return fail(XB_EINVAL, "%s: fpc_out is only produced by XB_MODE_STEP", who);
+  if (a->mode == XB_MODE_RESIDUAL) {
+    if (a->coarse_out == nullptr || a->norm_partials == nullptr)
+      return fail(XB_EINVAL, "%s: residual mode needs coarse_out and norm_partials", who);
+    if (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1))
+      return fail(XB_EINVAL, "%s: interior box must have even extents for restriction", who);
+  }
+  if (a->rows != 0 && (a->rows < 2 || a->rows > 64 || (a->rows & (a->rows - 1))))
+    return fail(XB_EINVAL, "%s: rows must be 0 or a power of two in [2, 64]", who);
+  if (a->mode == XB_MODE_SMOOTH && !(a->gamma >= 0.0 && a->gamma <= 1.0))
+    return fail(XB_EINVAL, "%s: gamma must be in [0,1]", who);
+  if (int r = check_bc(a->bc, false)) return r;
+  if (a->div_in == nullptr && (a->div_out != nullptr || a->grad_bc != nullptr))
+    return fail(XB_EINVAL, "%s: div_out / grad_bc without div_in", who);
+  if (a->div_in != nullptr) {
+    if (int r = check_bc(a->grad_bc, true)) return r;
+  }
+  return XB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched element-wise kernels: as the sweep batch, a job owns `nblocks`
+// consecutive blocks and walks its cells with a block-strided loop.
+template <typename T> struct ProlongJob {
+  int nx, ny, cnx, cny, ix0, ix1, iy0, iy1;
+  T k;     // divergence factor (used when div is set)
+  T *f;
+  const T *c;
+  T *div;
+  int32_t inst, block0, nblocks;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_prolong_batch(const ProlongJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
+                const int32_t *__restrict__ active) {
+  const int b = blockIdx.x;
+  const int j = block_job[b];
+  if (j < 0) return;
+  const ProlongJob<T> J = jobs[j];
+  if (active != nullptr && active[J.inst] == 0) return;
+  const int nxi = J.ix1 - J.ix0, nyi = J.iy1 - J.iy0;
+  const int64_t n = (int64_t)nxi * nyi;
+  const size_t N = (size_t)J.nx * J.ny, Nc = (size_t)J.cnx * J.cny;
+  for (int64_t t = (int64_t)(b - J.block0) * 256 + threadIdx.x; t < n; t += (int64_t)J.nblocks * 256) {
+    const int xi = (int)(t / nyi), yi = (int)(t % nyi);
+    const size_t cell = (size_t)(J.ix0 + xi) * J.ny + (J.iy0 + yi);
+    const size_t cc = (size_t)(J.ix0 + (xi >> 1)) * J.cny + (J.iy0 + (yi >> 1));
+    T v[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      v[d] = J.f[d * N + cell] + J.c[d * Nc + cc];
+      J.f[d * N + cell] = v[d];
+    }
+    if (J.div != nullptr) J.div[cell] = elast_div_of<T>(v, J.k);
+  }
+}
+
+template <typename T> struct FillJob {
+  T *out;
+  int64_t n;
+  T value;
+  int32_t inst, block0, nblocks;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_fill_batch(const FillJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
+             const int32_t *__restrict__ active) {
+  const int b = blockIdx.x;
+  const int j = block_job[b];
+  if (j < 0) return;
+  const FillJob<T> J = jobs[j];
+  if (active != nullptr && active[J.inst] == 0) return;
+  for (int64_t i = (int64_t)(b - J.block0) * 256 + threadIdx.x; i < J.n; i += (int64_t)J.nblocks * 256)
+    J.out[i] = J.value;
+}
+
+struct NormJob {
+  const double *partials;
+  int32_t count;
+  int32_t inst;
+};
+
+// Per-instance convergence test after a batched V-cycle: one wavefront per job sums
+// that instance's residual partials (the order of k_norm_finalize), then updates
+// its state and clears its activity flag once converged (r <= tol * r0), out of
+// cycles, or non-finite.  One block; thread 0 finally counts the active instances.
+__global__ void __launch_bounds__(1024)
+k_batch_converge(const NormJob *__restrict__ jobs, int njobs, int ninst, double tol, int max_cycles,
+                 double *__restrict__ r0, double *__restrict__ rlast, int32_t *__restrict__ cycles,
+                 int32_t *__restrict__ active, int32_t *__restrict__ status, int32_t *__restrict__ nactive) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (int j = wave; j < njobs; j += nwaves) {
+    const NormJob J = jobs[j];
+    if (active[J.inst] == 0) continue; // uniform per wavefront
+    const double s = wave_sum(J.partials, J.count, lane);
+    if (lane == 0) {
+      const int i = J.inst;
+      const double r = sqrt(s);
+      const int c = cycles[i] + 1;
+      cycles[i] = c;
+      rlast[i] = r;
+      if (!isfinite(s)) {
+        status[i] = 3;
+        active[i] = 0;
+      } else {
+        if (r0[i] < 0.0) r0[i] = r;
+        if (r <= tol * r0[i]) {
+          status[i] = 1;
+          active[i] = 0;
+        } else if (c >= max_cycles) {
+          status[i] = 2;
+          active[i] = 0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int i = 0; i < ninst; ++i) n += active[i] != 0;
+    *nactive = n;
+  }
+}
+
+// Host side of the batches: job tables are built here and copied into the
+// caller's device workspace: [jobs, padded to 256 B][block_job int32 per block].
+inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+struct BlockTable {
+  std::vector<int32_t> block_job;
+  void add(int job, int nblocks) { block_job.insert(block_job.end(), (size_t)nblocks, job); }
+  void pad_xcd() {
+    while (block_job.size() % 8) block_job.push_back(-1);
+  }
+};
+
+template <typename T> int build_sweep_jobs(const xb_elast_args *args, const int32_t *inst, int32_t n,
+                                           std::vector<SweepJob<T, ElastColl<T>>> &jobs, BlockTable &bt,
+                                           int &flags) {
+  for (int32_t k = 0; k < n; ++k) {
+    const xb_elast_args *a = &args[k];
+    if (int r = validate_elast(a, "xb_batch_prepare(elasticity job)")) return r;
+    if (a->dtype != args[0].dtype || a->mode != args[0].mode)
+      return fail(XB_EINVAL, "xb_batch_prepare: job %d: dtype and mode must agree across the batch", k);
+    const int f = (a->bc != nullptr && a->bc->cell_slot != nullptr ? 1 : 0) | (a->resid != nullptr ? 2 : 0) |
+                  (a->div_in != nullptr ? 4 : 0);
+    if (k == 0) flags = f;
+    else if (f != flags)
+      return fail(XB_EINVAL, "xb_batch_prepare: job %d: boundary table / residual / coupling presence must agree "
+                  "across the batch", k);
+    SweepJob<T, ElastColl<T>> J{};
+    elast_geom(a, J.g);
+    elast_coll<T>(a, J.coll, J.nl);
+    J.bc = bcdev<T>(a->bc);
+    J.fin = static_cast<const T *>(a->f_in);
+    J.fout = static_cast<T *>(a->f_out);
+    J.fpc = static_cast<T *>(a->fpc_out);
+    J.res = static_cast<const T *>(a->resid);
+    J.coarse = static_cast<T *>(a->coarse_out);
+    J.partials = a->norm_partials;
+    const double gamma = a->mode == XB_MODE_SMOOTH ? a->gamma : 1.0;
+    J.omg = T(1.0 - gamma);
+    J.gm = T(gamma);
+    J.inst = inst != nullptr ? inst[k] : k;
+    J.block0 = (int32_t)bt.block_job.size();
+    bt.add(k, cdiv(J.g.nitems, WAVES_PER_BLOCK));
+    jobs.push_back(J);
+  }
+  bt.pad_xcd();
+  return XB_OK;
+}
+
+int ew_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65535 * 8)); }
+
+template <typename T> int build_prolong_jobs(const xb_prolong_job *args, const int32_t *inst, int32_t n,
+                                             std::vector<ProlongJob<T>> &jobs, BlockTable &bt) {
+  for (int32_t k = 0; k < n; ++k) {
+    const xb_prolong_job *a = &args[k];
+    if (a->f == nullptr || a->coarse == nullptr) return fail(XB_EINVAL, "xb_batch_prepare: prolong job %d: null pointer", k);
+    if (a->ix0 < 0 || a->ix1 > a->nx || a->iy0 < 0 || a->iy1 > a->ny || a->ix0 >= a->ix1 || a->iy0 >= a->iy1 ||
+        ((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1))
+      return fail(XB_EINVAL, "xb_batch_prepare: prolong job %d: bad interior box", k);
+    if (a->cnx != a->ix0 + (a->ix1 - a->ix0) / 2 + (a->nx - a->ix1) ||
+        a->cny != a->iy0 + (a->iy1 - a->iy0) / 2 + (a->ny - a->iy1))
+      return fail(XB_EINVAL, "xb_batch_prepare: prolong job %d: coarse grid does not match the fine box", k);
+    ProlongJob<T> J{};
+    J.nx = a->nx; J.ny = a->ny; J.cnx = a->cnx; J.cny = a->cny;
+    J.ix0 = a->ix0; J.ix1 = a->ix1; J.iy0 = a->iy0; J.iy1 = a->iy1;
+    J.k = T(-1.5 / (3 * (a->lam + a->mu) + 1));
+    J.f = static_cast<T *>(a->f);
+    J.c = static_cast<const T *>(a->coarse);
+    J.div = static_cast<T *>(a->div);
+    J.inst = inst != nullptr ? inst[k] : k;
+    J.block0 = (int32_t)bt.block_job.size();
+    J.nblocks = ew_blocks((int64_t)(a->ix1 - a->ix0) * (a->iy1 - a->iy0));
+    bt.add(k, J.nblocks);
+    jobs.push_back(J);
+  }
+  return XB_OK;
+}
+
+template <typename T> int build_fill_jobs(const xb_fill_job *args, const int32_t *inst, int32_t n,
+                                          std::vector<FillJob<T>> &jobs, BlockTable &bt) {
+  for (int32_t k = 0; k < n; ++k) {
+    const xb_fill_job *a = &args[k];
+    if (a->n < 0 || (a->n > 0 && a->out == nullptr)) return fail(XB_EINVAL, "xb_batch_prepare: fill job %d: bad arguments", k);
+    FillJob<T> J{};
+    J.out = static_cast<T *>(a->out);
+    J.n = a->n;
+    J.value = T(a->value);
+    J.inst = inst != nullptr ? inst[k] : k;
+    J.block0 = (int32_t)bt.block_job.size();
+    J.nblocks = a->n > 0 ? ew_blocks(a->n) : 0;
+    bt.add(k, J.nblocks);
+    jobs.push_back(J);
+  }
+  return XB_OK;
+}
+
+int build_norm_jobs(const xb_norm_job *args, const int32_t *inst, int32_t n, std::vector<NormJob> &jobs) {
+  for (int32_t k = 0; k < n; ++k) {
+    if (args[k].partials == nullptr || args[k].count <= 0)
+      return fail(XB_EINVAL, "xb_batch_prepare: norm job %d: bad arguments", k);
+    jobs.push_back(NormJob{args[k].partials, args[k].count, inst != nullptr ? inst[k] : k});
+  }
+  return XB_OK;
+}
+
+// Table size and, when `table` is set, the upload of the tables.
+template <typename J>
+int finish_table(const std::vector<J> &jobs, const BlockTable &bt, int32_t kind, int32_t dtype, int32_t mode,
+                 int32_t flags, void *table, int64_t bytes, xb_batch_plan *plan, hipStream_t st, int64_t *need) {
+  const int64_t jb = align256((int64_t)(jobs.size() * sizeof(J)));
+  const int64_t total = jb + (int64_t)bt.block_job.size() * 4;
+  if (need != nullptr) {
+    *need = std::max<int64_t>(total, 256);
+    return XB_OK;
+  }
+  if (table == nullptr || bytes < total)
+    return fail(XB_EINVAL, "xb_batch_prepare: table of %lld bytes, %lld needed", (long long)bytes, (long long)total);
+  if (!jobs.empty()) {
+    if (hipMemcpyAsync(table, jobs.data(), jobs.size() * sizeof(J), hipMemcpyHostToDevice, st) != hipSuccess)
+      return fail(XB_EHIP, "xb_batch_prepare: job table upload failed");
+  }
+  if (!bt.block_job.empty()) {
+    if (hipMemcpyAsync(static_cast<char *>(table) + jb, bt.block_job.data(), bt.block_job.size() * 4,
+                       hipMemcpyHostToDevice, st) != hipSuccess)
+      return fail(XB_EHIP, "xb_batch_prepare: block table upload failed");
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return fail(XB_EHIP, "xb_batch_prepare: synchronisation failed");
+  plan->kind = kind;
+  plan->dtype = dtype;
+  plan->mode = mode;
+  plan->flags = flags;
+  plan->njobs = (int32_t)jobs.size();
+  plan->nblocks = (int32_t)bt.block_job.size();
+  plan->table = table;
+  plan->table_bytes = bytes;
+  plan->jobs_offset = 0;
+  plan->blocks_offset = jb;
+  return XB_OK;
+}
+
+template <typename T>
+int batch_prepare_t(int32_t kind, const void *jobs, const int32_t *inst, int32_t n, void *table, int64_t bytes,
+                    xb_batch_plan *plan, hipStream_t st, int64_t *need) {
+  BlockTable bt;
+  const int32_t dt = std::is_same<T, double>::value ? XB_F64 : XB_F32;
+  switch (kind) {
+  case XB_BATCH_ELAST_STEP: {
+    std::vector<SweepJob<T, ElastColl<T>>> v;
+    int flags = 0;
+    const xb_elast_args *a = static_cast<const xb_elast_args *>(jobs);
+    if (int r = build_sweep_jobs<T>(a, inst, n, v, bt, flags)) return r;
+    return finish_table(v, bt, kind, dt, a[0].mode, flags, table, bytes, plan, st, need);
+  }
+  case XB_BATCH_PROLONG_DIV: {
+    std::vector<ProlongJob<T>> v;
+    if (int r = build_prolong_jobs<T>(static_cast<const xb_prolong_job *>(jobs), inst, n, v, bt)) return r;
+    return finish_table(v, bt, kind, dt, 0, 0, table, bytes, plan, st, need);
+  }
+  case XB_BATCH_FILL: {
+    std::vector<FillJob<T>> v;
+    if (int r = build_fill_jobs<T>(static_cast<const xb_fill_job *>(jobs), inst, n, v, bt)) return r;
+    return finish_table(v, bt, kind, dt, 0, 0, table, bytes, plan, st, need);
+  }
+  default: {
+    std::vector<NormJob> v;
+    if (int r = build_norm_jobs(static_cast<const xb_norm_job *>(jobs), inst, n, v)) return r;
+    return finish_table(v, bt, kind, dt, 0, 0, table, bytes, plan, st, need);
+  }
+  }
+}
+
+template <typename T, int MODE>
+int launch_elast_batch(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
+  using J = SweepJob<T, ElastColl<T>>;
+  const J *jobs = reinterpret_cast<const J *>(static_cast<const char *>(p->table) + p->jobs_offset);
+  const int32_t *bj = reinterpret_cast<const int32_t *>(static_cast<const char *>(p->table) + p->blocks_offset);
+  const dim3 grid(p->nblocks), block(BLOCK);
+#define XB_SB(BC, RES, NL) \
+  hipLaunchKernelGGL((k_sweep_batch<T, ElastColl<T>, MODE, BC, RES, NL>), grid, block, 0, st, jobs, bj, active)
+  switch (p->flags) {
+  case 0: XB_SB(false, false, false); break;
+  case 1: XB_SB(true, false, false); break;
+  case 2: XB_SB(false, true, false); break;
+  case 3: XB_SB(true, true, false); break;
+  case 4: XB_SB(false, false, true); break;
+  case 5: XB_SB(true, false, true); break;
+  case 6: XB_SB(false, true, true); break;
+  default: XB_SB(true, true, true); break;
+  }
+#undef XB_SB
+  return check_launch("batched sweep kernel");
+}
+
+template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
+  const char *base = static_cast<const char *>(p->table);
+  const int32_t *bj = reinterpret_cast<const int32_t *>(base + p->blocks_offset);
+  switch (p->kind) {
+  case XB_BATCH_ELAST_STEP:
+    if (p->mode == XB_MODE_STEP) return launch_elast_batch<T, XB_MODE_STEP>(p, active, st);
+    if (p->mode == XB_MODE_SMOOTH) return launch_elast_batch<T, XB_MODE_SMOOTH>(p, active, st);
+    return launch_elast_batch<T, XB_MODE_RESIDUAL>(p, active, st);
+  case XB_BATCH_PROLONG_DIV:
+    hipLaunchKernelGGL(k_prolong_batch<T>, dim3(p->nblocks), dim3(256), 0, st,
+                       reinterpret_cast<const ProlongJob<T> *>(base + p->jobs_offset), bj, active);
+    return check_launch("batched prolongation kernel");
+  default:
+    hipLaunchKernelGGL(k_fill_batch<T>, dim3(p->nblocks), dim3(256), 0, st,
+                       reinterpret_cast<const FillJob<T> *>(base + p->jobs_offset), bj, active);
+    return check_launch("batched fill kernel");
   }
 }
 
@@ -629,32 +1039,7 @@ int64_t xb_elast_partials_count(const xb_elast_args *a) {
 }
 
 int xb_elast_step(const xb_elast_args *a, void *stream) {
-  if (a == nullptr) return fail(XB_EINVAL, "xb_elast_step: null args");
-  if (int r = check_dtype(a->dtype)) return r;
-  if (a->nx <= 0 || a->ny <= 0) return fail(XB_EINVAL, "xb_elast_step: bad grid %d x %d", a->nx, a->ny);
-  if (a->f_in == nullptr) return fail(XB_EINVAL, "xb_elast_step: null f_in");
-  if (a->mode < XB_MODE_STEP || a->mode > XB_MODE_RESIDUAL) return fail(XB_EINVAL, "xb_elast_step: bad mode %d", a->mode);
-  if (a->ix0 < 0 || a->ix1 > a->nx || a->iy0 < 0 || a->iy1 > a->ny || a->ix0 >= a->ix1 || a->iy0 >= a->iy1)
-    return fail(XB_EINVAL, "xb_elast_step: bad interior box [%d,%d)x[%d,%d)", a->ix0, a->ix1, a->iy0, a->iy1);
-  if (a->mode != XB_MODE_RESIDUAL && a->f_out == nullptr) return fail(XB_EINVAL, "xb_elast_step: null f_out");
-  if (a->mode != XB_MODE_STEP && a->fpc_out != nullptr)
-    return fail(XB_EINVAL, "xb_elast_step: fpc_out is only produced by XB_MODE_STEP");
-  if (a->mode == XB_MODE_RESIDUAL) {
-    if (a->coarse_out == nullptr || a->norm_partials == nullptr)
-      return fail(XB_EINVAL, "xb_elast_step: residual mode needs coarse_out and norm_partials");
-    if (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1))
-      return fail(XB_EINVAL, "xb_elast_step: interior box must have even extents for restriction");
-  }
-  if (a->rows != 0 && (a->rows < 2 || a->rows > 64 || (a->rows & (a->rows - 1))))
-    return fail(XB_EINVAL, "xb_elast_step: rows must be 0 or a power of two in [2, 64]");
-  if (a->mode == XB_MODE_SMOOTH && !(a->gamma >= 0.0 && a->gamma <= 1.0))
-    return fail(XB_EINVAL, "xb_elast_step: gamma must be in [0,1]");
-  if (int r = check_bc(a->bc, false)) return r;
-  if (a->div_in == nullptr && (a->div_out != nullptr || a->grad_bc != nullptr))
-    return fail(XB_EINVAL, "xb_elast_step: div_out / grad_bc without div_in");
-  if (a->div_in != nullptr) {
-    if (int r = check_bc(a->grad_bc, true)) return r;
-  }
+  if (int r = validate_elast(a, "xb_elast_step")) return r;
   hipStream_t st = static_cast<hipStream_t>(stream);
   return a->dtype == XB_F64 ? elast_step_t<double>(a, st) : elast_step_t<float>(a, st);
 }
@@ -869,6 +1254,51 @@ int xb_norm_finalize(const double *partials, int32_t n, double *out, void *strea
   if (partials == nullptr || out == nullptr || n <= 0) return fail(XB_EINVAL, "xb_norm_finalize: bad arguments");
   hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(WAVE), 0, static_cast<hipStream_t>(stream), partials, n, out);
   return check_launch("xb_norm_finalize");
+}
+
+int64_t xb_batch_table_bytes(int32_t kind, int32_t dtype, const void *jobs, int32_t njobs) {
+  if (check_dtype(dtype) != XB_OK || jobs == nullptr || njobs <= 0) return -1;
+  if (kind < XB_BATCH_ELAST_STEP || kind > XB_BATCH_NORM) return -1;
+  int64_t need = -1;
+  const int r = dtype == XB_F64 ? batch_prepare_t<double>(kind, jobs, nullptr, njobs, nullptr, 0, nullptr, nullptr, &need)
+                                : batch_prepare_t<float>(kind, jobs, nullptr, njobs, nullptr, 0, nullptr, nullptr, &need);
+  return r == XB_OK ? need : -1;
+}
+
+int xb_batch_prepare(int32_t kind, int32_t dtype, const void *jobs, const int32_t *inst, int32_t njobs, void *table,
+                     int64_t table_bytes, xb_batch_plan *plan, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (jobs == nullptr || njobs <= 0 || plan == nullptr) return fail(XB_EINVAL, "xb_batch_prepare: bad arguments");
+  if (kind < XB_BATCH_ELAST_STEP || kind > XB_BATCH_NORM) return fail(XB_EINVAL, "xb_batch_prepare: bad kind %d", kind);
+  if (kind == XB_BATCH_ELAST_STEP && static_cast<const xb_elast_args *>(jobs)[0].dtype != dtype)
+    return fail(XB_EINVAL, "xb_batch_prepare: job dtype differs from the batch dtype");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return dtype == XB_F64 ? batch_prepare_t<double>(kind, jobs, inst, njobs, table, table_bytes, plan, st, nullptr)
+                         : batch_prepare_t<float>(kind, jobs, inst, njobs, table, table_bytes, plan, st, nullptr);
+}
+
+int xb_batch_launch(const xb_batch_plan *plan, const int32_t *active, void *stream) {
+  if (plan == nullptr || plan->table == nullptr) return fail(XB_EINVAL, "xb_batch_launch: no plan");
+  if (plan->kind < XB_BATCH_ELAST_STEP || plan->kind > XB_BATCH_FILL)
+    return fail(XB_EINVAL, "xb_batch_launch: plan kind %d is not launchable", plan->kind);
+  if (plan->nblocks == 0) return XB_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return plan->dtype == XB_F64 ? batch_launch_t<double>(plan, active, st) : batch_launch_t<float>(plan, active, st);
+}
+
+int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converge_state *state, double tol,
+                      int32_t max_cycles, void *stream) {
+  if (plan == nullptr || plan->kind != XB_BATCH_NORM || plan->table == nullptr)
+    return fail(XB_EINVAL, "xb_batch_converge: needs a norm plan");
+  if (state == nullptr || state->r0 == nullptr || state->rlast == nullptr || state->cycles == nullptr ||
+      state->active == nullptr || state->status == nullptr || state->nactive == nullptr)
+    return fail(XB_EINVAL, "xb_batch_converge: incomplete state");
+  if (ninst <= 0 || max_cycles <= 0 || !(tol >= 0.0)) return fail(XB_EINVAL, "xb_batch_converge: bad arguments");
+  const NormJob *jobs = reinterpret_cast<const NormJob *>(static_cast<const char *>(plan->table) + plan->jobs_offset);
+  hipLaunchKernelGGL(k_batch_converge, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs,
+                     ninst, tol, max_cycles, state->r0, state->rlast, state->cycles, state->active, state->status,
+                     state->nactive);
+  return check_launch("xb_batch_converge");
 }
 
 } // extern "C"

@@ -44,6 +44,34 @@ class DarcyArgs(ctypes.Structure):
                 ("rows", c_int32), ("pad2_", c_int32)]
 
 
+XB_BATCH_ELAST_STEP, XB_BATCH_PROLONG_DIV, XB_BATCH_FILL, XB_BATCH_NORM = 1, 2, 3, 4
+
+
+class ProlongJob(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("cnx", c_int32), ("cny", c_int32),
+                ("ix0", c_int32), ("ix1", c_int32), ("iy0", c_int32), ("iy1", c_int32),
+                ("lam", c_double), ("mu", c_double), ("f", c_void_p), ("coarse", c_void_p), ("div", c_void_p)]
+
+
+class FillJob(ctypes.Structure):
+    _fields_ = [("n", c_int64), ("out", c_void_p), ("value", c_double)]
+
+
+class NormJob(ctypes.Structure):
+    _fields_ = [("partials", c_void_p), ("count", c_int32), ("pad_", c_int32)]
+
+
+class BatchPlan(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("dtype", c_int32), ("mode", c_int32), ("flags", c_int32),
+                ("njobs", c_int32), ("nblocks", c_int32), ("table", c_void_p), ("table_bytes", c_int64),
+                ("jobs_offset", c_int64), ("blocks_offset", c_int64)]
+
+
+class ConvergeState(ctypes.Structure):
+    _fields_ = [("r0", c_void_p), ("rlast", c_void_p), ("cycles", c_void_p), ("active", c_void_p),
+                ("status", c_void_p), ("nactive", c_void_p)]
+
+
 # name -> (restype, argtypes); every exported symbol of include/xlb_biot.h
 SIGNATURES = {
     "xb_abi_version": (c_int32, []),
@@ -71,6 +99,12 @@ SIGNATURES = {
     "xb_sqdiff_partials": (c_int32, [c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
                                      ctypes.POINTER(c_int32), c_void_p]),
     "xb_norm_finalize": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
+    "xb_batch_table_bytes": (c_int64, [c_int32, c_int32, c_void_p, c_int32]),
+    "xb_batch_prepare": (c_int32, [c_int32, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_int64,
+                                   ctypes.POINTER(BatchPlan), c_void_p]),
+    "xb_batch_launch": (c_int32, [ctypes.POINTER(BatchPlan), c_void_p, c_void_p]),
+    "xb_batch_converge": (c_int32, [ctypes.POINTER(BatchPlan), c_int32, ctypes.POINTER(ConvergeState), c_double,
+                                    c_int32, c_void_p]),
 }
 
 _lib = None

@@ -219,9 +219,21 @@ class LBMPoroelasticity(object):
 
     # ---- coupled step (poroelasticity.py:303-339)
     def _step_soa(self, fe, fd, dde, timestep):
+        ctx = self._pre_inner(fe, fd, dde, timestep)
+        fe = self._inner_solve(fe, ctx["force"], ctx["nlf"])
+        return self._post_inner(ctx, fe)
+
+    def coupling_force(self):
+        """The solution-dependent part of the elastic force, ``-gradient(div f, s)``
+        with ``s = theta alpha^2 / 2`` (poroelasticity.py:324)."""
+        s_nl = 0.5 * self.couplingRatio * self.alpha ** 2
+        return BiotNonlinearForce(self.elasticity, self.darcy, s_nl, fused=self.fusedCoupling)
+
+    def _pre_inner(self, fe, fd, dde, timestep):
+        """Everything of the coupled step before the elasticity iterations
+        (poroelasticity.py:303-324): the static part of the elastic force."""
         el, da = self.elasticity, self.darcy
         th, a, eps = self.couplingRatio, self.alpha, self.epsilon
-        fprev = fe
         source_base = self._lincomb((1.0 - th) * -a / eps, dde)
         if self.useDynamicSource:
             source_base = self._lincomb(1.0, source_base, 1.0, self._soa(self.get_darcy_source(timestep), 1))
@@ -231,13 +243,19 @@ class LBMPoroelasticity(object):
         else:
             _, g = da._macro_soa(fd, None, False, True)
             pgb = da._gradient_soa(source_base, 0.5, base=g, a=1.0, b=1.0)
-        s_nl = 0.5 * th * a ** 2
-        force = da._gradient_soa(el._divergence_soa(fprev), s_nl, base=pgb, a=-a * eps, b=-1.0)
+        nlf = self.coupling_force()
+        force = da._gradient_soa(el._divergence_soa(fe), nlf.scale, base=pgb, a=-a * eps, b=-1.0)
         if self.useDynamicForce:
             force = self._lincomb(1.0, force, 1.0, self._soa(self.get_elastic_force(timestep), 2))
-        nlf = BiotNonlinearForce(el, da, s_nl, fused=self.fusedCoupling)
-        fe = self._inner_solve(fe, force, nlf)
-        ext = nlf.soa_add(fe, force)
+        return dict(fprev=fe, fd=fd, source_base=source_base, pgb=pgb, force=force, nlf=nlf)
+
+    def _post_inner(self, ctx, fe):
+        """The rest of the coupled step (poroelasticity.py:326-339) from the new
+        elastic populations: stresses, divergence rate, pressure, Darcy step."""
+        el, da = self.elasticity, self.darcy
+        th, a, eps = self.couplingRatio, self.alpha, self.epsilon
+        fprev, fd, source_base, pgb = ctx["fprev"], ctx["fd"], ctx["source_base"], ctx["pgb"]
+        ext = ctx["nlf"].soa_add(fe, ctx["force"])
         eta = el._new(2)
         sigma = el._new(3)
         nat.check(nat.lib().xb_elast_macro(el.nx, el.ny, nat.dtype_code(self._tdt), el.lamda, el.mu, nat.ptr(fe),
