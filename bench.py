@@ -192,6 +192,81 @@ def bench_c3(torch, n=256, levels=4, steps=20, tol=1e-8, lam=1.0, mu=1.0, K=1.0,
             "fine_cell_vcycles_per_s_millions": round(n * n / vc_s / 1e6, 1)}
 
 
+def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smooth=8):
+    """Batched Biot: ``n_inst`` periodic Biot instances (sizes from {64, 128, 256, 512},
+    lambda, mu ~ U[0.5, 2], log-permeability mean ~ U[-1, 1], log-normal K), FP64,
+    ``steps`` physical steps each, elasticity V-cycles to the inner relative residual
+    ``tol``.  The same instances run one after another (each with its captured
+    V-cycle graph and a host-side residual test per cycle) and as one batch
+    (batched launches, device-side convergence masks, captured batched cycles).
+    Each instance's hierarchy depth is tuned first (shortest predicted time to the
+    tolerance) and used by both runs.  The V-cycles smooth ``k_smooth`` = 8 times
+    per level (the reference's MG_n_smoothing): with the default 2 (or 4) the
+    reference V-cycle diverges for these Lame parameters."""
+    import collections
+    import problems as P
+    from src import autotune
+    from src.batch import BiotBatch
+    specs = P.c5_specs(n_inst, seed)
+    fields = [P.c5_permeability(s) for s in specs]
+    t0 = time.perf_counter()
+    levels = []
+    for s, K in zip(specs, fields):
+        n = s["n"]
+        cands = [L for L in range(2, 6) if (n >> (L - 1)) >= 16]
+        key = (f"mgdepth/biot_periodic/n{n}/lam{s['lam']:.6f}/mu{s['mu']:.6f}/logk{s['logk_mean']:.6f}"
+               f"/seed{s['seed']}/k{k_smooth}/tol{tol:g}")
+        L, _ = autotune.tune_levels(
+            key, lambda L, s=s, K=K: P.biot_c5("gpu", s, L, tol, max_cycles, K=K, k_smooth=k_smooth).probe_levels(),
+            cands, tol)
+        levels.append(L)
+    tune_s = time.perf_counter() - t0
+    log(f"  depths tuned in {tune_s:.1f} s: {dict(collections.Counter(levels))}")
+    seq = [P.biot_c5("gpu", s, L, tol, max_cycles, K=K, k_smooth=k_smooth) for s, L, K in zip(specs, levels, fields)]
+    for p in seq:
+        p.prepare_inner()
+    bat = BiotBatch([P.biot_c5("gpu", s, L, tol, max_cycles, K=K, k_smooth=k_smooth)
+                     for s, L, K in zip(specs, levels, fields)])
+    seq_states = [p.initial_state() for p in seq]
+    bat_states = bat.initial_state()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq_out = []
+    for p, st in zip(seq, seq_states):
+        for t in range(steps):
+            o = p._step_soa(*st, t)
+            st = (o[0], o[1], o[4])
+        seq_out.append(st)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t0
+    log(f"  sequential {seq_s:.2f} s")
+    t0 = time.perf_counter()
+    for t in range(steps):
+        outs = bat.step(bat_states, t)
+        bat_states = [(o[0], o[1], o[4]) for o in outs]
+    torch.cuda.synchronize()
+    bat_s = time.perf_counter() - t0
+    log(f"  batched {bat_s:.2f} s")
+    same = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]) for a, b in zip(seq_out, bat_states))
+    per_step = [[p.inner_counts[t] for p in seq] for t in range(steps)]
+    counts = [c for p in seq for c in p.inner_counts]
+    rels = [r for p in seq for r in p.inner_residuals]
+    fine_cycles = sum(s["n"] ** 2 * sum(p.inner_counts) for s, p in zip(specs, seq))
+    return {"instances": n_inst, "physical_steps_each": steps, "tol": tol, "k_smooth": k_smooth,
+            "sizes": {str(k): v for k, v in sorted(collections.Counter(s["n"] for s in specs).items())},
+            "levels": {str(k): v for k, v in sorted(collections.Counter(levels).items())},
+            "sequential_s": round(seq_s, 4), "batched_s": round(bat_s, 4), "speedup": round(seq_s / bat_s, 3),
+            "instances_per_s_sequential": round(n_inst / seq_s, 3),
+            "instances_per_s_batched": round(n_inst / bat_s, 3),
+            "vcycles_total": int(sum(counts)), "vcycles_max": int(max(counts)),
+            "vcycles_median": int(sorted(counts)[len(counts) // 2]),
+            "vcycles_per_physical_step": [int(sum(c)) for c in per_step],
+            "solves_reaching_tol": int(sum(1 for r in rels if r <= tol)), "solves": len(rels),
+            "batched_fine_cell_vcycles_per_s_millions": round(fine_cycles / bat_s / 1e6, 1),
+            "bitwise_equal_to_sequential": bool(same), "depth_tuning_s": round(tune_s, 2),
+            "batched_host_split_s": {k: round(v, 4) for k, v in bat.timing.items()}}
+
+
 def cpu_baseline(n, precision, seconds):
     import torch
     import problems as P
@@ -222,6 +297,8 @@ def only(torch, args):
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"copy: {args.steps} launches of {a.numel() * 8} B")
+    elif args.only == "c5":
+        print(json.dumps(bench_c5(torch, n_inst=args.instances)), flush=True)
     elif args.only == "large_elast":
         r = bench_large_elasticity(torch, n, args.steps, 2)
         log(f"elasticity {n}^2: {r['kernel_s'] * 1e6:.1f} us/launch, {r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
@@ -239,7 +316,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy"],
+    ap.add_argument("--instances", type=int, default=64, help="instances of the batched Biot extra")
+    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c5"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -273,18 +351,30 @@ def main():
                      "bytes_per_cell": 152, "cells": r["cells"]},
     }
     extras = {}
+
+    def extra(name, fn):
+        # an extra that fails is reported, and the headline line is still printed
+        try:
+            extras[name] = fn()
+        except Exception as e:  # noqa: BLE001
+            log(f"extra {name} failed: {type(e).__name__}: {e}")
+            extras[name] = {"error": f"{type(e).__name__}: {e}"}
+
     if not args.no_extras:
         log("darcy C2 fp32")
         r32 = bench_darcy(torch, args.n, "f32/f32", args.steps // 2, args.warmup // 2)
         extras["c2_fp32_mlups"] = round(r32["cells"] * (args.steps // 2) / r32["elapsed"] / 1e6, 1)
         extras["c2_fp32_kernel_gbs"] = round(r32["bytes"] / r32["kernel_s"] / 1e9, 1)
         log("C1 64^2 to 1e-10")
-        extras["c1_elasticity_64_fp64"] = bench_c1(torch)
+        extra("c1_elasticity_64_fp64", lambda: bench_c1(torch))
         log("C3 Biot 256^2 MG (4 levels, lambda = mu = 1), inner 1e-8")
-        extras["c3_biot_256_mg4_fp64"] = bench_c3(torch, steps=5, max_cycles=60)
+        extra("c3_biot_256_mg4_fp64", lambda: bench_c3(torch, steps=5, max_cycles=60))
         lam, mu = 0.11 * 0.8 / (1 - 0.8 ** 2), 0.5 * 0.11 / 1.8
         log("Biot 256^2 MG (8 levels, material of the reference periodic driver), inner 1e-8")
-        extras["c3_biot_256_mg8_ref_material_fp64"] = bench_c3(torch, levels=8, steps=10, lam=lam, mu=mu, K=0.1)
+        extra("c3_biot_256_mg8_ref_material_fp64",
+              lambda: bench_c3(torch, levels=8, steps=10, lam=lam, mu=mu, K=0.1))
+        log(f"batched Biot: {args.instances} instances vs one after another")
+        extra("c5_batched_biot_fp64", lambda: bench_c5(torch, n_inst=args.instances))
         for n in (4096,):
             log(f"elasticity {n}^2 fp64 (HBM-resident)")
             rl = bench_large_elasticity(torch, n, 200, 20)

@@ -255,6 +255,7 @@ typedef struct xb_converge_state {
   int32_t *active;          /* 1 while iterating                                     */
   int32_t *status;          /* 0 running, 1 converged, 2 out of cycles, 3 non-finite */
   int32_t *nactive;         /* number of active instances after the update           */
+  const double *rref;       /* optional: residual scale per instance (<= 0: use r0)   */
 } xb_converge_state;
 
 /* Bytes of device workspace a plan of these jobs needs (-1 on bad arguments). */
@@ -271,11 +272,17 @@ int xb_batch_launch(const xb_batch_plan *plan, const int32_t *active, void *stre
 
 /* Convergence update after one V-cycle of every active instance (plan of kind NORM):
  * r = sqrt(sum of its partials) (same order as xb_norm_finalize); cycles += 1;
- * the first cycle sets r0; the instance stops when r <= tol * r0 (status 1), when
+ * the first cycle sets r0; the instance stops when r <= tol * s (status 1), with
+ * s = rref[i] where given and positive, else r0; when
  * cycles reaches max_cycles (2), or when r is not finite (3). Replaces the
  * host-side test of the residual-driven inner loop of the Biot step. */
 int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converge_state *state, double tol,
                       int32_t max_cycles, void *stream);
+
+/* out[inst] = sqrt(sum of the job's partials) for every job of a NORM plan (same
+ * order as xb_norm_finalize).  With the partials of a RESIDUAL sweep of the zero
+ * state this is the residual scale |S(0)| of each instance. */
+int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -265,8 +265,12 @@ def darcy_equilibrium(kind, n=16, precision="f64/f64"):
 # Biot
 
 def biot_periodic(kind, nx=32, levels=1, steps=1, precision="f64/f64", coupling="centered", alt=False,
-                  lam=1.0, mu=1.0, alpha=1.0, K=1.0):
-    """examples/poroelasticity/periodic.py: manufactured time-dependent solution."""
+                  lam=1.0, mu=1.0, alpha=1.0, K=1.0, K_field=None, k_smooth=None, gamma=None, device_hooks=False):
+    """examples/poroelasticity/periodic.py: manufactured time-dependent solution.
+    ``K_field`` (nx, nx) gives a heterogeneous permeability (per-cell Darcy rate
+    1/(3K+1/2)); the manufactured forcing then uses the mean rate.  With
+    ``device_hooks`` the hooks keep the (time-independent) spatial patterns on the
+    GPU and return SoA-backed views scaled by the time factor."""
     m = modules(kind)
 
     class PDarcy(m["Darcy"]):
@@ -283,9 +287,24 @@ def biot_periodic(kind, nx=32, levels=1, steps=1, precision="f64/f64", coupling=
         def __init__(self, **kw):
             super().__init__(PEl, PDarcy, **kw)
 
+        def _pattern(self, name, make):
+            import torch
+            pats = self.__dict__.setdefault("_patterns", {})
+            if name not in pats:
+                pats[name] = torch.as_tensor(make()[..., 0], dtype=torch.float64, device="cuda")
+            return pats[name]
+
         def get_elastic_force(self, timestep):
             Kd = (1.0 / self.darcy.omega - 0.5) / 3
             tf = 1 - np.exp(-8 * np.pi ** 2 * Kd * timestep / self.elasticity.nx ** 2)
+            if device_hooks:
+                import torch
+                pat = self._pattern("force", lambda: sum_function(
+                    self.elasticity.nx, self.elasticity.ny, [[], [2, 2], [], []],
+                    lambda k, l: self.epsilon ** 2 * np.pi ** 2 * 8 * self.elasticity.mu))
+                out = torch.zeros((2,) + tuple(pat.shape), dtype=torch.float64, device="cuda")
+                out[0] = pat * tf
+                return out.permute(1, 2, 0)
             fx = sum_function(self.elasticity.nx, self.elasticity.ny, [[], [2, 2], [], []],
                               lambda k, l: self.epsilon ** 2 * tf * np.pi ** 2 * 8 * self.elasticity.mu)
             return np.concatenate((fx, np.zeros_like(fx)), -1)
@@ -295,13 +314,59 @@ def biot_periodic(kind, nx=32, levels=1, steps=1, precision="f64/f64", coupling=
             tf = np.exp(-8 * np.pi ** 2 * Kd * timestep / self.elasticity.nx ** 2)
             factor = -np.pi ** 3 * Kd / self.darcy.nx ** 2 * (
                 128 * self.alpha * tf + (128 * self.elasticity.lamda + 240 * self.elasticity.mu) / self.alpha)
+            if device_hooks:
+                pat = self._pattern("source", lambda: sum_function(self.darcy.nx, self.darcy.ny,
+                                                                   [[], [], [], [2, 2]], lambda k, l: 1.0))
+                return (pat * factor).unsqueeze(-1)
             return sum_function(self.darcy.nx, self.darcy.ny, [[], [], [], [2, 2]], lambda k, l: factor)
 
-    return Periodic(**{"lattice_elasticity": m["D2Q8"](precision), "lattice_darcy": m["D2Q9"](precision),
+    mg = {}
+    if k_smooth is not None:
+        mg["MG_n_smoothing"] = k_smooth
+    if gamma is not None:
+        mg["MG_relaxation"] = gamma
+    return Periodic(**mg, **{"lattice_elasticity": m["D2Q8"](precision), "lattice_darcy": m["D2Q9"](precision),
                        "elasticity_steps": steps, "elasticity_MG_levels": levels, "coupling_method": coupling,
                        "alternative_gradient": alt, "alpha": alpha, "lambda": lam, "mu": mu,
-                       "omega": 1.0 / (3 * K + 0.5), "nx": nx, "ny": nx, "nz": 0, "precision": precision,
-                       "print_info_rate": 0})
+                       "omega": (1.0 / (3 * K + 0.5) if K_field is None
+                                 else (1.0 / (3.0 * K_field + 0.5)).reshape(nx, nx, 1)),
+                       "nx": nx, "ny": nx, "nz": 0, "precision": precision, "print_info_rate": 0})
+
+
+def c5_specs(n_inst=64, seed=3):
+    """Batched Biot instances: square sizes drawn from {64, 128, 256, 512}, lambda and
+    mu ~ U[0.5, 2], log-permeability mean ~ U[-1, 1] (NumPy PCG64, one seed); each
+    instance's heterogeneity is a GRF of variance 1 and correlation length 16."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = rng.choice([64, 128, 256, 512], size=n_inst)
+    lam = rng.uniform(0.5, 2.0, n_inst)
+    mu = rng.uniform(0.5, 2.0, n_inst)
+    logk = rng.uniform(-1.0, 1.0, n_inst)
+    return [dict(n=int(sizes[i]), lam=float(lam[i]), mu=float(mu[i]), logk_mean=float(logk[i]),
+                 seed=1000 * seed + i) for i in range(n_inst)]
+
+
+def c5_permeability(spec):
+    return lognormal_field(spec["n"], spec["n"], 1.0, 16.0, spec["seed"], mean=spec["logk_mean"])
+
+
+def biot_c5(kind, spec, levels, tol=1e-8, max_cycles=20000, precision="f64/f64", K=None, k_smooth=None,
+            gamma=None):
+    """One batched-Biot instance: the periodic manufactured Biot problem with the
+    spec's size, Lame parameters and log-normal permeability, multigrid elasticity
+    iterated to ``tol`` (relative to the residual of the zero state) each physical
+    step.  The pressure force is the lattice gradient of p (the reference's
+    ``alternative_gradient``): with a heterogeneous permeability the default form
+    -3 omega(x) sum_i c_i f_i has a nonzero mean, a net force on the periodic body,
+    for which the elastic problem has no steady state."""
+    n = spec["n"]
+    K = c5_permeability(spec) if K is None else K
+    p = biot_periodic(kind, nx=n, levels=levels, lam=spec["lam"], mu=spec["mu"], precision=precision, K_field=K,
+                      k_smooth=k_smooth, gamma=gamma, device_hooks=kind == "gpu", alt=True)
+    p.elasticityTol = tol
+    p.elasticityMaxSteps = max_cycles
+    p.elasticityTolScale = "rhs"
+    return p
 
 
 def _corner_fix(Base, remove):

@@ -81,24 +81,26 @@ def test_elasticity_batch_masks_and_cycle_limit():
         assert torch.equal(got[i], vg.state), i
 
 
-def _biot(n, lam, mu, levels, K=1.0):
+def _biot(n, lam, mu, levels, K=1.0, scale="cycle"):
     p = P.biot_periodic("gpu", nx=n, levels=levels, lam=lam, mu=mu, K=K)
     p.elasticityTol = 1e-6
     p.elasticityMaxSteps = 400
+    p.elasticityTolScale = scale
     return p
 
 
 BIOT = [(32, 0.5, 0.5, 2, 1.0), (64, 0.2444, 0.0306, 3, 0.1), (32, 2.0, 1.0, 2, 0.5)]
 
 
-def test_biot_batch_matches_sequential_bitwise():
+@pytest.mark.parametrize("scale", ["cycle", "rhs"])
+def test_biot_batch_matches_sequential_bitwise(scale):
     from src.batch import BiotBatch
-    say("biot batch vs sequential", BIOT)
-    seq = [_biot(*c) for c in BIOT]
+    say("biot batch vs sequential", BIOT, scale)
+    seq = [_biot(*c, scale=scale) for c in BIOT]
     states = [p.initial_state() for p in seq]
-    bb = BiotBatch([_biot(*c) for c in BIOT], poll=3)
+    bb = BiotBatch([_biot(*c, scale=scale) for c in BIOT], poll=3)
     bstates = bb.initial_state()
-    for t in range(2):
+    for t in range(3):
         ref = [p._step_soa(fe, fd, dde, t) for p, (fe, fd, dde) in zip(seq, states)]
         got = bb.step(bstates, t)
         for i in range(len(BIOT)):
@@ -108,3 +110,11 @@ def test_biot_batch_matches_sequential_bitwise():
                 assert torch.equal(a, b), (t, i, k, (a - b).abs().max().item())
         states = [(o[0], o[1], o[4]) for o in ref]
         bstates = [(o[0], o[1], o[4]) for o in got]
+    if scale == "rhs":
+        # against the residual of the zero state, warm-started solves reach the
+        # tolerance (against their own first cycle they can stall at round-off);
+        # the third instance, mu = 1 with two smoothing steps, is not a contracting
+        # V-cycle in the long run, and its parity is checked above all the same
+        for i, p in enumerate(seq[:2]):
+            assert all(r <= 1e-6 for r in p.inner_residuals), (i, p.inner_residuals, p.inner_counts)
+        say("inner residuals", [p.inner_residuals for p in seq], "cycles", [p.inner_counts for p in seq])

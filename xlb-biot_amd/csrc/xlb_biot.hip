@@ -770,12 +770,13 @@ struct NormJob {
 
 // Per-instance convergence test after a batched V-cycle: one wavefront per job sums
 // that instance's residual partials (the order of k_norm_finalize), then updates
-// its state and clears its activity flag once converged (r <= tol * r0), out of
-// cycles, or non-finite.  One block; thread 0 finally counts the active instances.
+// its state and clears its activity flag once converged (r <= tol * s, s the
+// instance's residual scale if given, else r0), out of cycles, or non-finite.  One block; thread 0 finally counts the active instances.
 __global__ void __launch_bounds__(1024)
 k_batch_converge(const NormJob *__restrict__ jobs, int njobs, int ninst, double tol, int max_cycles,
                  double *__restrict__ r0, double *__restrict__ rlast, int32_t *__restrict__ cycles,
-                 int32_t *__restrict__ active, int32_t *__restrict__ status, int32_t *__restrict__ nactive) {
+                 int32_t *__restrict__ active, int32_t *__restrict__ status, int32_t *__restrict__ nactive,
+                 const double *__restrict__ rref) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   for (int j = wave; j < njobs; j += nwaves) {
@@ -793,7 +794,8 @@ k_batch_converge(const NormJob *__restrict__ jobs, int njobs, int ninst, double 
         active[i] = 0;
       } else {
         if (r0[i] < 0.0) r0[i] = r;
-        if (r <= tol * r0[i]) {
+        const double scale = (rref != nullptr && rref[i] > 0.0) ? rref[i] : r0[i];
+        if (r <= tol * scale) {
           status[i] = 1;
           active[i] = 0;
         } else if (c >= max_cycles) {
@@ -808,6 +810,18 @@ k_batch_converge(const NormJob *__restrict__ jobs, int njobs, int ninst, double 
     int n = 0;
     for (int i = 0; i < ninst; ++i) n += active[i] != 0;
     *nactive = n;
+  }
+}
+
+// out[inst] = sqrt(sum of the job's partials), one wavefront per job.
+__global__ void __launch_bounds__(1024)
+k_batch_norms(const NormJob *__restrict__ jobs, int njobs, double *__restrict__ out) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (int j = wave; j < njobs; j += nwaves) {
+    const NormJob J = jobs[j];
+    const double s = wave_sum(J.partials, J.count, lane);
+    if (lane == 0) out[J.inst] = sqrt(s);
   }
 }
 
@@ -1297,8 +1311,16 @@ int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converg
   const NormJob *jobs = reinterpret_cast<const NormJob *>(static_cast<const char *>(plan->table) + plan->jobs_offset);
   hipLaunchKernelGGL(k_batch_converge, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs,
                      ninst, tol, max_cycles, state->r0, state->rlast, state->cycles, state->active, state->status,
-                     state->nactive);
+                     state->nactive, state->rref);
   return check_launch("xb_batch_converge");
+}
+
+int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream) {
+  if (plan == nullptr || plan->kind != XB_BATCH_NORM || plan->table == nullptr || out == nullptr)
+    return fail(XB_EINVAL, "xb_batch_norms: needs a norm plan and an output");
+  const NormJob *jobs = reinterpret_cast<const NormJob *>(static_cast<const char *>(plan->table) + plan->jobs_offset);
+  hipLaunchKernelGGL(k_batch_norms, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs, out);
+  return check_launch("xb_batch_norms");
 }
 
 } // extern "C"

@@ -69,7 +69,7 @@ class BatchPlan(ctypes.Structure):
 
 class ConvergeState(ctypes.Structure):
     _fields_ = [("r0", c_void_p), ("rlast", c_void_p), ("cycles", c_void_p), ("active", c_void_p),
-                ("status", c_void_p), ("nactive", c_void_p)]
+                ("status", c_void_p), ("nactive", c_void_p), ("rref", c_void_p)]
 
 
 # name -> (restype, argtypes); every exported symbol of include/xlb_biot.h
@@ -105,6 +105,7 @@ SIGNATURES = {
     "xb_batch_launch": (c_int32, [ctypes.POINTER(BatchPlan), c_void_p, c_void_p]),
     "xb_batch_converge": (c_int32, [ctypes.POINTER(BatchPlan), c_int32, ctypes.POINTER(ConvergeState), c_double,
                                     c_int32, c_void_p]),
+    "xb_batch_norms": (c_int32, [ctypes.POINTER(BatchPlan), c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -7,9 +7,18 @@ and the pipeline prologue.  The best value depends on grid size, lattice and
 dtype, so it is measured: each candidate runs a few graph-captured launches timed
 with HIP events, and the fastest is cached per key in a JSON file
 (``$XB_TUNE_CACHE``, default ``~/.cache/xlb_biot/tune.json``).
+
+The multigrid hierarchy depth is tuned the same way: the reference V-cycle (sum
+restriction, injection, smoothing only on the coarsest grid) contracts for some
+Lame parameters only with few levels and diverges with more (for mu >= 1 on the
+periodic D2Q8 problem, with three levels or more), so :func:`tune_levels`
+runs a few V-cycles of the actual problem at each candidate depth and keeps the
+depth with the shortest predicted time to the tolerance.
 """
 import json
+import math
 import os
+import time
 
 import torch
 
@@ -68,3 +77,69 @@ def tune(key, launch_with_rows, candidates=CANDIDATES, reps=20):
     best = min(timings, key=timings.get)
     store(key, best, timings)
     return best, timings
+
+
+def cycles_to_tol(norms, tol, tail=20):
+    """Cycles a residual history needs to reach ``tol`` times its first value: the
+    measured count when it gets there, else an extrapolation at the geometric rate
+    of its last ``tail`` cycles (inf when that rate is not below one)."""
+    r = [float(x) for x in norms]
+    r0 = r[0]
+    if not (r0 == r0) or r0 == float("inf"):
+        return float("inf")
+    if r0 == 0.0:
+        return 1.0
+    for k, v in enumerate(r):
+        if not (v == v) or v == float("inf"):
+            return float("inf")
+        if v <= tol * r0:
+            return float(k + 1)
+    tail = min(tail, len(r) - 1)
+    a, b = r[-1 - tail], r[-1]
+    if a <= 0.0 or b >= a:
+        return float("inf")
+    rate = (b / a) ** (1.0 / tail)
+    return len(r) + math.log(tol * r0 / b) / math.log(rate)
+
+
+def probe_vcycle(vcycle_graph, state, cycles=60):
+    """Residual norms of ``cycles`` replays of a V-cycle graph from ``state``
+    (recorded on the device, read once) and the seconds per cycle."""
+    vcycle_graph.load(state)
+    hist = torch.empty(cycles, dtype=torch.float64, device=state.device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(cycles):
+        vcycle_graph.replay()
+        hist[k].copy_(vcycle_graph.norm)
+    out = hist.cpu().tolist()
+    return out, (time.perf_counter() - t0) / cycles
+
+
+def tune_levels(key, probe, candidates, tol):
+    """Depth in ``candidates`` with the shortest predicted time to reach ``tol``
+    (cycles to tolerance, measured or extrapolated, times the cost of a cycle);
+    ``probe(L)`` returns ``(residual history, seconds per cycle)`` at depth L.
+    Deeper candidates stop being tried after one that cannot reach the tolerance
+    following one that can.  Cached under ``key``."""
+    c = _load()
+    if key in c and "levels" in c[key]:
+        return int(c[key]["levels"]), c[key].get("predicted_s", {})
+    scores = {}
+    best = None
+    for L in candidates:
+        hist, sec = probe(L)
+        scores[L] = cycles_to_tol(hist, tol) * sec
+        if best is None or scores[L] < scores[best]:
+            best = L
+        elif scores[L] == float("inf") and scores[best] < float("inf"):
+            break
+    c[key] = {"levels": int(best), "predicted_s": {str(k): (round(v, 6) if v != float("inf") else "inf")
+                                                   for k, v in scores.items()}}
+    try:
+        os.makedirs(os.path.dirname(cache_path()), exist_ok=True)
+        with open(cache_path(), "w") as fh:
+            json.dump(c, fh, indent=1, sort_keys=True)
+    except OSError:
+        pass
+    return best, scores

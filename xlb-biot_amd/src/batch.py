@@ -24,6 +24,7 @@ instances through their physical steps: the per-step glue of each instance
 code, and the elasticity iterations of all instances run as one batched solve.
 """
 import ctypes
+import time
 
 import numpy as np
 import torch
@@ -57,6 +58,9 @@ class BatchPlan:
 
     def launch(self, active):
         nat.check(nat.lib().xb_batch_launch(self._ref, nat.ptr(active), nat.stream()), "xb_batch_launch")
+
+    def norms(self, out):
+        nat.check(nat.lib().xb_batch_norms(self._ref, nat.ptr(out), nat.stream()), "xb_batch_norms")
 
     def converge(self, ninst, state, tol, max_cycles):
         nat.check(nat.lib().xb_batch_converge(self._ref, ninst, ctypes.byref(state), float(tol), int(max_cycles),
@@ -103,9 +107,15 @@ class ElasticityBatch:
     precision and one ``MG_n_smoothing``; ``nonlinear_forces``: per instance
     ``None`` or a fused coupling force (``BiotNonlinearForce``, all or none).
     :meth:`solve` iterates every instance until its finest-level residual norm is
-    at most ``tol`` times that of its first cycle, or ``max_cycles`` cycles."""
+    at most ``tol`` times its scale, or ``max_cycles`` cycles.  ``scale``: "cycle"
+    (the first cycle's residual) or "rhs" (the residual of the zero state under the
+    current forces, ``|S(0)|``, computed on the device before the cycles)."""
 
-    def __init__(self, solvers, nonlinear_forces=None, tol=1e-8, max_cycles=10000, poll=4, external_force=True):
+    def __init__(self, solvers, nonlinear_forces=None, tol=1e-8, max_cycles=10000, poll=4, external_force=True,
+                 scale="cycle"):
+        if scale not in ("cycle", "rhs"):
+            raise ValueError("scale must be 'cycle' or 'rhs'")
+        self.scale = scale
         if not solvers:
             raise ValueError("an elasticity batch needs at least one solver")
         nlfs = list(nonlinear_forces) if nonlinear_forces is not None else [None] * len(solvers)
@@ -139,9 +149,13 @@ class ElasticityBatch:
         self.active = torch.ones(n, dtype=torch.int32, device=dev)
         self.status = torch.zeros(n, dtype=torch.int32, device=dev)
         self.nactive = torch.zeros(1, dtype=torch.int32, device=dev)
+        # reference residual per instance (<= 0: none), see solve(r_ref=...)
+        self.rref = torch.full((n,), -1.0, dtype=torch.float64, device=dev)
         self.state = nat.ConvergeState(nat.ptr(self.r0), nat.ptr(self.rlast), nat.ptr(self.cycles),
-                                       nat.ptr(self.active), nat.ptr(self.status), nat.ptr(self.nactive))
+                                       nat.ptr(self.active), nat.ptr(self.status), nat.ptr(self.nactive),
+                                       nat.ptr(self.rref))
         self.schedules = [self._schedule(src) for src in (0, 1)]
+        self._rhs_plan = self._rhs_schedule() if scale == "rhs" else None
         # eager cycle (buffers, tables and tuned shapes settle), then capture
         self._reset()
         self._run_schedule(0)
@@ -248,6 +262,21 @@ class ElasticityBatch:
                                     self.device)
         return plans
 
+    def _rhs_schedule(self):
+        """Finest-level residual sweep of the zero state of every instance (its
+        partials feed the norm plan, its restriction goes to scratch)."""
+        jobs = []
+        for I in self.inst:
+            nx, ny = I.el.grid_size(0)
+            I.Z = torch.zeros((8, nx, ny), dtype=self.dtype, device=self.device)
+            I.dZ = torch.zeros((1, nx, ny), dtype=self.dtype, device=self.device)
+            a = self._sweep_args(I, 0, nat.XB_MODE_RESIDUAL, I.Z, None, I.dZ, None, coarse=I.R[1])
+            a.norm_partials = nat.ptr(I.P[0])
+            if int(nat.lib().xb_elast_partials_count(a)) != I.partial_count[0]:
+                raise RuntimeError("zero-state residual and cycle residual disagree in shape")
+            jobs.append(a)
+        return BatchPlan(nat.XB_BATCH_ELAST_STEP, self.dtype, jobs, list(range(self.n)), self.device)
+
     def _run_schedule(self, src):
         for p in self.schedules[src]:
             p.launch(self.active)
@@ -283,12 +312,17 @@ class ElasticityBatch:
                           "xb_elast_divergence")
 
     def solve(self, states, forces=None, active=None):
-        """V-cycles of every (active) instance until each converges.  Returns the
-        list of final SoA states (views of the batch's buffers, valid until the
-        next solve), and sets :attr:`last_cycles`, :attr:`last_rel` and
-        :attr:`last_status` (1 converged, 2 out of cycles)."""
+        """V-cycles of every (active) instance until each converges (residual at most
+        ``tol`` times its scale).  Returns the list of final SoA states (views of the
+        batch's buffers, valid until the next solve), and sets :attr:`last_cycles`,
+        :attr:`last_rel`, :attr:`last_scale` and :attr:`last_status` (1 converged,
+        2 out of cycles)."""
         self.load(states, forces)
         self._reset(active)
+        self.rref.fill_(-1.0)
+        if self._rhs_plan is not None:
+            self._rhs_plan.launch(self.active)
+            self._norm_plan.norms(self.rref)
         cur, done = 0, 0
         limit = self.max_cycles + 2 * self.poll
         while done < limit:
@@ -307,8 +341,11 @@ class ElasticityBatch:
             raise FloatingPointError(f"V-cycle residual is not finite for instances {bad}")
         self.last_cycles = cycles
         self.last_status = status
+        ref = self.rref.cpu().numpy()
+        scale = np.where(ref > 0, ref, r0)
+        self.last_scale = scale
         with np.errstate(invalid="ignore", divide="ignore"):
-            self.last_rel = np.where(r0 > 0, rl / np.where(r0 > 0, r0, 1.0), 0.0)
+            self.last_rel = np.where(scale > 0, rl / np.where(scale > 0, scale, 1.0), 0.0)
         return [I.X[int(cycles[i]) % 2] for i, I in enumerate(self.inst)]
 
 
@@ -330,10 +367,15 @@ class BiotBatch:
         for p in problems:
             if not p.fusedCoupling:
                 raise ValueError("batched Biot steps use the fused coupling force")
+        scales = {p.elasticityTolScale for p in problems}
+        if len(scales) != 1:
+            raise ValueError("every problem needs the same elasticity_tol_scale")
         self.problems = problems
         self.batch = ElasticityBatch([p.elasticity for p in problems], [p.coupling_force() for p in problems],
-                                     tol=tols.pop(), max_cycles=maxs.pop(), poll=poll)
+                                     tol=tols.pop(), max_cycles=maxs.pop(), poll=poll, scale=scales.pop())
         self.cycles = []
+        # host seconds spent in the per-instance step glue and in the batched solves
+        self.timing = {"pre": 0.0, "solve": 0.0, "post": 0.0}
 
     def initial_state(self):
         return [p.initial_state() for p in self.problems]
@@ -342,14 +384,21 @@ class BiotBatch:
         """One physical step of every problem: ``states`` is a list of
         ``(f_el, f_darcy, dt_div_eta)`` SoA triples; returns the list of 7-tuples of
         ``LBMPoroelasticity._step_soa``."""
+        t0 = time.perf_counter()
         ctxs = [p._pre_inner(fe, fd, dde, timestep) for p, (fe, fd, dde) in zip(self.problems, states)]
+        t1 = time.perf_counter()
         fes = self.batch.solve([s[0] for s in states], [c["force"] for c in ctxs])
+        t2 = time.perf_counter()
         self.cycles.append(self.batch.last_cycles.copy())
         out = []
         for i, (p, c) in enumerate(zip(self.problems, ctxs)):
             p.inner_counts.append(int(self.batch.last_cycles[i]))
             p.inner_residuals.append(float(self.batch.last_rel[i]))
             out.append(p._post_inner(c, fes[i].clone()))
+        t3 = time.perf_counter()
+        self.timing["pre"] += t1 - t0
+        self.timing["solve"] += t2 - t1
+        self.timing["post"] += t3 - t2
         return out
 
     def run(self, t_max):

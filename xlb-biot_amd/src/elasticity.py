@@ -609,6 +609,18 @@ class LBMElasticity(object):
             return f, norm
         return f
 
+    def rhs_residual_norm(self, external_force=None, nonlinear_force=None):
+        """Finest-level residual norm of the zero state, ``|S(0)|`` over the interior
+        (the scale of the right-hand side), as a device scalar."""
+        f0 = torch.zeros((8, self.nx, self.ny), dtype=self._tdt, device=self.device)
+        if self._fusable(nonlinear_force):
+            d0 = torch.zeros((1, self.nx, self.ny), dtype=self._tdt, device=self.device)
+            _, ws, n = self._residual_nl(f0, d0, 0, None, external_force, nonlinear_force)
+        else:
+            ext = self._ext_soa(external_force, nonlinear_force, f0, 0)
+            _, ws, n = self._residual(f0, 0, None, ext)
+        return ws.finalize(n)[0]
+
     def vcycle_graph(self, external_force=None, nonlinear_force=None):
         """A V-cycle captured as HIP graphs: see :class:`VCycleGraph`."""
         return VCycleGraph(self, external_force, nonlinear_force)
@@ -741,21 +753,25 @@ class VCycleGraph:
         for _ in range(n):
             self.replay()
 
-    def run_to(self, tol, max_cycles, patience=None):
+    def run_to(self, tol, max_cycles, patience=None, r_ref=None):
         """Replay until the cycle's residual norm is below ``tol`` times the first
-        cycle's (one device-to-host read per cycle).  With ``patience``, also stop once
-        the residual has not improved for that many cycles, restoring the state of the
-        best cycle.  Returns ``(cycles, best relative residual, reached)``."""
+        cycle's, or times ``r_ref`` when that is given and positive (one
+        device-to-host read per cycle).  With ``patience``, also stop once the residual has not improved for
+        that many cycles, restoring the state of the best cycle.  Returns ``(cycles,
+        best relative residual, reached)``; :attr:`last_r0` is the first cycle's norm."""
         r0 = None
+        scale = None
         best, best_k, best_state = None, 0, None
         for k in range(max_cycles):
             self.replay()
             r = self.norm.item()
             if not np.isfinite(r):
                 raise FloatingPointError(f"V-cycle residual is not finite after {k + 1} cycles")
-            r0 = r if r0 is None else r0
-            if r <= tol * r0:
-                return k + 1, r / r0, True
+            if r0 is None:
+                r0 = self.last_r0 = r
+                scale = r_ref if r_ref is not None and r_ref > 0.0 else r0
+            if r <= tol * scale:
+                return k + 1, (r / scale if scale > 0 else 0.0), True
             if patience is not None:
                 if best is None or r < best:
                     best, best_k = r, k
@@ -764,8 +780,8 @@ class VCycleGraph:
                 elif k - best_k >= patience:
                     if best_state is not None:
                         self.buf[self.cur].copy_(best_state)
-                    return k + 1, best / r0, False
-        return max_cycles, (best if best is not None else r) / r0, False
+                    return k + 1, best / scale, False
+        return max_cycles, (best if best is not None else r) / scale, False
 
 
 # ---------------------------------------------------------------------------

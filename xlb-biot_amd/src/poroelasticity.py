@@ -71,6 +71,12 @@ class LBMPoroelasticity(object):
         # stop the residual-driven loop when the residual has not improved for this many
         # cycles (the reference V-cycle has no exact coarse solve and can stall or grow)
         self.elasticityPatience = kwargs.get("elasticity_patience", None)
+        # "cycle": tolerance relative to the first residual of each inner solve;
+        # "rhs": relative to the residual of the zero state under the step's force,
+        # |S(0)| (a warm-started solve then stops above the round-off floor)
+        self.elasticityTolScale = kwargs.get("elasticity_tol_scale", "cycle")
+        if self.elasticityTolScale not in ("cycle", "rhs"):
+            raise ValueError("elasticity_tol_scale must be 'cycle' or 'rhs'")
         self.inner_residuals = []
         self.useGraphs = kwargs.get("use_graphs", True)
         self.fusedCoupling = kwargs.get("fused_coupling", True)
@@ -271,6 +277,26 @@ class LBMPoroelasticity(object):
         fd, _ = da._step_soa(fd, source)
         return fe, fd, eta, sigma, dde, pressure, pressure_grad
 
+    def prepare_inner(self):
+        """Capture the graphs of the inner elasticity iterations now (they are
+        otherwise captured in the first step)."""
+        if not self.useGraphs:
+            return
+        el = self.elasticity
+        nlf = self.coupling_force()
+        force = torch.zeros((2, el.nx, el.ny), dtype=self._tdt, device=el.device)
+        self._inner = el.vcycle_graph(force, nlf) if el.levels > 1 else _PlainInnerGraph(el, force, nlf)
+        self._inner_key = (nlf.scale, nlf.fused)
+
+    def probe_levels(self, cycles=60):
+        """Residual norms of the first ``cycles`` V-cycles of the first physical step
+        and the seconds per cycle (for :func:`autotune.tune_levels`)."""
+        from .autotune import probe_vcycle
+        fe, fd, dde = self.initial_state()
+        ctx = self._pre_inner(fe, fd, dde, 0)
+        vg = self.elasticity.vcycle_graph(ctx["force"], ctx["nlf"])
+        return probe_vcycle(vg, fe, cycles)
+
     def _inner_solve(self, fe, force, nlf):
         """The elasticity iterations of one physical step (poroelasticity.py:325-326):
         ``elasticity_steps`` steps / V-cycles, or, with ``elasticity_tol``, V-cycles
@@ -305,7 +331,10 @@ class LBMPoroelasticity(object):
             inner.run(self.elasticitySteps)
             count = self.elasticitySteps
         else:
-            count, rel, _ = inner.run_to(tol, self.elasticityMaxSteps, self.elasticityPatience)
+            r_ref = None
+            if self.elasticityTolScale == "rhs":
+                r_ref = self.elasticity.rhs_residual_norm(force, nlf).item()
+            count, rel, _ = inner.run_to(tol, self.elasticityMaxSteps, self.elasticityPatience, r_ref)
             self.inner_residuals.append(rel)
         self.inner_counts.append(count)
         self.last_residual = getattr(inner, "norm", None)
