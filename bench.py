@@ -192,6 +192,26 @@ def bench_c3(torch, n=256, levels=4, steps=20, tol=1e-8, lam=1.0, mu=1.0, K=1.0,
             "fine_cell_vcycles_per_s_millions": round(n * n / vc_s / 1e6, 1)}
 
 
+def vcycle_bytes(n, levels, k, S=8):
+    """Algorithmic HBM bytes of one fused-coupling V-cycle of a periodic n x n grid:
+    level 0 smoothing reads f, the force and the divergence and writes f and the
+    divergence (20 S per cell), the residual sweep reads f, force and divergence and
+    writes the restriction (11 S + 8 S / 4); coarse levels also read the restricted
+    residual (smoothing 26 S, residual 17 S + 8 S / 4), are zero-filled (9 S) and
+    prolonged (8 S / 4 read coarse, 16 S + 1 S per fine cell)."""
+    total = 0.0
+    for lvl in range(levels):
+        c = (n >> lvl) ** 2
+        coarsest = lvl == levels - 1
+        smooth = 20 if lvl == 0 else 26
+        total += 2 * k * smooth * c
+        if not coarsest:
+            total += ((11 if lvl == 0 else 17) + 2) * c   # residual + restriction out
+            total += 9 * c / 4                             # zero fill of the coarse state
+            total += (2 + 17) * c                          # prolongation
+    return total * S
+
+
 def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smooth=8):
     """Batched Biot: ``n_inst`` periodic Biot instances (sizes from {64, 128, 256, 512},
     lambda, mu ~ U[0.5, 2], log-permeability mean ~ U[-1, 1], log-normal K), FP64,
@@ -220,13 +240,20 @@ def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smo
             key, lambda L, s=s, K=K: P.biot_c5("gpu", s, L, tol, max_cycles, K=K, k_smooth=k_smooth).probe_levels(),
             cands, tol)
         levels.append(L)
+    # launch shapes of the single-grid sweeps for every (size, depth) in the set
+    for n, L in sorted(set((s["n"], L) for s, L in zip(specs, levels))):
+        spec = next(s for s, L_ in zip(specs, levels) if s["n"] == n and L_ == L)
+        P.biot_c5("gpu", spec, L, tol, max_cycles, k_smooth=k_smooth,
+                  K=fields[specs.index(spec)]).elasticity.autotune(external_force=True)
     tune_s = time.perf_counter() - t0
-    log(f"  depths tuned in {tune_s:.1f} s: {dict(collections.Counter(levels))}")
+    log(f"  depths and launch shapes tuned in {tune_s:.1f} s: {dict(collections.Counter(levels))}")
     seq = [P.biot_c5("gpu", s, L, tol, max_cycles, K=K, k_smooth=k_smooth) for s, L, K in zip(specs, levels, fields)]
     for p in seq:
         p.prepare_inner()
+    t0 = time.perf_counter()
     bat = BiotBatch([P.biot_c5("gpu", s, L, tol, max_cycles, K=K, k_smooth=k_smooth)
                      for s, L, K in zip(specs, levels, fields)])
+    log(f"  batch built in {time.perf_counter() - t0:.1f} s, smoothing launch shapes {bat.batch.smooth_rows}")
     seq_states = [p.initial_state() for p in seq]
     bat_states = bat.initial_state()
     torch.cuda.synchronize()
@@ -248,6 +275,30 @@ def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smo
     bat_s = time.perf_counter() - t0
     log(f"  batched {bat_s:.2f} s")
     same = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]) for a, b in zip(seq_out, bat_states))
+    # one batched cycle with every instance active, and one 512^2 instance's own cycle
+    eb = bat.batch
+    eb._reset()
+    eb.rref.fill_(-1.0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    eb.graphs[0].replay()
+    e0.record()
+    for r in range(6):
+        eb.graphs[(r + 1) % 2].replay()
+    e1.record()
+    torch.cuda.synchronize()
+    all_active = int(eb.nactive.item())
+    batch_cycle_s = e0.elapsed_time(e1) / 1e3 / 6
+    batch_bytes = sum(vcycle_bytes(s_["n"], L, k_smooth) for s_, L in zip(specs, levels))
+    big = max(range(n_inst), key=lambda i: (specs[i]["n"], -levels[i]))
+    vg = seq[big]._inner
+    vg.replay()
+    e0.record()
+    for _ in range(6):
+        vg.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    one_cycle_s = e0.elapsed_time(e1) / 1e3 / 6
+    one_bytes = vcycle_bytes(specs[big]["n"], levels[big], k_smooth)
     per_step = [[p.inner_counts[t] for p in seq] for t in range(steps)]
     counts = [c for p in seq for c in p.inner_counts]
     rels = [r for p in seq for r in p.inner_residuals]
@@ -264,7 +315,15 @@ def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smo
             "solves_reaching_tol": int(sum(1 for r in rels if r <= tol)), "solves": len(rels),
             "batched_fine_cell_vcycles_per_s_millions": round(fine_cycles / bat_s / 1e6, 1),
             "bitwise_equal_to_sequential": bool(same), "depth_tuning_s": round(tune_s, 2),
-            "batched_host_split_s": {k: round(v, 4) for k, v in bat.timing.items()}}
+            "batched_host_split_s": {k: round(v, 4) for k, v in bat.timing.items()},
+            "batched_cycle_all_active": {"instances_active": all_active, "ms": round(batch_cycle_s * 1e3, 3),
+                                         "algorithmic_gb": round(batch_bytes / 1e9, 3),
+                                         "gbs": round(batch_bytes / batch_cycle_s / 1e9, 1),
+                                         "hbm_frac": round(batch_bytes / batch_cycle_s / 1e9 / HBM_PEAK_GBS, 4)},
+            "single_instance_cycle": {"n": specs[big]["n"], "levels": levels[big], "ms": round(one_cycle_s * 1e3, 3),
+                                      "gbs": round(one_bytes / one_cycle_s / 1e9, 1)},
+            "batch_smooth_rows": {str(k): v for k, v in bat.batch.smooth_rows.items()},
+            "per_instance": [[s_["n"], L, [int(c) for c in p.inner_counts]] for s_, L, p in zip(specs, levels, seq)]}
 
 
 def cpu_baseline(n, precision, seconds):

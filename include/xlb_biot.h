@@ -243,7 +243,7 @@ typedef struct xb_batch_plan { /* filled by xb_batch_prepare                    
   int32_t njobs, nblocks;
   void *table;
   int64_t table_bytes;
-  int64_t jobs_offset, blocks_offset;
+  int64_t jobs_offset, blocks_offset, insts_offset;
 } xb_batch_plan;
 
 /* Per-instance convergence state (device arrays of ninst entries, nactive one int).

@@ -2,7 +2,12 @@
 
 Usage: ``python xlb-biot_amd/build.py``.  Packed-FP32 instructions are disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``); the host pass warns that it
-does not know the feature, which is harmless.
+does not know the feature, which is harmless.  Floating-point contraction is off
+(``-ffp-contract=off``): the collision is inlined in several places of the sweep
+(pipeline prologue, unrolled loop bodies), and with contraction the compiler may
+fuse multiply-adds differently in each copy, so results would depend in the last
+bit on the launch shape; without it every cell gets the same arithmetic wherever
+it is computed (the kernels are bandwidth-bound, the extra instructions are free).
 """
 import os
 import subprocess
@@ -25,7 +30,7 @@ def hipcc():
 def build(verbose=False):
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"), "-o", tmp] + SOURCES
     res = subprocess.run(cmd, capture_output=True, text=True)

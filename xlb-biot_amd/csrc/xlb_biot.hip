@@ -314,12 +314,13 @@ template <typename T, class Coll> struct SweepJob {
 template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES, bool HAS_NL>
 __global__ void __launch_bounds__(BLOCK)
 k_sweep_batch(const SweepJob<T, Coll> *__restrict__ jobs, const int32_t *__restrict__ block_job,
-              const int32_t *__restrict__ active) {
+              const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
   const int b = xcd_block_id();
   const int j = block_job[b];
+  const int i = block_inst[b];
   if (j < 0) return;
+  if (active != nullptr && active[i] == 0) return;
   const SweepJob<T, Coll> J = jobs[j];
-  if (active != nullptr && active[J.inst] == 0) return;
   const int item = (b - J.block0) * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
   if (item >= J.g.nitems) return; // uniform per wavefront
   sweep_item<T, Coll, MODE, HAS_BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
@@ -719,12 +720,13 @@ template <typename T> struct ProlongJob {
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_prolong_batch(const ProlongJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
-                const int32_t *__restrict__ active) {
+                const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
   const int b = blockIdx.x;
   const int j = block_job[b];
+  const int i = block_inst[b];
   if (j < 0) return;
+  if (active != nullptr && active[i] == 0) return;
   const ProlongJob<T> J = jobs[j];
-  if (active != nullptr && active[J.inst] == 0) return;
   const int nxi = J.ix1 - J.ix0, nyi = J.iy1 - J.iy0;
   const int64_t n = (int64_t)nxi * nyi;
   const size_t N = (size_t)J.nx * J.ny, Nc = (size_t)J.cnx * J.cny;
@@ -752,12 +754,13 @@ template <typename T> struct FillJob {
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_fill_batch(const FillJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
-             const int32_t *__restrict__ active) {
+             const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
   const int b = blockIdx.x;
   const int j = block_job[b];
+  const int i = block_inst[b];
   if (j < 0) return;
+  if (active != nullptr && active[i] == 0) return;
   const FillJob<T> J = jobs[j];
-  if (active != nullptr && active[J.inst] == 0) return;
   for (int64_t i = (int64_t)(b - J.block0) * 256 + threadIdx.x; i < J.n; i += (int64_t)J.nblocks * 256)
     J.out[i] = J.value;
 }
@@ -830,10 +833,17 @@ k_batch_norms(const NormJob *__restrict__ jobs, int njobs, double *__restrict__ 
 inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 struct BlockTable {
-  std::vector<int32_t> block_job;
-  void add(int job, int nblocks) { block_job.insert(block_job.end(), (size_t)nblocks, job); }
+  std::vector<int32_t> block_job;  // job of each block (-1: padding)
+  std::vector<int32_t> block_inst; // instance of each block's job (read beside block_job)
+  void add(int job, int inst, int nblocks) {
+    block_job.insert(block_job.end(), (size_t)nblocks, job);
+    block_inst.insert(block_inst.end(), (size_t)nblocks, inst);
+  }
   void pad_xcd() {
-    while (block_job.size() % 8) block_job.push_back(-1);
+    while (block_job.size() % 8) {
+      block_job.push_back(-1);
+      block_inst.push_back(0);
+    }
   }
 };
 
@@ -866,7 +876,7 @@ template <typename T> int build_sweep_jobs(const xb_elast_args *args, const int3
     J.gm = T(gamma);
     J.inst = inst != nullptr ? inst[k] : k;
     J.block0 = (int32_t)bt.block_job.size();
-    bt.add(k, cdiv(J.g.nitems, WAVES_PER_BLOCK));
+    bt.add(k, J.inst, cdiv(J.g.nitems, WAVES_PER_BLOCK));
     jobs.push_back(J);
   }
   bt.pad_xcd();
@@ -896,7 +906,7 @@ template <typename T> int build_prolong_jobs(const xb_prolong_job *args, const i
     J.inst = inst != nullptr ? inst[k] : k;
     J.block0 = (int32_t)bt.block_job.size();
     J.nblocks = ew_blocks((int64_t)(a->ix1 - a->ix0) * (a->iy1 - a->iy0));
-    bt.add(k, J.nblocks);
+    bt.add(k, J.inst, J.nblocks);
     jobs.push_back(J);
   }
   return XB_OK;
@@ -914,7 +924,7 @@ template <typename T> int build_fill_jobs(const xb_fill_job *args, const int32_t
     J.inst = inst != nullptr ? inst[k] : k;
     J.block0 = (int32_t)bt.block_job.size();
     J.nblocks = a->n > 0 ? ew_blocks(a->n) : 0;
-    bt.add(k, J.nblocks);
+    bt.add(k, J.inst, J.nblocks);
     jobs.push_back(J);
   }
   return XB_OK;
@@ -934,7 +944,8 @@ template <typename J>
 int finish_table(const std::vector<J> &jobs, const BlockTable &bt, int32_t kind, int32_t dtype, int32_t mode,
                  int32_t flags, void *table, int64_t bytes, xb_batch_plan *plan, hipStream_t st, int64_t *need) {
   const int64_t jb = align256((int64_t)(jobs.size() * sizeof(J)));
-  const int64_t total = jb + (int64_t)bt.block_job.size() * 4;
+  const int64_t ib = jb + align256((int64_t)bt.block_job.size() * 4);
+  const int64_t total = ib + (int64_t)bt.block_inst.size() * 4;
   if (need != nullptr) {
     *need = std::max<int64_t>(total, 256);
     return XB_OK;
@@ -947,6 +958,8 @@ int finish_table(const std::vector<J> &jobs, const BlockTable &bt, int32_t kind,
   }
   if (!bt.block_job.empty()) {
     if (hipMemcpyAsync(static_cast<char *>(table) + jb, bt.block_job.data(), bt.block_job.size() * 4,
+                       hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(static_cast<char *>(table) + ib, bt.block_inst.data(), bt.block_inst.size() * 4,
                        hipMemcpyHostToDevice, st) != hipSuccess)
       return fail(XB_EHIP, "xb_batch_prepare: block table upload failed");
   }
@@ -961,6 +974,7 @@ int finish_table(const std::vector<J> &jobs, const BlockTable &bt, int32_t kind,
   plan->table_bytes = bytes;
   plan->jobs_offset = 0;
   plan->blocks_offset = jb;
+  plan->insts_offset = ib;
   return XB_OK;
 }
 
@@ -1000,9 +1014,10 @@ int launch_elast_batch(const xb_batch_plan *p, const int32_t *active, hipStream_
   using J = SweepJob<T, ElastColl<T>>;
   const J *jobs = reinterpret_cast<const J *>(static_cast<const char *>(p->table) + p->jobs_offset);
   const int32_t *bj = reinterpret_cast<const int32_t *>(static_cast<const char *>(p->table) + p->blocks_offset);
+  const int32_t *bi = reinterpret_cast<const int32_t *>(static_cast<const char *>(p->table) + p->insts_offset);
   const dim3 grid(p->nblocks), block(BLOCK);
 #define XB_SB(BC, RES, NL) \
-  hipLaunchKernelGGL((k_sweep_batch<T, ElastColl<T>, MODE, BC, RES, NL>), grid, block, 0, st, jobs, bj, active)
+  hipLaunchKernelGGL((k_sweep_batch<T, ElastColl<T>, MODE, BC, RES, NL>), grid, block, 0, st, jobs, bj, bi, active)
   switch (p->flags) {
   case 0: XB_SB(false, false, false); break;
   case 1: XB_SB(true, false, false); break;
@@ -1020,6 +1035,7 @@ int launch_elast_batch(const xb_batch_plan *p, const int32_t *active, hipStream_
 template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
   const char *base = static_cast<const char *>(p->table);
   const int32_t *bj = reinterpret_cast<const int32_t *>(base + p->blocks_offset);
+  const int32_t *bi = reinterpret_cast<const int32_t *>(base + p->insts_offset);
   switch (p->kind) {
   case XB_BATCH_ELAST_STEP:
     if (p->mode == XB_MODE_STEP) return launch_elast_batch<T, XB_MODE_STEP>(p, active, st);
@@ -1027,11 +1043,11 @@ template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *
     return launch_elast_batch<T, XB_MODE_RESIDUAL>(p, active, st);
   case XB_BATCH_PROLONG_DIV:
     hipLaunchKernelGGL(k_prolong_batch<T>, dim3(p->nblocks), dim3(256), 0, st,
-                       reinterpret_cast<const ProlongJob<T> *>(base + p->jobs_offset), bj, active);
+                       reinterpret_cast<const ProlongJob<T> *>(base + p->jobs_offset), bj, bi, active);
     return check_launch("batched prolongation kernel");
   default:
     hipLaunchKernelGGL(k_fill_batch<T>, dim3(p->nblocks), dim3(256), 0, st,
-                       reinterpret_cast<const FillJob<T> *>(base + p->jobs_offset), bj, active);
+                       reinterpret_cast<const FillJob<T> *>(base + p->jobs_offset), bj, bi, active);
     return check_launch("batched fill kernel");
   }
 }

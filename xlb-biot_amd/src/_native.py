@@ -64,7 +64,7 @@ class NormJob(ctypes.Structure):
 class BatchPlan(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("dtype", c_int32), ("mode", c_int32), ("flags", c_int32),
                 ("njobs", c_int32), ("nblocks", c_int32), ("table", c_void_p), ("table_bytes", c_int64),
-                ("jobs_offset", c_int64), ("blocks_offset", c_int64)]
+                ("jobs_offset", c_int64), ("blocks_offset", c_int64), ("insts_offset", c_int64)]
 
 
 class ConvergeState(ctypes.Structure):

@@ -30,6 +30,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import autotune
 
 
 class BatchPlan:
@@ -112,7 +113,7 @@ class ElasticityBatch:
     current forces, ``|S(0)|``, computed on the device before the cycles)."""
 
     def __init__(self, solvers, nonlinear_forces=None, tol=1e-8, max_cycles=10000, poll=4, external_force=True,
-                 scale="cycle"):
+                 scale="cycle", smooth_rows="tune"):
         if scale not in ("cycle", "rhs"):
             raise ValueError("scale must be 'cycle' or 'rhs'")
         self.scale = scale
@@ -154,6 +155,19 @@ class ElasticityBatch:
         self.state = nat.ConvergeState(nat.ptr(self.r0), nat.ptr(self.rlast), nat.ptr(self.cycles),
                                        nat.ptr(self.active), nat.ptr(self.status), nat.ptr(self.nactive),
                                        nat.ptr(self.rref))
+        # launch shape of the batched smoothing sweeps per level: the single-grid
+        # choice favours many short wavefront walks (parallelism on one small grid),
+        # a batch has parallelism to spare and prefers longer walks (fewer halo
+        # re-reads, fewer blocks).  Residual sweeps keep the single-grid shape so that
+        # their norm partials, hence the stopping decisions, are those of the
+        # instances solved one by one.
+        L = max(I.levels for I in self.inst)
+        if smooth_rows == "tune":
+            self.smooth_rows = self._tune_smooth_rows(L)
+        elif isinstance(smooth_rows, dict):
+            self.smooth_rows = dict(smooth_rows)
+        else:
+            self.smooth_rows = {lvl: smooth_rows for lvl in range(L)} if smooth_rows else {}
         self.schedules = [self._schedule(src) for src in (0, 1)]
         self._rhs_plan = self._rhs_schedule() if scale == "rhs" else None
         # eager cycle (buffers, tables and tuned shapes settle), then capture
@@ -169,7 +183,7 @@ class ElasticityBatch:
         torch.cuda.synchronize()
 
     # ---- schedule of one V-cycle of every instance (ping-pong parity src)
-    def _sweep_args(self, I, level, mode, f_in, f_out, div_in, div_out, coarse=None, partials=None):
+    def _sweep_args(self, I, level, mode, f_in, f_out, div_in, div_out, coarse=None, partials=None, rows=None):
         el = I.el
         ext = I.ext if level == 0 and self.external_force else None
         resid = I.R[level] if level > 0 else None
@@ -178,7 +192,45 @@ class ElasticityBatch:
             a.norm_partials = nat.ptr(partials)
         if self.fused:
             el._nl_args(a, div_in, div_out, I.nlf, level)
+        if rows is None and mode == nat.XB_MODE_SMOOTH:
+            rows = getattr(self, "smooth_rows", {}).get(level)
+        if rows:
+            a.rows = int(rows)
         return a
+
+    def _signature(self):
+        import hashlib
+        sig = repr(sorted((I.el.nx, I.el.ny, I.levels, I.el._tables.get(0) is not None) for I in self.inst))
+        return hashlib.sha1(f"{sig}/{self.dtype}/{self.fused}/{self.external_force}".encode()).hexdigest()[:16]
+
+    def _tune_smooth_rows(self, L, candidates=autotune.CANDIDATES, reps=5):
+        """Launch shape of each level's batched smoothing sweep: every candidate is
+        timed on the whole batch with HIP events; the fastest is kept and cached."""
+        out = {}
+        sig = self._signature()
+        active = torch.ones(self.n, dtype=torch.int32, device=self.device)
+        for level in range(L):
+            key = f"batch/elast_smooth/{sig}/level{level}"
+            rows = autotune.lookup(key)
+            if rows:
+                out[level] = rows
+                continue
+            ids = [i for i, I in enumerate(self.inst) if I.levels > level]
+
+            def run(r, level=level, ids=ids):
+                jobs = []
+                for i in ids:
+                    I = self.inst[i]
+                    f, o, d, od = ((I.X[0], I.T[0], I.dX[0], I.dT[0]) if level == 0 else
+                                   (I.C[level][0], I.C[level][1], I.D[level][0], I.D[level][1]))
+                    jobs.append(self._sweep_args(I, level, nat.XB_MODE_SMOOTH, f, o, d, od, rows=r))
+                plan = BatchPlan(nat.XB_BATCH_ELAST_STEP, self.dtype, jobs, ids, self.device)
+                return autotune.time_launch(lambda: plan.launch(active), reps)
+            timings = {r: run(r) for r in candidates}
+            best = min(timings, key=timings.get)
+            autotune.store(key, best, timings)
+            out[level] = best
+        return out
 
     def _schedule(self, src):
         """A list of ``BatchPlan`` objects (and the final norm plan) for one cycle."""

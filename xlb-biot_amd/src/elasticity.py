@@ -330,9 +330,10 @@ class LBMElasticity(object):
                           autotune.lookup(self._tune_key(level, mode, ext is not None, resid is not None)), 0)
         return a
 
-    def autotune(self, levels=None):
+    def autotune(self, levels=None, external_force=False):
         """Measure and cache the best sweep launch shape for the step / smoothing /
-        residual kernels of every level of this solver."""
+        residual kernels of every level of this solver (with ``external_force``, for
+        the finest-level launches that carry an external force, as in the Biot step)."""
         lib = nat.lib()
         out = {}
         for level in (range(self.levels) if levels is None else levels):
@@ -344,9 +345,11 @@ class LBMElasticity(object):
             modes = [nat.XB_MODE_STEP] if self.levels == 1 else [nat.XB_MODE_SMOOTH]
             if level < self.levels - 1:
                 modes.append(nat.XB_MODE_RESIDUAL)
+            ext = (torch.zeros((2, nx, ny), dtype=self._tdt, device=self.device)
+                   if external_force and level == 0 else None)
             for mode in modes:
                 coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=self._tdt, device=self.device)
-                a = self._launch(f, o, level, mode, resid=r, coarse=coarse)
+                a = self._launch(f, o, level, mode, ext=ext, resid=r, coarse=coarse)
                 a.rows = 2  # the most wavefronts, hence the most partial sums
                 ws = NormWorkspace(lib.xb_elast_partials_count(a), self.device)
                 a.norm_partials = nat.ptr(ws.partials)
@@ -354,7 +357,7 @@ class LBMElasticity(object):
                 def run(rows, a=a):
                     a.rows = rows
                     nat.check(lib.xb_elast_step(a, nat.stream()), "xb_elast_step(tune)")
-                key = self._tune_key(level, mode, False, r is not None)
+                key = self._tune_key(level, mode, ext is not None, r is not None)
                 out[key] = autotune.tune(key, run)
         return out
 
