@@ -96,6 +96,10 @@ typedef struct xb_elast_args {
   void *div_out;            /* (nx, ny) divergence of f_out (STEP / SMOOTH)         */
   double nl_scale;
   const xb_bc_table *grad_bc;
+  /* Optional per-cell Lame parameters (nx, ny) each, both or neither; they replace
+   * lam, mu (not with the fused coupling force). */
+  const void *lam_field;
+  const void *mu_field;
 } xb_elast_args;
 
 typedef struct xb_darcy_args {
@@ -148,21 +152,67 @@ int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, 
                              void *div, void *stream);
 
 /* eta (2,nx,ny), sigma (3,nx,ny) from f and the optional forces.
- * Replaces update_macroscopic (src/elasticity.py:1002-1038). Either output may be null. */
+ * Replaces update_macroscopic (src/elasticity.py:1002-1038). Either output may be null.
+ * lam_field / mu_field (optional, both or neither): per-cell Lame parameters. */
 int xb_elast_macro(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
                    const void *f, const void *force_a, const void *force_b,
-                   void *eta_out, void *sigma_out, void *stream);
+                   void *eta_out, void *sigma_out, const void *lam_field, const void *mu_field,
+                   void *stream);
 
 /* div = (-1.5/(3(lam+mu)+1)) * m_s(f - f_sub) (f_sub optional).
  * Replaces update_divergence (src/elasticity.py:1040-1058). */
 int xb_elast_divergence(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
-                        const void *f, const void *f_sub, void *div_out, void *stream);
+                        const void *f, const void *f_sub, void *div_out,
+                        const void *lam_field, const void *mu_field, void *stream);
 
 /* f = equilibrium(eta, sigma) with eta shifted by -F/2 for each given force.
  * Replaces equilibrium (src/elasticity.py:908-957). */
 int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
                          const void *eta, const void *sigma, const void *force_a,
-                         const void *force_b, void *f_out, void *stream);
+                         const void *force_b, void *f_out, const void *lam_field,
+                         const void *mu_field, void *stream);
+
+/* ---- adjoint steps ---------------------------------------------------------------
+ * No reference counterpart as kernels: they give what jax.grad through the jitted
+ * _step (src/elasticity.py:1169-1219) / AdvectionDiffusionReactionBGK.step
+ * (src/darcy.py:131-171) would.  With S the step (collide, post-collision rules,
+ * stream, post-streaming rules; affine in f), they compute
+ *   abar_out = (dS/df)^T abar_in  (+ abar_add)
+ * and add abar_in . dS/dtheta, evaluated at the step's input f_in, to the FP64
+ * gradient accumulators that are given.  Walls as in the forward step; no multigrid
+ * residual, no fused coupling force. */
+typedef struct xb_elast_adjoint_args {
+  int32_t nx, ny, dtype, pad_;
+  double lam, mu;             /* used where the fields are null                       */
+  const void *lam_field;      /* optional (nx, ny), with mu_field                     */
+  const void *mu_field;
+  const void *abar_in;        /* (8, nx, ny) adjoint of the step's output             */
+  const void *f_in;           /* (8, nx, ny) populations entering the step            */
+  void *abar_out;             /* (8, nx, ny) adjoint of the step's input (!= abar_in) */
+  const void *abar_add;       /* optional (8, nx, ny) added to abar_out               */
+  double *grad_lam;           /* optional (nx, ny): += dJ/dlambda(x)                  */
+  double *grad_mu;            /* optional (nx, ny): += dJ/dmu(x)                      */
+  double *grad_force;         /* optional (2, nx, ny): += dJ/dF(x), F the total force */
+  const xb_bc_table *bc;      /* the forward step's rule table (optional)             */
+} xb_elast_adjoint_args;
+
+typedef struct xb_darcy_adjoint_args {
+  int32_t nx, ny, dtype, pad_;
+  double omega;               /* used where omega_field is null                      */
+  const void *omega_field;    /* optional (nx, ny)                                   */
+  const void *source_a;       /* the forward step's sources (optional)               */
+  const void *source_b;
+  const void *abar_in;        /* (9, nx, ny)                                         */
+  const void *f_in;           /* (9, nx, ny)                                         */
+  void *abar_out;             /* (9, nx, ny), != abar_in                             */
+  const void *abar_add;       /* optional (9, nx, ny)                                */
+  double *grad_omega;         /* optional (nx, ny): += dJ/domega(x)                  */
+  double *grad_source;        /* optional (nx, ny): += dJ/dS(x), S = source_a + _b   */
+  const xb_bc_table *bc;
+} xb_darcy_adjoint_args;
+
+int xb_elast_adjoint(const xb_elast_adjoint_args *args, void *stream);
+int xb_darcy_adjoint(const xb_darcy_adjoint_args *args, void *stream);
 
 /* ---- Darcy (D2Q9 BGK advection-diffusion-reaction, zero velocity) ------------------ */
 

@@ -44,6 +44,8 @@ class OracleDarcy:
     def _omega(self, dtype):
         if isinstance(self.omega, float):
             return self.omega
+        if isinstance(self.omega, torch.Tensor):
+            return self.omega.to(dtype)  # keeps an autograd graph (gradient checks)
         return torch.as_tensor(np.asarray(self.omega), dtype=dtype)
 
     def bounding_box_indices(self):

@@ -44,6 +44,33 @@ class OracleElasticity:
         self._create_boundary_data()
         force = self.get_force()
         self.force = None if force is None else torch.as_tensor(np.asarray(force), dtype=self.precisionPolicy.compute_dtype)
+        # per-cell Lame parameters (nx, ny) (an extension of the reference's scalars, for
+        # the gradient checks: tensors that may require grad)
+        self.lam_field = kwargs.get("lambda_field")
+        self.mu_field = kwargs.get("mu_field")
+
+    def _coef(self):
+        """(cs, cd, h, n) of the collision, scalars or per-cell (nx, ny, 1) tensors,
+        from elasticity.py:811-813 and :976-979."""
+        if self.lam_field is None:
+            lm, mu = self.lamda + self.mu, self.mu
+            g = 1.0 / (12.0 * lm - 4.0)
+            g = 0.0 if abs(g) > 100 else g
+        else:
+            lm = (self.lam_field + self.mu_field)[..., None]
+            mu = self.mu_field[..., None]
+            g = 1.0 / (12.0 * lm - 4.0)
+            g = torch.where(g.abs() > 100, torch.zeros_like(g), g)
+        return (3 * lm - 1) / (3 * lm + 1), (6 * mu - 1) / (6 * mu + 1), 0.25 + 0.5 * g, -0.25 * g
+
+    def _to_f(self, m, h, n):
+        # m @ m_to_f with per-cell h, n in the column of m_s
+        if self.lam_field is None:
+            return torch.matmul(m, self.m_to_f)
+        base = self.m_to_f.clone()
+        base[2, :] = 0.0
+        hn = torch.cat([h.expand(*m.shape[:-1], 4), n.expand(*m.shape[:-1], 4)], dim=-1)
+        return torch.matmul(m, base) + m[..., 2:3] * hn
 
     # ---- geometry (elasticity.py:501-604)
     def grid_size(self, level=0):
@@ -119,13 +146,18 @@ class OracleElasticity:
             eta = eta - 0.5 * self.force
         if external_force is not None:
             eta = eta - 0.5 * torch.as_tensor(external_force).to(pp.compute_dtype)
-        cs = -1.0 - 1 / (3 * (self.lamda + self.mu))
+        if self.lam_field is None:
+            lm, mu = self.lamda + self.mu, self.mu
+        else:
+            lm, mu = (self.lam_field + self.mu_field)[..., None], self.mu_field[..., None]
+        cs = -1.0 - 1 / (3 * lm)
         ms = (sigma[..., 0:1] + sigma[..., 2:3]) * cs
-        cd = -1.0 - 1 / (6 * self.mu)
+        cd = -1.0 - 1 / (6 * mu)
         md = (sigma[..., 0:1] - sigma[..., 2:3]) * cd
         m11 = sigma[..., 1:2] * cd
         m = torch.cat((eta, ms, md, m11, eta / 3.0), dim=-1)
-        return pp.cast_to_output(torch.matmul(m, self.m_to_f))
+        _, _, h, n = self._coef()
+        return pp.cast_to_output(self._to_f(m, h, n))
 
     def collision(self, f, external_force, use_force=True):
         # elasticity.py:959-1000
@@ -133,8 +165,7 @@ class OracleElasticity:
         f = pp.cast_to_compute(f)
         external_force = pp.cast_to_compute(external_force)
         m = torch.matmul(f, self.f_to_m)
-        cs = (3 * (self.lamda + self.mu) - 1) / (3 * (self.lamda + self.mu) + 1)
-        cd = (6 * self.mu - 1) / (6 * self.mu + 1)
+        cs, cd, h, n = self._coef()
         if use_force and self.force is not None and external_force is not None:
             force = self.force + external_force
         elif external_force is not None:
@@ -148,7 +179,7 @@ class OracleElasticity:
                            m[..., 0:2] / 3.0 + force / 6.0], dim=-1)
         else:
             m = torch.cat([m[..., 0:2], m[..., 2:3] * cs, m[..., 3:5] * cd, m[..., 0:2] / 3.0], dim=-1)
-        return pp.cast_to_output(torch.matmul(m, self.m_to_f))
+        return pp.cast_to_output(self._to_f(m, h, n))
 
     def update_macroscopic(self, f, external_force=None):
         # elasticity.py:1002-1038
@@ -158,16 +189,21 @@ class OracleElasticity:
             eta = eta + 0.5 * self.force
         if external_force is not None:
             eta = eta + 0.5 * external_force
-        cs = -1.5 * (self.lamda + self.mu) / (3 * (self.lamda + self.mu) + 1)
+        if self.lam_field is None:
+            lm, mu = self.lamda + self.mu, self.mu
+        else:
+            lm, mu = self.lam_field + self.mu_field, self.mu_field
+        cs = -1.5 * lm / (3 * lm + 1)
         ms = m[..., 2] * cs
-        cd = -3 * self.mu / (6 * self.mu + 1)
+        cd = -3 * mu / (6 * mu + 1)
         md = m[..., 3] * cd
         sigma = torch.stack((ms + md, (2 * cd) * m[..., 4], ms - md), dim=-1)
         return eta, sigma
 
     def update_divergence(self, f):
         # elasticity.py:1040-1058
-        return (-1.5 / (3 * (self.lamda + self.mu) + 1)) * torch.matmul(f.to(self.f_to_m.dtype), self.f_to_m[:, 2:3])
+        lm = self.lamda + self.mu if self.lam_field is None else (self.lam_field + self.mu_field)[..., None]
+        return (-1.5 / (3 * lm + 1)) * torch.matmul(f.to(self.f_to_m.dtype), self.f_to_m[:, 2:3])
 
     # ---- step (elasticity.py:1128-1219)
     def _bcs(self, level):

@@ -443,3 +443,30 @@ def biot_consolidation(kind, nx=32, levels=2, steps=2, precision="f64/f64", k_sm
                             "coupling_method": coupling, "alpha": alpha, "lambda": lam, "mu": mu,
                             "epsilon": 1.0 / nx, "omega": 1.0 / (3 * K + 0.5), "nx": nx, "ny": nx + 2, "nz": 0,
                             "precision": precision, "print_info_rate": 0})
+
+
+def consolidation_elasticity(kind, nx=16, lam=2.0, mu=1.0, precision="f64/f64", load=1e-3, lambda_field=None,
+                             mu_field=None):
+    """The elastic half of the 2-D consolidation set-up (examples/poroelasticity/2D.py):
+    fixed bottom (full-way bounce-back), traction at the top (corner-fixed
+    anti-bounce-back), periodic in x, ``ny = nx + 2``; single grid.  Optional per-cell
+    Lame fields (the oracle takes tensors, which may require grad)."""
+    m = modules(kind)
+    ElAntiBB = _corner_fix(m["EABB"], EL_CORNERS)
+    EBBF = m["EBBF"]
+
+    class El(m["Elasticity"]):
+        def initialize_macroscopic_fields(self):
+            return np.zeros((self.nx, self.ny, 2)), np.zeros((self.nx, self.ny, 3))
+
+        def set_boundary_conditions(self):
+            self.BCs.append(EBBF(tuple(self.boundingBoxIndices["bottom"].T), self.gridInfo, self.precisionPolicy))
+            force = np.fromfunction(lambda x, d: (d == 1) * (1 - np.cos(np.pi * ((2 * x + 1) / self.nx + 0.5))) * -load,
+                                    (self.nx, 2))
+            self.BCs.append(ElAntiBB(tuple(self.boundingBoxIndices["top"].T), self.gridInfo, self.precisionPolicy, force))
+
+    kw = {"lambda": lam, "mu": mu, "nx": nx, "ny": nx + 2, "nz": 0, "precision": precision,
+          "lattice": m["D2Q8"](precision), "print_info_rate": 0}
+    if lambda_field is not None:
+        kw["lambda_field"], kw["mu_field"] = lambda_field, mu_field
+    return El(**kw)

@@ -68,6 +68,19 @@ __device__ __forceinline__ void apply_rules(T (&v)[Q], const T (&src)[Q], const 
 // ---------------------------------------------------------------------------
 // D2Q8 elasticity: moment relaxation (reference src/elasticity.py:959-1000).
 // Directions c = (1,0) (0,1) (-1,0) (0,-1) (1,1) (-1,1) (-1,-1) (1,-1).
+
+// Relaxation rates and m_to_f entries from the Lame parameters of one cell
+// (elasticity.py:805-848; g is set to 0 where |g| > 100, :811-813).
+template <typename T> __device__ __forceinline__ void elast_coef(T lam, T mu, T &cs, T &cd, T &h, T &n) {
+  const T lm = lam + mu;
+  T g = T(1) / (T(12) * lm - T(4));
+  if (fabs(g) > T(100)) g = T(0);
+  cs = (T(3) * lm - T(1)) / (T(3) * lm + T(1));
+  cd = (T(6) * mu - T(1)) / (T(6) * mu + T(1));
+  h = T(0.25) + T(0.5) * g;
+  n = T(-0.25) * g;
+}
+
 template <typename T> struct ElastColl {
   static constexpr int Q = 8;
   __host__ __device__ static constexpr int cx(int d) { return d == 0 || d == 4 || d == 7 ? 1 : (d == 2 || d == 5 || d == 6 ? -1 : 0); }
@@ -75,22 +88,27 @@ template <typename T> struct ElastColl {
   T cs, cd, h, n;       // relaxation of m_s, of m_d and m_11; m_to_f entries h, n
   const T *fa = nullptr; // body force (2, N)
   const T *fb = nullptr; // external force (2, N)
+  const T *lamf = nullptr; // per-cell Lame parameters (N each; both or neither):
+  const T *muf = nullptr;  // then cs, cd, h, n are formed per cell
   size_t N = 0;
   bool has_b = false;    // an external force takes part (given, or formed in the sweep)
 
   // Per-cell operands besides the populations, loaded ahead of the collision so that
   // the sweep can keep them in flight with the populations of the same row.
   struct Aux {
-    T a0, a1, b0, b1;
+    T a0, a1, b0, b1, lam, mu;
   };
   __device__ __forceinline__ Aux load_aux(size_t cell) const {
-    Aux x{T(0), T(0), T(0), T(0)};
+    Aux x{T(0), T(0), T(0), T(0), T(0), T(0)};
     if (fa != nullptr) { x.a0 = fa[cell]; x.a1 = fa[N + cell]; }
     if (fb != nullptr) { x.b0 = fb[cell]; x.b1 = fb[N + cell]; }
+    if (lamf != nullptr) { x.lam = lamf[cell]; x.mu = muf[cell]; }
     return x;
   }
 
   __device__ __forceinline__ void collide(const T (&f)[8], const Aux &x, T (&o)[8]) const {
+    T cs_ = cs, cd_ = cd, h_ = h, n_ = n;
+    if (lamf != nullptr) elast_coef<T>(x.lam, x.mu, cs_, cd_, h_, n_);
     const T m10 = f[0] - f[2] + f[4] - f[5] - f[6] + f[7];
     const T m01 = f[1] - f[3] + f[4] + f[5] - f[6] - f[7];
     const T ms = f[0] + f[1] + f[2] + f[3] + T(2) * (f[4] + f[5] + f[6] + f[7]);
@@ -106,8 +124,8 @@ template <typename T> struct ElastColl {
       M5 = m10 / T(3) + F0 / T(6);
       M6 = m01 / T(3) + F1 / T(6);
     }
-    const T M2 = ms * cs, M3 = md * cd, M4 = m11 * cd;
-    const T hM2 = h * M2, nM2 = n * M2;
+    const T M2 = ms * cs_, M3 = md * cd_, M4 = m11 * cd_;
+    const T hM2 = h_ * M2, nM2 = n_ * M2;
     o[0] = T(0.5) * M0 + hM2 + T(0.25) * M3 - T(0.5) * M5;
     o[1] = T(0.5) * M1 + hM2 - T(0.25) * M3 - T(0.5) * M6;
     o[2] = -T(0.5) * M0 + hM2 + T(0.25) * M3 + T(0.5) * M5;

@@ -414,9 +414,16 @@ __global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__
 }
 
 template <typename T>
-__global__ void k_elast_macro(int64_t N, T cs, T cd, const T *__restrict__ f, const T *__restrict__ fa,
-                              const T *__restrict__ fb, T *__restrict__ eta, T *__restrict__ sig) {
+__global__ void k_elast_macro(int64_t N, T cs0, T cd0, const T *__restrict__ f, const T *__restrict__ fa,
+                              const T *__restrict__ fb, T *__restrict__ eta, T *__restrict__ sig,
+                              const T *__restrict__ lamf, const T *__restrict__ muf) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    T cs = cs0, cd = cd0;
+    if (lamf != nullptr) {
+      const T lm = lamf[i] + muf[i];
+      cs = T(-1.5) * lm / (T(3) * lm + T(1));
+      cd = T(-3) * muf[i] / (T(6) * muf[i] + T(1));
+    }
     T v[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) v[d] = f[d * N + i];
@@ -440,20 +447,29 @@ __global__ void k_elast_macro(int64_t N, T cs, T cd, const T *__restrict__ f, co
 }
 
 template <typename T>
-__global__ void k_elast_div(int64_t N, T k, const T *__restrict__ f, const T *__restrict__ fs, T *__restrict__ out) {
+__global__ void k_elast_div(int64_t N, T k0, const T *__restrict__ f, const T *__restrict__ fs, T *__restrict__ out,
+                            const T *__restrict__ lamf, const T *__restrict__ muf) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
     T v[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) v[d] = fs != nullptr ? f[d * N + i] - fs[d * N + i] : f[d * N + i];
+    const T k = lamf != nullptr ? T(-1.5) / (T(3) * (lamf[i] + muf[i]) + T(1)) : k0;
     out[i] = elast_div_of<T>(v, k);
   }
 }
 
 template <typename T>
-__global__ void k_elast_equilibrium(int64_t N, T cs, T cd, T h, T n, const T *__restrict__ eta,
+__global__ void k_elast_equilibrium(int64_t N, T cs0, T cd0, T h0, T n0, const T *__restrict__ eta,
                                     const T *__restrict__ sig, const T *__restrict__ fa, const T *__restrict__ fb,
-                                    T *__restrict__ f) {
+                                    T *__restrict__ f, const T *__restrict__ lamf, const T *__restrict__ muf) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    T cs = cs0, cd = cd0, h = h0, n = n0;
+    if (lamf != nullptr) {
+      T r0, r1;
+      elast_coef<T>(lamf[i], muf[i], r0, r1, h, n);
+      cs = T(-1) - T(1) / (T(3) * (lamf[i] + muf[i]));
+      cd = T(-1) - T(1) / (T(6) * muf[i]);
+    }
     T e0 = eta[i], e1 = eta[N + i];
     if (fa != nullptr) { e0 = e0 - T(0.5) * fa[i]; e1 = e1 - T(0.5) * fa[N + i]; }
     if (fb != nullptr) { e0 = e0 - T(0.5) * fb[i]; e1 = e1 - T(0.5) * fb[N + i]; }
@@ -643,6 +659,8 @@ template <typename T> void elast_coll(const xb_elast_args *a, ElastColl<T> &c, N
   c.fb = static_cast<const T *>(a->force_b);
   c.N = (size_t)a->nx * a->ny;
   c.has_b = a->force_b != nullptr || a->div_in != nullptr;
+  c.lamf = static_cast<const T *>(a->lam_field);
+  c.muf = static_cast<const T *>(a->mu_field);
   nl = NLDev<T>();
   if (a->div_in != nullptr) {
     nl.div_in = static_cast<const T *>(a->div_in);
@@ -702,6 +720,10 @@ int validate_elast(const xb_elast_args *a, const char *who) {
   if (a->div_in != nullptr) {
     if (int r = check_bc(a->grad_bc, true)) return r;
   }
+  if ((a->lam_field == nullptr) != (a->mu_field == nullptr))
+    return fail(XB_EINVAL, "%s: lam_field and mu_field go together", who);
+  if (a->lam_field != nullptr && a->div_in != nullptr)
+    return fail(XB_EUNSUPPORTED, "%s: per-cell Lame fields with the fused coupling force", who);
   return XB_OK;
 }
 
@@ -1052,6 +1074,209 @@ template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint (transposed) steps.  A step is S(f) = PS(Stream(PC(C(f)))) with C the
+// collision, PC / PS the post-collision / post-streaming rule tables and Stream the
+// pull f_d(x) <- p_d(x - c_d); it is affine in f.  Given abar = dJ/d(S(f)) these
+// kernels write dJ/df = C^T PC^T Stream^T PS^T abar (Stream^T pulls from x + c_d) and
+// add the parameter derivatives of this step, abar . dS/dtheta at the state f, to
+// FP64 accumulators.  One thread per cell, grid-stride; the only cross-cell reads are
+// the 8 (9) neighbours of abar and, with walls, their rule slots.
+
+// adjoint of the post-collision populations of cell i (Stream^T and PS^T)
+template <typename T, int Q, class Lat, bool HAS_BC>
+__device__ __forceinline__ void adj_post_collision(const T *__restrict__ ab, const BCDev<T> &bc, int nx, int ny,
+                                                   int x, int y, size_t i, size_t N, T (&pb)[Q]) {
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    const size_t jn = (size_t)wrapi(x + Lat::cx(d), nx) * ny + wrapi(y + Lat::cy(d), ny);
+    T v = ab[d * N + jn];
+    if constexpr (HAS_BC) {
+      if (bc.code_ps != nullptr) {
+        const int s = bc.slot[jn];
+        if (s >= 0 && bc.code_ps[(size_t)s * Q + d] != 0) v = T(0); // the neighbour's rule replaced it
+      }
+    }
+    pb[d] = v;
+  }
+  if constexpr (HAS_BC) {
+    const int s = bc.slot[i];
+    if (s >= 0 && bc.code_ps != nullptr) {
+      // post-streaming rules of this cell read its own post-collision populations
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        const int c = bc.code_ps[(size_t)s * Q + d];
+        if (c != 0 && c != XB_RULE_ZERO) {
+          const T a = ab[d * N + i];
+          const int k = (c > 0 ? c : -c) - 1;
+#pragma unroll
+          for (int e = 0; e < Q; ++e)
+            if (e == k) pb[e] = c > 0 ? pb[e] + a : pb[e] - a;
+        }
+      }
+    }
+  }
+}
+
+// PC^T: split the post-collision adjoint into the collision output's adjoint (ob) and
+// the direct pre-collision contributions of the rules (fpre)
+template <typename T, int Q, bool HAS_BC>
+__device__ __forceinline__ void adj_pc_rules(const BCDev<T> &bc, size_t i, const T (&pb)[Q], T (&ob)[Q],
+                                             T (&fpre)[Q]) {
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    ob[d] = pb[d];
+    fpre[d] = T(0);
+  }
+  if constexpr (HAS_BC) {
+    const int s = bc.slot[i];
+    if (s >= 0 && bc.code_pc != nullptr) {
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        const int c = bc.code_pc[(size_t)s * Q + d];
+        if (c != 0) {
+          ob[d] = T(0);
+          if (c != XB_RULE_ZERO) {
+            const int k = (c > 0 ? c : -c) - 1;
+#pragma unroll
+            for (int e = 0; e < Q; ++e)
+              if (e == k) fpre[e] = c > 0 ? fpre[e] + pb[d] : fpre[e] - pb[d];
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename T> struct ElastAdjDev {
+  int nx, ny;
+  T lam, mu;
+  const T *lamf, *muf;
+  const T *ab_in, *f_in;
+  T *ab_out;
+  const T *ab_add;
+  double *g_lam, *g_mu, *g_force;
+  BCDev<T> bc;
+};
+
+template <typename T, bool HAS_BC>
+__global__ void __launch_bounds__(256) k_elast_adjoint(const ElastAdjDev<T> A) {
+  using C = ElastColl<T>;
+  const int nx = A.nx, ny = A.ny;
+  const size_t N = (size_t)nx * ny;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    T pb[8], ob[8], fpre[8];
+    adj_post_collision<T, 8, C, HAS_BC>(A.ab_in, A.bc, nx, ny, x, y, i, N, pb);
+    adj_pc_rules<T, 8, HAS_BC>(A.bc, i, pb, ob, fpre);
+    T lam = A.lam, mu = A.mu;
+    if (A.lamf != nullptr) {
+      lam = A.lamf[i];
+      mu = A.muf[i];
+    }
+    T cs, cd, h, n;
+    elast_coef<T>(lam, mu, cs, cd, h, n);
+    // T^T: adjoints of the post-collision moments (M0..M6)
+    const T S03 = ob[0] + ob[1] + ob[2] + ob[3], S47 = ob[4] + ob[5] + ob[6] + ob[7];
+    const T B0 = T(0.5) * (ob[0] - ob[2]);
+    const T B1 = T(0.5) * (ob[1] - ob[3]);
+    const T B2 = h * S03 + n * S47;
+    const T B3 = T(0.25) * (ob[0] - ob[1] + ob[2] - ob[3]);
+    const T B4 = T(0.25) * (ob[4] - ob[5] + ob[6] - ob[7]);
+    const T B5 = T(0.5) * (ob[2] - ob[0]) + T(0.25) * (ob[4] - ob[5] - ob[6] + ob[7]);
+    const T B6 = T(0.5) * (ob[3] - ob[1]) + T(0.25) * (ob[4] + ob[5] - ob[6] - ob[7]);
+    // moments of the populations: m10, m01 (+F), m_s * cs, m_d * cd, m_11 * cd
+    const T b10 = B0 + B5 / T(3), b01 = B1 + B6 / T(3);
+    const T bs = cs * B2, bd = cd * B3, b11 = cd * B4;
+    T out[8];
+    out[0] = b10 + bs + bd + fpre[0];
+    out[1] = b01 + bs - bd + fpre[1];
+    out[2] = -b10 + bs + bd + fpre[2];
+    out[3] = -b01 + bs - bd + fpre[3];
+    out[4] = b10 + b01 + T(2) * bs + b11 + fpre[4];
+    out[5] = -b10 + b01 + T(2) * bs - b11 + fpre[5];
+    out[6] = -b10 - b01 + T(2) * bs + b11 + fpre[6];
+    out[7] = b10 - b01 + T(2) * bs - b11 + fpre[7];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) A.ab_out[d * N + i] = A.ab_add != nullptr ? out[d] + A.ab_add[d * N + i] : out[d];
+    if (A.g_lam != nullptr || A.g_mu != nullptr) {
+      T f[8];
+#pragma unroll
+      for (int d = 0; d < 8; ++d) f[d] = A.f_in[d * N + i];
+      const T ms = f[0] + f[1] + f[2] + f[3] + T(2) * (f[4] + f[5] + f[6] + f[7]);
+      const T md = f[0] - f[1] + f[2] - f[3];
+      const T m11 = f[4] - f[5] + f[6] - f[7];
+      const double M2 = (double)ms * (double)cs;
+      const double bcs = (double)B2 * (double)ms;
+      const double bcd = (double)B3 * (double)md + (double)B4 * (double)m11;
+      const double bh = (double)S03 * M2, bn = (double)S47 * M2;
+      const double lm = (double)lam + (double)mu;
+      double g = 1.0 / (12.0 * lm - 4.0);
+      const double dg = std::fabs(g) > 100.0 ? 0.0 : -12.0 * g * g;
+      const double dcs = 6.0 / ((3.0 * lm + 1.0) * (3.0 * lm + 1.0));
+      const double dcd = 12.0 / ((6.0 * (double)mu + 1.0) * (6.0 * (double)mu + 1.0));
+      const double bL = bcs * dcs + (0.5 * bh - 0.25 * bn) * dg;
+      if (A.g_lam != nullptr) A.g_lam[i] += bL;
+      if (A.g_mu != nullptr) A.g_mu[i] += bL + bcd * dcd;
+    }
+    if (A.g_force != nullptr) {
+      A.g_force[i] += (double)(B0 + B5 / T(6));
+      A.g_force[N + i] += (double)(B1 + B6 / T(6));
+    }
+  }
+}
+
+template <typename T> struct DarcyAdjDev {
+  int nx, ny;
+  T omega;
+  const T *omf, *sa, *sb;
+  const T *ab_in, *f_in;
+  T *ab_out;
+  const T *ab_add;
+  double *g_omega, *g_source;
+  BCDev<T> bc;
+};
+
+template <typename T, bool HAS_BC>
+__global__ void __launch_bounds__(256) k_darcy_adjoint(const DarcyAdjDev<T> A) {
+  using C = DarcyColl<T>;
+  const int nx = A.nx, ny = A.ny;
+  const size_t N = (size_t)nx * ny;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    T pb[9], ob[9], fpre[9];
+    adj_post_collision<T, 9, C, HAS_BC>(A.ab_in, A.bc, nx, ny, x, y, i, N, pb);
+    adj_pc_rules<T, 9, HAS_BC>(A.bc, i, pb, ob, fpre);
+    const T om = A.omf != nullptr ? A.omf[i] : A.omega;
+    T Sw = T(0);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) Sw = Sw + T(C::wd(d)) * ob[d];
+    // o_d = f_d - om (f_d - rho w_d) + S (1 - om/2) w_d, rho = sum f + S/2
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const T v = (T(1) - om) * ob[d] + om * Sw + fpre[d];
+      A.ab_out[d * N + i] = A.ab_add != nullptr ? v + A.ab_add[d * N + i] : v;
+    }
+    if (A.g_omega != nullptr) {
+      double rho = 0.0, fo = 0.0;
+      T f[9];
+#pragma unroll
+      for (int d = 0; d < 9; ++d) {
+        f[d] = A.f_in[d * N + i];
+        rho += (double)f[d];
+        fo += (double)f[d] * (double)ob[d];
+      }
+      double S = 0.0;
+      if (A.sa != nullptr) S += (double)A.sa[i];
+      if (A.sb != nullptr) S += (double)A.sb[i];
+      rho += 0.5 * S;
+      // d o_d / d om = rho w_d - f_d - S w_d / 2
+      A.g_omega[i] += (double)Sw * rho - fo - 0.5 * S * (double)Sw;
+    }
+    if (A.g_source != nullptr) A.g_source[i] += (double)Sw;
+  }
+}
+
 } // namespace
 
 // ============================================================================
@@ -1116,43 +1341,53 @@ int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, 
 }
 
 int xb_elast_macro(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *f, const void *force_a,
-                   const void *force_b, void *eta_out, void *sigma_out, void *stream) {
+                   const void *force_b, void *eta_out, void *sigma_out, const void *lam_field, const void *mu_field,
+                   void *stream) {
   if (int r = check_dtype(dtype)) return r;
   if (nx <= 0 || ny <= 0 || f == nullptr) return fail(XB_EINVAL, "xb_elast_macro: bad arguments");
+  if ((lam_field == nullptr) != (mu_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_macro: lam_field and mu_field go together");
   const int64_t N = (int64_t)nx * ny;
   const double cs = -1.5 * (lam + mu) / (3 * (lam + mu) + 1);
   const double cd = -3 * mu / (6 * mu + 1);
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == XB_F64)
     hipLaunchKernelGGL(k_elast_macro<double>, ew_grid(N), dim3(256), 0, st, N, cs, cd, (const double *)f,
-                       (const double *)force_a, (const double *)force_b, (double *)eta_out, (double *)sigma_out);
+                       (const double *)force_a, (const double *)force_b, (double *)eta_out, (double *)sigma_out,
+                       (const double *)lam_field, (const double *)mu_field);
   else
     hipLaunchKernelGGL(k_elast_macro<float>, ew_grid(N), dim3(256), 0, st, N, (float)cs, (float)cd, (const float *)f,
-                       (const float *)force_a, (const float *)force_b, (float *)eta_out, (float *)sigma_out);
+                       (const float *)force_a, (const float *)force_b, (float *)eta_out, (float *)sigma_out,
+                       (const float *)lam_field, (const float *)mu_field);
   return check_launch("xb_elast_macro");
 }
 
 int xb_elast_divergence(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *f, const void *f_sub,
-                        void *div_out, void *stream) {
+                        void *div_out, const void *lam_field, const void *mu_field, void *stream) {
   if (int r = check_dtype(dtype)) return r;
   if (nx <= 0 || ny <= 0 || f == nullptr || div_out == nullptr) return fail(XB_EINVAL, "xb_elast_divergence: bad arguments");
+  if ((lam_field == nullptr) != (mu_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_divergence: lam_field and mu_field go together");
   const int64_t N = (int64_t)nx * ny;
   const double k = -1.5 / (3 * (lam + mu) + 1);
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == XB_F64)
     hipLaunchKernelGGL(k_elast_div<double>, ew_grid(N), dim3(256), 0, st, N, k, (const double *)f,
-                       (const double *)f_sub, (double *)div_out);
+                       (const double *)f_sub, (double *)div_out, (const double *)lam_field, (const double *)mu_field);
   else
     hipLaunchKernelGGL(k_elast_div<float>, ew_grid(N), dim3(256), 0, st, N, (float)k, (const float *)f,
-                       (const float *)f_sub, (float *)div_out);
+                       (const float *)f_sub, (float *)div_out, (const float *)lam_field, (const float *)mu_field);
   return check_launch("xb_elast_divergence");
 }
 
 int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *eta,
-                         const void *sigma, const void *force_a, const void *force_b, void *f_out, void *stream) {
+                         const void *sigma, const void *force_a, const void *force_b, void *f_out,
+                         const void *lam_field, const void *mu_field, void *stream) {
   if (int r = check_dtype(dtype)) return r;
   if (nx <= 0 || ny <= 0 || eta == nullptr || sigma == nullptr || f_out == nullptr)
     return fail(XB_EINVAL, "xb_elast_equilibrium: bad arguments");
+  if ((lam_field == nullptr) != (mu_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_equilibrium: lam_field and mu_field go together");
   const int64_t N = (int64_t)nx * ny;
   double g = 1.0 / (12.0 * (lam + mu) - 4.0);
   if (std::fabs(g) > 100.0) g = 0.0;
@@ -1162,11 +1397,12 @@ int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, doub
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == XB_F64)
     hipLaunchKernelGGL(k_elast_equilibrium<double>, ew_grid(N), dim3(256), 0, st, N, cs, cd, h, n, (const double *)eta,
-                       (const double *)sigma, (const double *)force_a, (const double *)force_b, (double *)f_out);
+                       (const double *)sigma, (const double *)force_a, (const double *)force_b, (double *)f_out,
+                       (const double *)lam_field, (const double *)mu_field);
   else
     hipLaunchKernelGGL(k_elast_equilibrium<float>, ew_grid(N), dim3(256), 0, st, N, (float)cs, (float)cd, (float)h,
                        (float)n, (const float *)eta, (const float *)sigma, (const float *)force_a,
-                       (const float *)force_b, (float *)f_out);
+                       (const float *)force_b, (float *)f_out, (const float *)lam_field, (const float *)mu_field);
   return check_launch("xb_elast_equilibrium");
 }
 
@@ -1329,6 +1565,67 @@ int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converg
                      ninst, tol, max_cycles, state->r0, state->rlast, state->cycles, state->active, state->status,
                      state->nactive, state->rref);
   return check_launch("xb_batch_converge");
+}
+
+int xb_elast_adjoint(const xb_elast_adjoint_args *a, void *stream) {
+  if (a == nullptr) return fail(XB_EINVAL, "xb_elast_adjoint: null args");
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->nx <= 0 || a->ny <= 0 || a->abar_in == nullptr || a->abar_out == nullptr)
+    return fail(XB_EINVAL, "xb_elast_adjoint: bad arguments");
+  if (a->abar_in == a->abar_out) return fail(XB_EINVAL, "xb_elast_adjoint: abar_in and abar_out must differ");
+  if ((a->grad_lam != nullptr || a->grad_mu != nullptr) && a->f_in == nullptr)
+    return fail(XB_EINVAL, "xb_elast_adjoint: parameter gradients need f_in");
+  if ((a->lam_field == nullptr) != (a->mu_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_adjoint: lam_field and mu_field go together");
+  if (int r = check_bc(a->bc, false)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool has_bc = a->bc != nullptr && a->bc->cell_slot != nullptr;
+  const dim3 gr = ew_grid((int64_t)a->nx * a->ny);
+  if (a->dtype == XB_F64) {
+    ElastAdjDev<double> A{a->nx, a->ny, a->lam, a->mu, (const double *)a->lam_field, (const double *)a->mu_field,
+                          (const double *)a->abar_in, (const double *)a->f_in, (double *)a->abar_out,
+                          (const double *)a->abar_add, a->grad_lam, a->grad_mu, a->grad_force, bcdev<double>(a->bc)};
+    if (has_bc) hipLaunchKernelGGL((k_elast_adjoint<double, true>), gr, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_elast_adjoint<double, false>), gr, dim3(256), 0, st, A);
+  } else {
+    ElastAdjDev<float> A{a->nx, a->ny, (float)a->lam, (float)a->mu, (const float *)a->lam_field,
+                         (const float *)a->mu_field, (const float *)a->abar_in, (const float *)a->f_in,
+                         (float *)a->abar_out, (const float *)a->abar_add, a->grad_lam, a->grad_mu, a->grad_force,
+                         bcdev<float>(a->bc)};
+    if (has_bc) hipLaunchKernelGGL((k_elast_adjoint<float, true>), gr, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_elast_adjoint<float, false>), gr, dim3(256), 0, st, A);
+  }
+  return check_launch("xb_elast_adjoint");
+}
+
+int xb_darcy_adjoint(const xb_darcy_adjoint_args *a, void *stream) {
+  if (a == nullptr) return fail(XB_EINVAL, "xb_darcy_adjoint: null args");
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->nx <= 0 || a->ny <= 0 || a->abar_in == nullptr || a->abar_out == nullptr)
+    return fail(XB_EINVAL, "xb_darcy_adjoint: bad arguments");
+  if (a->abar_in == a->abar_out) return fail(XB_EINVAL, "xb_darcy_adjoint: abar_in and abar_out must differ");
+  if (a->grad_omega != nullptr && a->f_in == nullptr)
+    return fail(XB_EINVAL, "xb_darcy_adjoint: the relaxation-rate gradient needs f_in");
+  if (int r = check_bc(a->bc, false)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool has_bc = a->bc != nullptr && a->bc->cell_slot != nullptr;
+  const dim3 gr = ew_grid((int64_t)a->nx * a->ny);
+  if (a->dtype == XB_F64) {
+    DarcyAdjDev<double> A{a->nx, a->ny, a->omega, (const double *)a->omega_field, (const double *)a->source_a,
+                          (const double *)a->source_b, (const double *)a->abar_in, (const double *)a->f_in,
+                          (double *)a->abar_out, (const double *)a->abar_add, a->grad_omega, a->grad_source,
+                          bcdev<double>(a->bc)};
+    if (has_bc) hipLaunchKernelGGL((k_darcy_adjoint<double, true>), gr, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_darcy_adjoint<double, false>), gr, dim3(256), 0, st, A);
+  } else {
+    DarcyAdjDev<float> A{a->nx, a->ny, (float)a->omega, (const float *)a->omega_field, (const float *)a->source_a,
+                         (const float *)a->source_b, (const float *)a->abar_in, (const float *)a->f_in,
+                         (float *)a->abar_out, (const float *)a->abar_add, a->grad_omega, a->grad_source,
+                         bcdev<float>(a->bc)};
+    if (has_bc) hipLaunchKernelGGL((k_darcy_adjoint<float, true>), gr, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_darcy_adjoint<float, false>), gr, dim3(256), 0, st, A);
+  }
+  return check_launch("xb_darcy_adjoint");
 }
 
 int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream) {

@@ -33,7 +33,7 @@ class ElastArgs(ctypes.Structure):
                 ("bc", ctypes.POINTER(BCTable)), ("coarse_out", c_void_p), ("norm_partials", c_void_p),
                 ("rows", c_int32), ("pad_", c_int32),
                 ("div_in", c_void_p), ("div_out", c_void_p), ("nl_scale", c_double),
-                ("grad_bc", ctypes.POINTER(BCTable))]
+                ("grad_bc", ctypes.POINTER(BCTable)), ("lam_field", c_void_p), ("mu_field", c_void_p)]
 
 
 class DarcyArgs(ctypes.Structure):
@@ -42,6 +42,21 @@ class DarcyArgs(ctypes.Structure):
                 ("f_in", c_void_p), ("f_out", c_void_p), ("fpc_out", c_void_p),
                 ("source_a", c_void_p), ("source_b", c_void_p), ("bc", ctypes.POINTER(BCTable)),
                 ("rows", c_int32), ("pad2_", c_int32)]
+
+
+class ElastAdjointArgs(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("dtype", c_int32), ("pad_", c_int32),
+                ("lam", c_double), ("mu", c_double), ("lam_field", c_void_p), ("mu_field", c_void_p),
+                ("abar_in", c_void_p), ("f_in", c_void_p), ("abar_out", c_void_p), ("abar_add", c_void_p),
+                ("grad_lam", c_void_p), ("grad_mu", c_void_p), ("grad_force", c_void_p),
+                ("bc", ctypes.POINTER(BCTable))]
+
+
+class DarcyAdjointArgs(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("dtype", c_int32), ("pad_", c_int32),
+                ("omega", c_double), ("omega_field", c_void_p), ("source_a", c_void_p), ("source_b", c_void_p),
+                ("abar_in", c_void_p), ("f_in", c_void_p), ("abar_out", c_void_p), ("abar_add", c_void_p),
+                ("grad_omega", c_void_p), ("grad_source", c_void_p), ("bc", ctypes.POINTER(BCTable))]
 
 
 XB_BATCH_ELAST_STEP, XB_BATCH_PROLONG_DIV, XB_BATCH_FILL, XB_BATCH_NORM = 1, 2, 3, 4
@@ -84,11 +99,13 @@ SIGNATURES = {
                                            c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, c_void_p,
                                            c_void_p]),
     "xb_elast_macro": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p, c_void_p,
-                                 c_void_p, c_void_p, c_void_p]),
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "xb_elast_divergence": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
-                                      c_void_p, c_void_p]),
+                                      c_void_p, c_void_p, c_void_p, c_void_p]),
     "xb_elast_equilibrium": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
-                                       c_void_p, c_void_p, c_void_p, c_void_p]),
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "xb_elast_adjoint": (c_int32, [ctypes.POINTER(ElastAdjointArgs), c_void_p]),
+    "xb_darcy_adjoint": (c_int32, [ctypes.POINTER(DarcyAdjointArgs), c_void_p]),
     "xb_darcy_step": (c_int32, [ctypes.POINTER(DarcyArgs), c_void_p]),
     "xb_darcy_macro": (c_int32, [c_int32, c_int32, c_int32, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),

@@ -360,7 +360,7 @@ class ElasticityBatch:
             if self.fused:
                 el = I.el
                 nat.check(nat.lib().xb_elast_divergence(el.nx, el.ny, nat.dtype_code(self.dtype), el.lamda, el.mu,
-                                                        nat.ptr(I.X[0]), None, nat.ptr(I.dX[0]), nat.stream()),
+                                                        nat.ptr(I.X[0]), None, nat.ptr(I.dX[0]), None, None, nat.stream()),
                           "xb_elast_divergence")
 
     def solve(self, states, forces=None, active=None):

@@ -57,10 +57,30 @@ class LBMElasticity(object):
         force = self.get_force()
         self._force_soa = None if force is None else as_soa(force, self.nx, self.ny, 2, self._tdt, self.device)
         self.force = None if self._force_soa is None else view(self._force_soa)
+        # extension: per-cell Lame parameters (nx, ny[, 1]) on the single grid; they
+        # replace lambda / mu in the collision, stresses, divergence and equilibrium
+        self._lam_field = self._mu_field = None
+        lf, mf = kwargs.get("lambda_field"), kwargs.get("mu_field")
+        if (lf is None) != (mf is None):
+            raise ValueError("lambda_field and mu_field go together")
+        if lf is not None:
+            if self.levels != 1:
+                raise NotImplementedError("per-cell Lame fields are supported on the single grid (MG_levels = 1)")
+            self.set_lame_fields(lf, mf)
         self._norm_ws = {}
         self._runners = {}
         self.mlups = None
         self.last_residual = None
+
+    def set_lame_fields(self, lam_field, mu_field):
+        """Per-cell Lame parameters ``(nx, ny[, 1])`` (arrays or device tensors; a
+        tensor of the solver's dtype and layout is used without a copy)."""
+        self._lam_field = as_soa(lam_field, self.nx, self.ny, 1, self._tdt, self.device)
+        self._mu_field = as_soa(mu_field, self.nx, self.ny, 1, self._tdt, self.device)
+        self._runners = {}
+
+    def _fields(self):
+        return nat.ptr(self._lam_field), nat.ptr(self._mu_field)
 
     @property
     def _tdt(self):
@@ -328,6 +348,8 @@ class LBMElasticity(object):
                           x0, x1, y0, y1, table.ref() if table is not None else None,
                           nat.ptr(coarse), nat.ptr(partials),
                           autotune.lookup(self._tune_key(level, mode, ext is not None, resid is not None)), 0)
+        if level == 0 and self._lam_field is not None:
+            a.lam_field, a.mu_field = self._fields()
         return a
 
     def autotune(self, levels=None, external_force=False):
@@ -441,7 +463,8 @@ class LBMElasticity(object):
         f = self._new(8)
         nat.check(nat.lib().xb_elast_equilibrium(self.nx, self.ny, nat.dtype_code(self._tdt), self.lamda, self.mu,
                                                  nat.ptr(eta_s), nat.ptr(sig_s), nat.ptr(self._force_soa),
-                                                 nat.ptr(ext), nat.ptr(f), nat.stream()), "xb_elast_equilibrium")
+                                                 nat.ptr(ext), nat.ptr(f), *self._fields(), nat.stream()),
+                  "xb_elast_equilibrium")
         return view(f)
 
     def update_macroscopic(self, f, external_force=None):
@@ -450,14 +473,16 @@ class LBMElasticity(object):
         eta, sig = self._new(2), self._new(3)
         nat.check(nat.lib().xb_elast_macro(self.nx, self.ny, nat.dtype_code(self._tdt), self.lamda, self.mu,
                                            nat.ptr(fs), nat.ptr(self._force_soa), nat.ptr(ext), nat.ptr(eta),
-                                           nat.ptr(sig), nat.stream()), "xb_elast_macro")
+                                           nat.ptr(sig), *self._fields(), nat.stream()), "xb_elast_macro")
         return view(eta), view(sig)
 
     def _divergence_soa(self, f, f_sub=None):
         nx, ny = f.shape[1], f.shape[2]
         out = torch.empty((1, nx, ny), dtype=f.dtype, device=f.device)
+        fields = self._fields() if (nx, ny) == (self.nx, self.ny) else (None, None)
         nat.check(nat.lib().xb_elast_divergence(nx, ny, nat.dtype_code(f.dtype), self.lamda, self.mu, nat.ptr(f),
-                                                nat.ptr(f_sub), nat.ptr(out), nat.stream()), "xb_elast_divergence")
+                                                nat.ptr(f_sub), nat.ptr(out), *fields, nat.stream()),
+                  "xb_elast_divergence")
         return out
 
     def update_divergence(self, f):
@@ -738,7 +763,7 @@ class VCycleGraph:
             el = self.el
             nat.check(nat.lib().xb_elast_divergence(el.nx, el.ny, nat.dtype_code(f_soa.dtype), el.lamda, el.mu,
                                                     nat.ptr(self.buf[self.cur]), None, nat.ptr(self.div[self.cur]),
-                                                    nat.stream()), "xb_elast_divergence")
+                                                    None, None, nat.stream()), "xb_elast_divergence")
 
     def set_force(self, force_soa):
         if self.ext is None:

@@ -266,7 +266,7 @@ class LBMPoroelasticity(object):
         sigma = el._new(3)
         nat.check(nat.lib().xb_elast_macro(el.nx, el.ny, nat.dtype_code(self._tdt), el.lamda, el.mu, nat.ptr(fe),
                                            nat.ptr(el._force_soa), nat.ptr(ext), nat.ptr(eta), nat.ptr(sigma),
-                                           nat.stream()), "xb_elast_macro")
+                                           *el._fields(), nat.stream()), "xb_elast_macro")
         dde = el._divergence_soa(fe, fprev)
         source = self._lincomb(1.0, source_base, -(th * a / eps), dde)
         pressure, _ = da._macro_soa(fd, source, True, False)
@@ -429,7 +429,7 @@ class _PlainInnerGraph:
         if self.fused:
             el = self.el
             nat.check(nat.lib().xb_elast_divergence(el.nx, el.ny, nat.dtype_code(f.dtype), el.lamda, el.mu,
-                                                    nat.ptr(self.a), None, nat.ptr(self.da), nat.stream()),
+                                                    nat.ptr(self.a), None, nat.ptr(self.da), *el._fields(), nat.stream()),
                       "xb_elast_divergence")
 
     def set_force(self, force):
