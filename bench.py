@@ -326,6 +326,101 @@ def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smo
             "per_instance": [[s_["n"], L, [int(c) for c in p.inner_counts]] for s_, L, p in zip(specs, levels, seq)]}
 
 
+def _eta(f):
+    """Displacement moments (m10, m01) of SoA D2Q8 populations, differentiable."""
+    import torch
+    e0 = f[0] - f[2] + f[4] - f[5] - f[6] + f[7]
+    e1 = f[1] - f[3] + f[4] + f[5] - f[6] - f[7]
+    return torch.stack((e0, e1))
+
+
+def bench_adjoint(torch, n=1024, steps=1000, K=50, precision="f32/f32"):
+    """Gradient cost against forward cost on the 2-D consolidation geometry.
+
+    Elasticity: (n) x (n + 2) cells, fixed bottom, traction top, periodic in x, per-cell
+    lambda = 2 and mu = 1 fields; misfit 1/2 |eta - eta*|^2 of the displacement after
+    ``steps`` pseudo-steps, eta* from the same solve with mu perturbed by a seeded
+    log-normal factor (seed 2).  Darcy: n x n (+ ghost layers), log-normal K (variance
+    2, correlation 32, seed 1), omega = 1/(3K + 1/2); misfit 1/2 |p - p*|^2, p* from
+    K perturbed (seed 2).  Gradients with respect to mu(x) and K(x) through the
+    checkpointed adjoint (checkpoints every K steps, graph-replayed segments); the
+    state is FP32, the gradient accumulators FP64."""
+    import problems as P
+    from src.adjoint import darcy_solve, elasticity_solve, ElasticityStepper, DarcyStepper
+    out = {}
+    S = 8 if precision == "f64/f64" else 4
+    dt = torch.float64 if S == 8 else torch.float32
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    # ---- elasticity
+    el = P.consolidation_elasticity("gpu", nx=n, precision=precision)
+    nx, ny = el.nx, el.ny
+    lam = torch.full((nx, ny), 2.0, dtype=dt, device="cuda")
+    mu_t = torch.full((nx, ny), 1.0, dtype=dt, device="cuda")
+    pert = torch.as_tensor(P.lognormal_field(nx, ny, 0.1, 32.0, 2), dtype=dt, device="cuda")
+    f0 = torch.zeros((8, nx, ny), dtype=dt, device="cuda")
+    with torch.no_grad():
+        target = _eta(elasticity_solve(el, lam, mu_t * pert, f0, steps, checkpoint_every=K))
+    mu = mu_t.clone().requires_grad_(True)
+    # warm-up forward + backward: the solve's buffers and graphs are captured once
+    (0.5 * ((_eta(elasticity_solve(el, lam, mu, f0, steps, checkpoint_every=K)) - target) ** 2).sum()).backward()
+    mu.grad = None
+    fN, t_fwd = timed(lambda: elasticity_solve(el, lam, mu, f0, steps, checkpoint_every=K))
+    J = 0.5 * ((_eta(fN) - target) ** 2).sum()
+    _, t_bwd = timed(lambda: J.backward())
+    # plain forward (no checkpoints) for reference
+    _, t_plain = timed(lambda: el.run_steps(f0.permute(1, 2, 0), steps, chunk=K))
+    # adjoint kernel alone
+    st = ElasticityStepper(el, lam.detach(), mu.detach())
+    a0, a1 = torch.randn_like(f0), torch.empty_like(f0)
+    g = st.grad_buffers(False)
+    kern = event_pair_time(torch, lambda: st.adjoint(a0, f0, a1, g), 50)
+    adj_bytes = nx * ny * (26 * S + 32)
+    out["elasticity"] = {"grid": [nx, ny], "steps": steps, "checkpoint_every": K, "precision": precision,
+                         "forward_s": round(t_fwd, 4), "plain_forward_s": round(t_plain, 4),
+                         "gradient_s": round(t_bwd, 4), "gradient_over_forward": round(t_bwd / t_fwd, 3),
+                         "adjoint_kernel_us": round(kern * 1e6, 2),
+                         "adjoint_kernel_gbs": round(adj_bytes / kern / 1e9, 1),
+                         "adjoint_bytes_per_cell": 26 * S + 32,
+                         "grad_norm": float(mu.grad.double().norm()), "misfit": float(J)}
+    log(f"  elasticity adjoint: forward {t_fwd:.3f} s, gradient {t_bwd:.3f} s")
+
+    # ---- Darcy
+    d = P.darcy_c2("gpu", n=n, precision=precision, heterogeneous=False)
+    nxd, nyd = d.nx, d.ny
+    Kf = torch.as_tensor(P.lognormal_field(nxd, nyd, 2.0, 32.0, 1), dtype=dt, device="cuda")
+    Kp = Kf * torch.as_tensor(P.lognormal_field(nxd, nyd, 0.1, 32.0, 2), dtype=dt, device="cuda")
+    fd0 = torch.as_tensor(d.assign_fields_sharded()).to(dt).permute(2, 0, 1).contiguous()
+    with torch.no_grad():
+        ptarget = darcy_solve(d, 1.0 / (3.0 * Kp + 0.5), fd0, steps, checkpoint_every=K).sum(0)
+    Kg = Kf.clone().requires_grad_(True)
+    (0.5 * ((darcy_solve(d, 1.0 / (3.0 * Kg + 0.5), fd0, steps, checkpoint_every=K).sum(0) - ptarget) ** 2)
+     .sum()).backward()
+    Kg.grad = None
+    fdN, t_fwd = timed(lambda: darcy_solve(d, 1.0 / (3.0 * Kg + 0.5), fd0, steps, checkpoint_every=K))
+    Jd = 0.5 * ((fdN.sum(0) - ptarget) ** 2).sum()
+    _, t_bwd = timed(lambda: Jd.backward())
+    std = DarcyStepper(d, (1.0 / (3.0 * Kf + 0.5)).detach())
+    b0, b1 = torch.randn_like(fd0), torch.empty_like(fd0)
+    gd = std.grad_buffers(False)
+    kern = event_pair_time(torch, lambda: std.adjoint(b0, fd0, b1, gd), 50)
+    adj_bytes = nxd * nyd * (28 * S + 16)
+    out["darcy"] = {"grid": [nxd, nyd], "steps": steps, "checkpoint_every": K, "precision": precision,
+                    "forward_s": round(t_fwd, 4), "gradient_s": round(t_bwd, 4),
+                    "gradient_over_forward": round(t_bwd / t_fwd, 3),
+                    "adjoint_kernel_us": round(kern * 1e6, 2), "adjoint_kernel_gbs": round(adj_bytes / kern / 1e9, 1),
+                    "adjoint_bytes_per_cell": 28 * S + 16,
+                    "grad_norm": float(Kg.grad.double().norm()), "misfit": float(Jd)}
+    log(f"  darcy adjoint: forward {t_fwd:.3f} s, gradient {t_bwd:.3f} s")
+    return out
+
+
 def cpu_baseline(n, precision, seconds):
     import torch
     import problems as P
@@ -358,6 +453,8 @@ def only(torch, args):
         log(f"copy: {args.steps} launches of {a.numel() * 8} B")
     elif args.only == "c5":
         print(json.dumps(bench_c5(torch, n_inst=args.instances)), flush=True)
+    elif args.only == "adjoint":
+        print(json.dumps(bench_adjoint(torch)), flush=True)
     elif args.only == "large_elast":
         r = bench_large_elasticity(torch, n, args.steps, 2)
         log(f"elasticity {n}^2: {r['kernel_s'] * 1e6:.1f} us/launch, {r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
@@ -376,7 +473,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched Biot extra")
-    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c5"],
+    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c5", "adjoint"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -434,6 +531,8 @@ def main():
               lambda: bench_c3(torch, levels=8, steps=10, lam=lam, mu=mu, K=0.1))
         log(f"batched Biot: {args.instances} instances vs one after another")
         extra("c5_batched_biot_fp64", lambda: bench_c5(torch, n_inst=args.instances))
+        log("adjoint: consolidation geometry 1024, FP32, 1000 pseudo-steps, checkpoints every 50")
+        extra("adjoint_1024_fp32", lambda: bench_adjoint(torch))
         for n in (4096,):
             log(f"elasticity {n}^2 fp64 (HBM-resident)")
             rl = bench_large_elasticity(torch, n, 200, 20)

@@ -228,3 +228,32 @@ def test_steady_state_adjoint_matches_differences_of_converged_solves():
         fd = (Jp - Jm) / (2 * eps)
         ad = _dot(grads["mu"], dmu)
         assert abs(fd - ad) <= 1e-6 * abs(ad), (k, fd, ad)
+
+
+def test_reused_solve_buffers_guard_and_private_solves():
+    from src.adjoint import elasticity_solve
+    say("adjoint session reuse guard")
+    nx, n, K = 12, 20, 10
+    el = P.consolidation_elasticity("gpu", nx=nx)
+    ny = el.ny
+    lam, mu1 = (_cuda(t) for t in _fields(nx, ny, 71))
+    mu2 = mu1 * 1.1
+    f0 = torch.zeros((8, nx, ny), dtype=torch.float64, device="cuda")
+    target = _rand((8, nx, ny), 72, 1e-4)
+
+    def J(mu, reuse=True):
+        return 0.5 * ((elasticity_solve(el, lam, mu, f0, n, checkpoint_every=K, reuse=reuse) - target) ** 2).sum()
+    grads = []
+    for m in (mu1, mu2):
+        mm = m.clone().requires_grad_(True)
+        J(mm).backward()
+        grads.append(mm.grad.clone())
+    a = mu1.clone().requires_grad_(True)
+    b = mu2.clone().requires_grad_(True)
+    Ja, Jb = J(a), J(b)
+    with pytest.raises(RuntimeError, match="overwritten"):
+        Ja.backward()
+    a.grad = None
+    Ja, Jb = J(a, reuse=False), J(b, reuse=False)
+    (Ja + Jb).backward()
+    assert torch.equal(a.grad, grads[0]) and torch.equal(b.grad, grads[1])

@@ -19,7 +19,11 @@ Backward is the discrete adjoint of exactly those ``n`` steps.  It recomputes th
 states from checkpoints stored every ``checkpoint_every`` steps (one segment of
 states is held at a time, so memory stays at ``n / K`` checkpoints plus ``K``
 states), and replays the recompute and the adjoint steps of a segment as captured
-HIP graphs.  :func:`elasticity_steady_adjoint` / :func:`darcy_steady_adjoint` give
+HIP graphs.  The buffers and graphs of a solve are kept on the solver object per
+(n, K, operands) and reused by later calls (an optimisation loop captures once);
+a call's gradients are returned as new tensors.  A later solve of the same key
+overwrites the checkpoints of an earlier one: the earlier one's backward then
+raises (solve with ``reuse=False`` to keep several solves alive at once).  :func:`elasticity_steady_adjoint` / :func:`darcy_steady_adjoint` give
 the gradient of a converged steady state by the fixed-point (implicit) adjoint:
 iterate ``a <- (dS/df)^T a + dJ/df`` to convergence, then one parameter pass.
 """
@@ -57,18 +61,27 @@ class ElasticityStepper:
         if el.levels != 1:
             raise ValueError("adjoint steps are single-grid (MG_levels = 1)")
         self.el = el
-        self.lam = _soa(lam_field, 1, el.nx, el.ny, el._tdt, el.device)
-        self.mu = _soa(mu_field, 1, el.nx, el.ny, el._tdt, el.device)
-        el._lam_field, el._mu_field = self.lam, self.mu
-        el._runners = {}
-        self.force = _soa(force, 2, el.nx, el.ny, el._tdt, el.device)
+        # owned copies: captured graphs keep these addresses, bind() refreshes values
+        self.lam = _soa(lam_field, 1, el.nx, el.ny, el._tdt, el.device).clone()
+        self.mu = _soa(mu_field, 1, el.nx, el.ny, el._tdt, el.device).clone()
+        self.force = None if force is None else _soa(force, 2, el.nx, el.ny, el._tdt, el.device).clone()
         self.shape = (8, el.nx, el.ny)
         self.dtype, self.device = el._tdt, el.device
         table = el._tables.get(0)
         self._bc = table.ref() if table is not None else None
 
+    def bind(self, lam_field, mu_field, force=None):
+        el = self.el
+        self.lam.copy_(_soa(lam_field, 1, el.nx, el.ny, el._tdt, el.device))
+        self.mu.copy_(_soa(mu_field, 1, el.nx, el.ny, el._tdt, el.device))
+        if self.force is not None:
+            self.force.copy_(_soa(force, 2, el.nx, el.ny, el._tdt, el.device))
+
     def step(self, src, dst):
-        return self.el._step_into(src, dst, self.force)
+        a = self.el._launch(src, dst, 0, nat.XB_MODE_STEP, ext=self.force)
+        a.lam_field, a.mu_field = nat.ptr(self.lam), nat.ptr(self.mu)
+        nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step")
+        return dst
 
     def adjoint(self, ab_in, f_in, ab_out, grads=None, ab_add=None):
         """``ab_out = (dS/df)^T ab_in (+ ab_add)``; with ``grads`` (dict of FP64
@@ -98,17 +111,22 @@ class DarcyStepper:
 
     def __init__(self, da, omega_field, source=None):
         self.da = da
-        self.omega = _soa(omega_field, 1, da.nx, da.ny, da._tdt, da.device)
-        da._omega_field = self.omega
-        da._runners = {}
-        self.source = _soa(source, 1, da.nx, da.ny, da._tdt, da.device)
+        self.omega = _soa(omega_field, 1, da.nx, da.ny, da._tdt, da.device).clone()
+        self.source = None if source is None else _soa(source, 1, da.nx, da.ny, da._tdt, da.device).clone()
         self.shape = (9, da.nx, da.ny)
         self.dtype, self.device = da._tdt, da.device
         self._bc = da._bc_table.ref() if da._bc_table is not None else None
 
+    def bind(self, omega_field, source=None):
+        da = self.da
+        self.omega.copy_(_soa(omega_field, 1, da.nx, da.ny, da._tdt, da.device))
+        if self.source is not None:
+            self.source.copy_(_soa(source, 1, da.nx, da.ny, da._tdt, da.device))
+
     def step(self, src, dst):
-        nat.check(nat.lib().xb_darcy_step(self.da._args(src, dst, self.source, None), nat.stream()),
-                  "xb_darcy_step")
+        a = self.da._args(src, dst, self.source, None)
+        a.omega_field = nat.ptr(self.omega)
+        nat.check(nat.lib().xb_darcy_step(a, nat.stream()), "xb_darcy_step")
         return dst
 
     def adjoint(self, ab_in, f_in, ab_out, grads=None, ab_add=None):
@@ -147,9 +165,23 @@ class CheckpointedSolve:
         self.ab = [torch.empty(shape, dtype=dt, device=dev) for _ in range(2)]
         self._fwd_graph = self._rec_graph = self._adj_graph = None
         self._grads = None
+        self.grads = None  # FP64 accumulators owned by the session (see gradient_buffers)
+        self._grads_extra = None
+        self.generation = 0  # forward passes run; backward checks it saw the last one
+
+    def gradient_buffers(self, want_extra):
+        """The session's FP64 accumulators, zeroed (kept across calls so that the
+        captured backward graphs stay valid)."""
+        if self.grads is None or self._grads_extra != bool(want_extra):
+            self.grads = self.s.grad_buffers(want_extra)
+            self._grads_extra = bool(want_extra)
+        for g in self.grads.values():
+            g.zero_()
+        return self.grads
 
     # forward
     def forward(self, f0):
+        self.generation += 1
         s, K = self.s, self.K
         a = self.states[0]
         b = self.states[1] if K > 1 else torch.empty_like(a)
@@ -229,14 +261,27 @@ class CheckpointedSolve:
         return self.ab[0].clone()
 
 
+def _check_generation(ctx):
+    if ctx.cs.generation != ctx.generation:
+        raise RuntimeError("the checkpoints of this solve were overwritten by a later solve of the same solver, "
+                           "steps and interval; run its backward first, or solve with reuse=False")
+
+
 class _ElasticitySolveFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, lam_field, mu_field, force, f0, el, n, K, use_graphs):
-        st = ElasticityStepper(el, lam_field.detach(), mu_field.detach(),
-                               None if force is None else force.detach())
-        cs = CheckpointedSolve(st, n, K, use_graphs)
+    def forward(ctx, lam_field, mu_field, force, f0, el, n, K, use_graphs, reuse):
+        key = ("elasticity", n, K, use_graphs, force is not None)
+        sessions = el.__dict__.setdefault("_adjoint_sessions", {}) if reuse else {}
+        if key in sessions:
+            cs = sessions[key]
+            st = cs.s
+            st.bind(lam_field.detach(), mu_field.detach(), None if force is None else force.detach())
+        else:
+            st = ElasticityStepper(el, lam_field.detach(), mu_field.detach(),
+                                   None if force is None else force.detach())
+            cs = sessions[key] = CheckpointedSolve(st, n, K, use_graphs)
         out = cs.forward(_soa(f0.detach(), 8, el.nx, el.ny, el._tdt, el.device))
-        ctx.cs, ctx.st = cs, st
+        ctx.cs, ctx.st, ctx.generation = cs, st, cs.generation
         ctx.has_force = force is not None
         ctx.shapes = (lam_field.shape, mu_field.shape, None if force is None else force.shape, f0.shape)
         ctx.dtypes = (lam_field.dtype, mu_field.dtype, None if force is None else force.dtype, f0.dtype)
@@ -245,8 +290,9 @@ class _ElasticitySolveFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         st, cs = ctx.st, ctx.cs
+        _check_generation(ctx)
         need_force = ctx.has_force and ctx.needs_input_grad[2]
-        grads = st.grad_buffers(need_force)
+        grads = cs.gradient_buffers(need_force)
         ab0 = cs.backward(grad_out.contiguous().to(st.dtype), grads)
         sl, sm, sf, s0 = ctx.shapes
         dl, dm, df, d0 = ctx.dtypes
@@ -254,24 +300,32 @@ class _ElasticitySolveFn(torch.autograd.Function):
         g_mu = grads["mu"].reshape(sm).to(dm) if ctx.needs_input_grad[1] else None
         g_f = grads["force"].reshape(sf).to(df) if need_force else None
         g_0 = ab0.reshape(s0).to(d0) if ctx.needs_input_grad[3] else None
-        return g_lam, g_mu, g_f, g_0, None, None, None, None
+        return g_lam, g_mu, g_f, g_0, None, None, None, None, None
 
 
-def elasticity_solve(el, lam_field, mu_field, f0, n, force=None, checkpoint_every=50, use_graphs=True):
+def elasticity_solve(el, lam_field, mu_field, f0, n, force=None, checkpoint_every=50, use_graphs=True, reuse=True):
     """Populations after ``n`` single-grid steps of ``el`` (SoA ``(8, nx, ny)``) with
     per-cell Lame fields ``(1, nx, ny)`` (or ``(nx, ny)``), an optional external force
     ``(2, nx, ny)`` and initial populations ``f0`` ``(8, nx, ny)``, all SoA.  Differentiable with respect
     to the fields, the force and ``f0``."""
-    return _ElasticitySolveFn.apply(lam_field, mu_field, force, f0, el, int(n), int(checkpoint_every), use_graphs)
+    return _ElasticitySolveFn.apply(lam_field, mu_field, force, f0, el, int(n), int(checkpoint_every), use_graphs,
+                                    reuse)
 
 
 class _DarcySolveFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, omega_field, source, f0, da, n, K, use_graphs):
-        st = DarcyStepper(da, omega_field.detach(), None if source is None else source.detach())
-        cs = CheckpointedSolve(st, n, K, use_graphs)
+    def forward(ctx, omega_field, source, f0, da, n, K, use_graphs, reuse):
+        key = ("darcy", n, K, use_graphs, source is not None)
+        sessions = da.__dict__.setdefault("_adjoint_sessions", {}) if reuse else {}
+        if key in sessions:
+            cs = sessions[key]
+            st = cs.s
+            st.bind(omega_field.detach(), None if source is None else source.detach())
+        else:
+            st = DarcyStepper(da, omega_field.detach(), None if source is None else source.detach())
+            cs = sessions[key] = CheckpointedSolve(st, n, K, use_graphs)
         out = cs.forward(_soa(f0.detach(), 9, da.nx, da.ny, da._tdt, da.device))
-        ctx.cs, ctx.st = cs, st
+        ctx.cs, ctx.st, ctx.generation = cs, st, cs.generation
         ctx.has_source = source is not None
         ctx.shapes = (omega_field.shape, None if source is None else source.shape, f0.shape)
         ctx.dtypes = (omega_field.dtype, None if source is None else source.dtype, f0.dtype)
@@ -280,24 +334,25 @@ class _DarcySolveFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         st, cs = ctx.st, ctx.cs
+        _check_generation(ctx)
         need_src = ctx.has_source and ctx.needs_input_grad[1]
-        grads = st.grad_buffers(need_src)
+        grads = cs.gradient_buffers(need_src)
         ab0 = cs.backward(grad_out.contiguous().to(st.dtype), grads)
         so, ss, s0 = ctx.shapes
         do, ds, d0 = ctx.dtypes
         g_om = grads["omega"].reshape(so).to(do) if ctx.needs_input_grad[0] else None
         g_s = grads["source"].reshape(ss).to(ds) if need_src else None
         g_0 = ab0.reshape(s0).to(d0) if ctx.needs_input_grad[2] else None
-        return g_om, g_s, g_0, None, None, None, None
+        return g_om, g_s, g_0, None, None, None, None, None
 
 
-def darcy_solve(da, omega_field, f0, n, source=None, checkpoint_every=50, use_graphs=True):
+def darcy_solve(da, omega_field, f0, n, source=None, checkpoint_every=50, use_graphs=True, reuse=True):
     """Populations after ``n`` steps of ``da`` (SoA ``(9, nx, ny)``) with a per-cell
     relaxation rate ``omega_field`` and an optional external source ``(1, nx, ny)``.
     Differentiable with respect to omega, the source and ``f0``.  For the
     permeability, omega = 1 / (3 K + 1/2): pass ``1 / (3 * K + 0.5)`` built from a
     ``K`` that requires grad."""
-    return _DarcySolveFn.apply(omega_field, source, f0, da, int(n), int(checkpoint_every), use_graphs)
+    return _DarcySolveFn.apply(omega_field, source, f0, da, int(n), int(checkpoint_every), use_graphs, reuse)
 
 
 def _fixed_point_adjoint(stepper, f_star, dJdf, tol, max_steps, check_every, want_extra):
