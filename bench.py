@@ -8,12 +8,15 @@ covariance of correlation length 16 cells, p = 1 on the left and p = 0 on the
 right (anti-bounce-back), no flux at the bottom and top (bounce-back).  One
 "step" is one Darcy LBM update of the whole grid.  ``value`` is MLUPS in the
 reference convention (cells x steps / s / 1e6), with the populations resident in
-HBM; the steps replay captured HIP graphs of 64 updates.
+HBM; the steps replay captured HIP graphs of 64 updates, each launch advancing two
+steps (``xb_darcy_step2``: populations read once, collided, streamed and
+wall-corrected twice in registers, written once).
 
-``roofline`` prices the dominant kernel (the fused Darcy sweep) at its
-algorithmic bytes, 19 values per cell (9 populations read, 9 written, the per-cell
-relaxation rate read) x 8 B, over its average launch duration measured with HIP
-events on the launch stream.  ``cpu_baseline`` times the CPU restatement of the
+``roofline`` prices the dominant kernel (the two-step Darcy sweep) at its
+algorithmic bytes per launch, 19 values per cell (9 populations read, 9 written,
+the per-cell relaxation rate read) x 8 B, over its average launch duration measured
+with HIP events on the launch stream; ``per_step_equivalent_gbs`` is what single
+steps would have to sustain for the same step rate.  ``cpu_baseline`` times the CPU restatement of the
 reference algorithm (torch CPU, float64, all host cores) on a bounded sample of
 the same workload.
 """
@@ -75,8 +78,12 @@ def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True):
     f, wall, gpu = timed_steps(torch, lambda f, k: d.run_steps(f, k, chunk), f, steps)
     S = 8 if precision == "f64/f64" else 4
     cells = d.nx * d.ny
-    # the timed graphs hold only the sweep kernel: GPU time / steps = average launch duration
-    return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps, bytes=cells * 19 * S)
+    # the timed graphs hold only sweep launches of two steps each (chunk % 4 == 0):
+    # GPU time / (steps / 2) = average launch duration; a launch reads and writes
+    # every population once (compulsory 19 values per cell)
+    spl = 2 if chunk % 4 == 0 else 1
+    return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 19 * S,
+                steps_per_launch=spl)
 
 
 def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=32):
@@ -89,7 +96,9 @@ def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=3
     f, wall, gpu = timed_steps(torch, lambda f, k: el.run_steps(f, k, chunk), f, steps)
     S = 8 if precision == "f64/f64" else 4
     cells = n * n
-    return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps, bytes=cells * 18 * S)
+    spl = 2 if chunk % 4 == 0 else 1
+    return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 18 * S,
+                steps_per_launch=spl)
 
 
 def residual_solve(torch, d, tol, max_steps, check_every=1000, chunk=64):
@@ -503,8 +512,11 @@ def main():
                    "nx": args.n + 2, "ny": args.n + 2, "precision": "f64/f64", "graph_chunk": 64},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_sweep<double, DarcyColl, STEP, bc>", "kernel_us": round(r["kernel_s"] * 1e6, 3),
-                     "bytes_per_cell": 152, "cells": r["cells"]},
+                     "kernel": "k_sweep2<double, DarcyColl, rules in LDS>" if r["steps_per_launch"] == 2
+                     else "k_sweep<double, DarcyColl, STEP, rules>",
+                     "kernel_us": round(r["kernel_s"] * 1e6, 3), "steps_per_launch": r["steps_per_launch"],
+                     "bytes_per_cell_per_launch": 152, "cells": r["cells"],
+                     "per_step_equivalent_gbs": round(achieved * r["steps_per_launch"], 1)},
     }
     extras = {}
 
@@ -539,12 +551,14 @@ def main():
             gbs = rl["bytes"] / rl["kernel_s"] / 1e9
             extras[f"elasticity_{n}_fp64"] = {"mlups": round(rl["cells"] * 200 / rl["elapsed"] / 1e6, 1),
                                               "kernel_us": round(rl["kernel_s"] * 1e6, 2), "gbs": round(gbs, 1),
+                                              "steps_per_launch": rl["steps_per_launch"],
                                               "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": 144}
             log(f"darcy {n}^2 fp64 (HBM-resident)")
             rd = bench_darcy(torch, n - 2, "f64/f64", 200, 20, chunk=32)
             gbs = rd["bytes"] / rd["kernel_s"] / 1e9
             extras[f"darcy_{n}_fp64"] = {"mlups": round(rd["cells"] * 200 / rd["elapsed"] / 1e6, 1),
                                          "kernel_us": round(rd["kernel_s"] * 1e6, 2), "gbs": round(gbs, 1),
+                                         "steps_per_launch": rd["steps_per_launch"],
                                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": 152}
     line["extra"] = extras
     if not args.no_cpu:

@@ -86,7 +86,7 @@ typedef struct xb_elast_args {
   void *coarse_out;         /* RESIDUAL: (8, (ix1-ix0)/2, (iy1-iy0)/2)              */
   double *norm_partials;    /* RESIDUAL: xb_elast_partials_count() doubles          */
   int32_t rows;             /* launch shape: rows per wavefront (power of 2 in
-                               [2, 64]) or 0 for the built-in choice                */
+                               [1, 64]; >= 2 for RESIDUAL) or 0 for the built-in    */
   int32_t pad_;
   /* Fused Biot coupling force (all optional; active when div_in is set):
    * force_b is replaced by force_b + G(div_in), G(q) = sum_d qs_d (-3 nl_scale w_d c_d)
@@ -132,6 +132,11 @@ const char *xb_last_error(void);
  * and the squared norm of the V-cycle (:1278-1287, _restrict :850-876, the norm
  * at :1307-1311). */
 int xb_elast_step(const xb_elast_args *args, void *stream);
+
+/* Two plain steps in one pass: f_out = S(S(f_in)) with S the XB_MODE_STEP step of these
+ * args (populations read and written once per two steps; bitwise the same as two
+ * xb_elast_step calls).  STEP mode only: no fpc_out, resid or div_in; grid >= 2 x 2. */
+int xb_elast_step2(const xb_elast_args *args, void *stream);
 
 /* Number of doubles in norm_partials for a RESIDUAL call with these args. */
 int64_t xb_elast_partials_count(const xb_elast_args *args);
@@ -219,6 +224,9 @@ int xb_darcy_adjoint(const xb_darcy_adjoint_args *args, void *stream);
 /* Fused BGK collide -> BCs -> stream -> BCs.  Replaces
  * AdvectionDiffusionReactionBGK.step (src/darcy.py:131-171, collision :106-129). */
 int xb_darcy_step(const xb_darcy_args *args, void *stream);
+
+/* Two Darcy steps in one pass (as xb_elast_step2; no fpc_out). */
+int xb_darcy_step2(const xb_darcy_args *args, void *stream);
 
 /* p = sum_d f_d + (s_a + s_b)/2 ; grad = sum_d f_d (-3 omega) c_d.
  * Replaces update_macroscopic (src/darcy.py:36-73). Either output may be null. */

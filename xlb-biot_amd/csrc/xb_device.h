@@ -39,7 +39,40 @@ template <typename T> struct BCDev {
   const int8_t *code_ps = nullptr;
   const T *const_pc = nullptr;
   const T *const_ps = nullptr;
+  int nslots = 0;
 };
+
+// Rule tables of at most LDS_BC_SLOTS slots are copied into LDS by each wavefront
+// of a sweep that meets a rule cell (into its own region, the first time one of
+// its rows has one): the rule reads then go through the LDS counter and never
+// force a wait on the populations still in flight (a vector-memory wait counts
+// loads in issue order, so waiting for a rule load would also wait for every row
+// prefetched after it), and wavefronts without rule cells do no extra work.  Sweep
+// kernels take the rule mode as a template argument:
+//   0 no table, 1 table read from global memory, 2 table staged in LDS.
+constexpr int LDS_BC_SLOTS = 32;
+
+template <typename T, int Q> struct BCShared {
+  int8_t code_pc[LDS_BC_SLOTS * Q];
+  int8_t code_ps[LDS_BC_SLOTS * Q];
+  T const_pc[LDS_BC_SLOTS * Q];
+  T const_ps[LDS_BC_SLOTS * Q];
+};
+
+// Copy of a small rule table into one wavefront's LDS region, by all lanes of that
+// wavefront (no barrier: LDS operations of one wavefront complete in order).
+template <typename T, int Q>
+__device__ __forceinline__ void stage_rules_wave(const BCDev<T> &bc, BCShared<T, Q> &sh) {
+  const int n = bc.nslots * Q;
+  for (int k = (int)(threadIdx.x & (WAVE - 1)); k < n; k += WAVE) {
+    sh.code_pc[k] = bc.code_pc != nullptr ? bc.code_pc[k] : int8_t(0);
+    sh.const_pc[k] = bc.const_pc != nullptr ? bc.const_pc[k] : T(0);
+    sh.code_ps[k] = bc.code_ps != nullptr ? bc.code_ps[k] : int8_t(0);
+    sh.const_ps[k] = bc.const_ps != nullptr ? bc.const_ps[k] : T(0);
+  }
+}
+
+inline int bc_mode(const int32_t *slot, int nslots) { return slot == nullptr ? 0 : (nslots <= LDS_BC_SLOTS ? 2 : 1); }
 
 // v[d] <- rule(v[d], src) + const[d]  (rule codes of xlb_biot.h).  Selection by
 // compare chains keeps src in registers (no dynamic register indexing).

@@ -51,20 +51,20 @@ struct SweepGeom {
   int ix0, ix1, iy0, iy1; // box carrying the subtracted residual
 };
 
-// Rows per wavefront: enough items to give every SIMD several waves, never fewer
-// than 2 (residual restriction pairs rows) and at most 64.
+// Rows per wavefront: enough items to give every SIMD several waves, at most 64
+// and at least min_rows (2 for the residual sweep, whose restriction pairs rows).
 SweepGeom make_geom(int nx, int ny, int ox, int oy, int ex, int ey, int ix0, int ix1, int iy0, int iy1,
-                    int rows_hint) {
+                    int rows_hint, int strip = STRIP, int min_rows = 2) {
   SweepGeom g{};
   g.nx = nx; g.ny = ny; g.ox = ox; g.oy = oy; g.ex = ex; g.ey = ey;
   g.ix0 = ix0; g.ix1 = ix1; g.iy0 = iy0; g.iy1 = iy1;
-  g.nstrips = cdiv(ey, STRIP);
+  g.nstrips = cdiv(ey, strip);
   int rows = 64;
-  if (rows_hint >= 2 && rows_hint <= 64 && (rows_hint & (rows_hint - 1)) == 0) {
+  if (rows_hint >= min_rows && rows_hint <= 64 && (rows_hint & (rows_hint - 1)) == 0) {
     rows = rows_hint;
   } else {
     const int target = 8192;
-    while (rows > 2 && (int64_t)g.nstrips * cdiv(ex, rows) < target) rows >>= 1;
+    while (rows > min_rows && (int64_t)g.nstrips * cdiv(ex, rows) < target) rows >>= 1;
   }
   g.rows = rows;
   g.nitems = g.nstrips * cdiv(ex, rows);
@@ -75,12 +75,32 @@ SweepGeom make_geom(int nx, int ny, int ox, int oy, int ex, int ey, int ix0, int
 // One wavefront item of the sweep: a strip of 62 cells along y walked down `g.rows`
 // rows.  Shared by the single-grid kernel and the batched kernel, so both do the
 // same arithmetic in the same order.
-template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES, bool HAS_NL>
+template <typename T, class Coll, int MODE, int BC, bool HAS_RES, bool HAS_NL>
 __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll, const T *__restrict__ fin,
                                            T *__restrict__ fout, T *__restrict__ fpc, const T *__restrict__ res,
                                            const BCDev<T> &bc, const T omg, const T gamma, T *__restrict__ coarse,
-                                           double *__restrict__ partials, const NLDev<T> &nl, const int item) {
+                                           double *__restrict__ partials, const NLDev<T> &nl,
+                                           BCShared<T, Coll::Q> *sh, const int item) {
   constexpr int Q = Coll::Q;
+  constexpr bool HAS_BC = BC != 0;
+  bool staged = false; // BC == 2: the table is in this wavefront's LDS region
+  auto stage_if_needed = [&](int slot) {
+    if constexpr (BC == 2) {
+      if (!staged && __ballot(slot >= 0) != 0) { // uniform per wavefront
+        stage_rules_wave<T, Q>(bc, *sh);
+        staged = true;
+      }
+    }
+  };
+  // rule rows of a slot, from LDS (BC == 2) or global memory (BC == 1)
+  auto rules_pc = [&](T(&v)[Q], const T(&src)[Q], int slot) {
+    if constexpr (BC == 2) apply_rules<T, Q>(v, src, sh->code_pc + slot * Q, sh->const_pc + slot * Q);
+    else apply_rules<T, Q>(v, src, bc.code_pc + (size_t)slot * Q, bc.const_pc + (size_t)slot * Q);
+  };
+  auto rules_ps = [&](T(&v)[Q], const T(&src)[Q], int slot) {
+    if constexpr (BC == 2) apply_rules<T, Q>(v, src, sh->code_ps + slot * Q, sh->const_ps + slot * Q);
+    else apply_rules<T, Q>(v, src, bc.code_ps + (size_t)slot * Q, bc.const_ps + (size_t)slot * Q);
+  };
   const int lane = threadIdx.x & (WAVE - 1);
 
   const int strip = item % g.nstrips;
@@ -166,8 +186,8 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
       coll.collide(pre, rw.aux, pc);
     }
     if constexpr (HAS_BC) {
-      if (rw.slot >= 0 && bc.code_pc != nullptr)
-        apply_rules<T, Q>(pc, pre, bc.code_pc + (size_t)rw.slot * Q, bc.const_pc + (size_t)rw.slot * Q);
+      stage_if_needed(rw.slot);
+      if (rw.slot >= 0 && bc.code_pc != nullptr) rules_pc(pc, pre, rw.slot);
     }
   };
 
@@ -192,8 +212,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
       v[d] = s;
     }
     if constexpr (HAS_BC) {
-      if (S1 >= 0 && bc.code_ps != nullptr)
-        apply_rules<T, Q>(v, R1, bc.code_ps + (size_t)S1 * Q, bc.const_ps + (size_t)S1 * Q);
+      if (S1 >= 0 && bc.code_ps != nullptr) rules_ps(v, R1, S1);
     }
     const size_t cell = (size_t)x * g.ny + y;
     if constexpr (MODE == XB_MODE_STEP) {
@@ -278,15 +297,169 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
   }
 }
 
-template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES, bool HAS_NL>
+template <typename T, class Coll, int MODE, int BC, bool HAS_RES, bool HAS_NL>
 __global__ void __launch_bounds__(BLOCK)
 k_sweep(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
         T *__restrict__ fpc, const T *__restrict__ res, const BCDev<T> bc, const T omg, const T gamma,
         T *__restrict__ coarse, double *__restrict__ partials, const NLDev<T> nl) {
   const int item = xcd_block_id() * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
-  if (item >= g.nitems) return; // uniform per wavefront
-  sweep_item<T, Coll, MODE, HAS_BC, HAS_RES, HAS_NL>(g, coll, fin, fout, fpc, res, bc, omg, gamma, coarse, partials,
-                                                     nl, item);
+  if constexpr (BC == 2) {
+    __shared__ BCShared<T, Coll::Q> sh[WAVES_PER_BLOCK];
+    if (item >= g.nitems) return; // uniform per wavefront
+    sweep_item<T, Coll, MODE, BC, HAS_RES, HAS_NL>(g, coll, fin, fout, fpc, res, bc, omg, gamma, coarse, partials,
+                                                   nl, &sh[threadIdx.x >> 6], item);
+  } else {
+    if (item >= g.nitems) return;
+    sweep_item<T, Coll, MODE, BC, HAS_RES, HAS_NL>(g, coll, fin, fout, fpc, res, bc, omg, gamma, coarse, partials,
+                                                   nl, nullptr, item);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two plain steps per launch (temporal blocking): f_out = S(S(f_in)).  A wavefront
+// owns 60 output cells along y (two halo lanes on each side) and walks input rows
+// xs-2 .. xe+1.  Stage 1 collides each input row into a ring of three rows and
+// streams the middle one (post-streaming rules included): the step-1 populations of
+// that row, still in registers.  Stage 2 collides those into a second ring and
+// streams again into the output.  Each population is read and written once per two
+// steps; per cell the arithmetic is exactly that of two k_sweep STEP launches.
+constexpr int STRIP2 = WAVE - 4;
+
+template <typename T, class Coll, int BC>
+__device__ __forceinline__ void sweep2_item(const SweepGeom &g, const Coll &coll, const T *__restrict__ fin,
+                                            T *__restrict__ fout, const BCDev<T> &bc, BCShared<T, Coll::Q> *sh,
+                                            const int item) {
+  constexpr int Q = Coll::Q;
+  constexpr bool HAS_BC = BC != 0;
+  bool staged = false;
+  using Aux = typename Coll::Aux;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int strip = item % g.nstrips;
+  const int rb = item / g.nstrips;
+  const int ys = g.oy + strip * STRIP2;
+  const int yend = min(ys + STRIP2, g.oy + g.ey);
+  const int yv = ys - 2 + lane;
+  const bool out_lane = lane >= 2 && yv < yend;
+  const int y = wrapi(min(yv, yend + 1), g.ny);
+  const int xs = g.ox + rb * g.rows;
+  const int xe = min(xs + g.rows, g.ox + g.ex);
+  const size_t N = (size_t)g.nx * g.ny;
+
+  auto load = [&](int x, T(&f)[Q], Aux &aux, int &slot) {
+    const size_t cell = (size_t)x * g.ny + y;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) f[d] = fin[d * N + cell];
+    aux = coll.load_aux(cell);
+    slot = -1;
+    if constexpr (HAS_BC) slot = bc.slot[cell];
+  };
+  auto collide = [&](const T(&pre)[Q], const Aux &aux, int slot, T(&pc)[Q]) {
+    coll.collide(pre, aux, pc);
+    if constexpr (BC == 2) {
+      if (!staged && __ballot(slot >= 0) != 0) { // uniform per wavefront
+        stage_rules_wave<T, Q>(bc, *sh);
+        staged = true;
+      }
+    }
+    if constexpr (HAS_BC) {
+      if (slot >= 0 && bc.code_pc != nullptr) {
+        if constexpr (BC == 2) apply_rules<T, Q>(pc, pre, sh->code_pc + slot * Q, sh->const_pc + slot * Q);
+        else apply_rules<T, Q>(pc, pre, bc.code_pc + (size_t)slot * Q, bc.const_pc + (size_t)slot * Q);
+      }
+    }
+  };
+  // streamed populations of the middle row of (r0, r1, r2) = rows (x-1, x, x+1)
+  auto stream = [&](const T(&r0)[Q], const T(&r1)[Q], const T(&r2)[Q], int slot, T(&v)[Q]) {
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      T s = Coll::cx(d) == 1 ? r0[d] : (Coll::cx(d) == 0 ? r1[d] : r2[d]);
+      if (Coll::cy(d) == 1) s = from_prev_lane(s);
+      else if (Coll::cy(d) == -1) s = from_next_lane(s);
+      v[d] = s;
+    }
+    if constexpr (HAS_BC) {
+      if (slot >= 0 && bc.code_ps != nullptr) {
+        if constexpr (BC == 2) apply_rules<T, Q>(v, r1, sh->code_ps + slot * Q, sh->const_ps + slot * Q);
+        else apply_rules<T, Q>(v, r1, bc.code_ps + (size_t)slot * Q, bc.const_ps + (size_t)slot * Q);
+      }
+    }
+  };
+
+  T R0[Q], R1[Q], R2[Q], Q0[Q], Q1[Q], Q2[Q], cur[Q], nxt[Q];
+#pragma unroll
+  for (int d = 0; d < Q; ++d) R1[d] = R2[d] = Q1[d] = Q2[d] = T(0);
+  Aux acur, anxt, aP;
+  int scur, snxt = -1, sP = -1, sPP = -1;
+  load(wrapi(xs - 2, g.nx), cur, acur, scur);
+  for (int i = xs - 2; i <= xe + 1; ++i) {
+    if (i + 1 <= xe + 1) load(wrapi(i + 1, g.nx), nxt, anxt, snxt);
+    T C[Q];
+    collide(cur, acur, scur, C);
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      R0[d] = R1[d];
+      R1[d] = R2[d];
+      R2[d] = C[d];
+    }
+    if (i >= xs) { // (R0, R1, R2) = collided rows (i-2, i-1, i): step 1 of row i-1
+      T v1[Q], D[Q];
+      stream(R0, R1, R2, sP, v1);
+      collide(v1, aP, sP, D);
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        Q0[d] = Q1[d];
+        Q1[d] = Q2[d];
+        Q2[d] = D[d];
+      }
+      if (i >= xs + 2) { // (Q0, Q1, Q2) = step-1 rows (i-3, i-2, i-1) collided: output row i-2
+        T o[Q];
+        stream(Q0, Q1, Q2, sPP, o);
+        if (out_lane) {
+          const size_t cell = (size_t)(i - 2) * g.ny + y;
+#pragma unroll
+          for (int d = 0; d < Q; ++d) fout[d * N + cell] = o[d];
+        }
+      }
+    }
+    sPP = sP;
+    sP = scur;
+    aP = acur;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) cur[d] = nxt[d];
+    acur = anxt;
+    scur = snxt;
+  }
+}
+
+template <typename T, class Coll, int BC>
+__global__ void __launch_bounds__(BLOCK)
+k_sweep2(const SweepGeom g, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout, const BCDev<T> bc) {
+  const int item = xcd_block_id() * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
+  if constexpr (BC == 2) {
+    __shared__ BCShared<T, Coll::Q> sh[WAVES_PER_BLOCK];
+    if (item >= g.nitems) return;
+    sweep2_item<T, Coll, BC>(g, coll, fin, fout, bc, &sh[threadIdx.x >> 6], item);
+  } else {
+    if (item >= g.nitems) return;
+    sweep2_item<T, Coll, BC>(g, coll, fin, fout, bc, nullptr, item);
+  }
+}
+
+template <typename T> BCDev<T> bcdev(const xb_bc_table *t);
+
+template <typename T, class Coll>
+int launch_sweep2(const SweepGeom &g, const Coll &coll, const void *fin, void *fout, const xb_bc_table *bct,
+                  hipStream_t st) {
+  const BCDev<T> bc = bcdev<T>(bct);
+  const dim3 grid(cdiv(g.nitems, WAVES_PER_BLOCK)), block(BLOCK);
+  const T *fi = static_cast<const T *>(fin);
+  T *fo = static_cast<T *>(fout);
+  switch (bc_mode(bc.slot, bc.nslots)) {
+  case 2: hipLaunchKernelGGL((k_sweep2<T, Coll, 2>), grid, block, 0, st, g, coll, fi, fo, bc); break;
+  case 1: hipLaunchKernelGGL((k_sweep2<T, Coll, 1>), grid, block, 0, st, g, coll, fi, fo, bc); break;
+  default: hipLaunchKernelGGL((k_sweep2<T, Coll, 0>), grid, block, 0, st, g, coll, fi, fo, bc); break;
+  }
+  return check_launch("two-step sweep kernel");
 }
 
 // ---------------------------------------------------------------------------
@@ -311,20 +484,27 @@ template <typename T, class Coll> struct SweepJob {
   int32_t block0; // first block of this job in the launch
 };
 
-template <typename T, class Coll, int MODE, bool HAS_BC, bool HAS_RES, bool HAS_NL>
+template <typename T, class Coll, int MODE, int BC, bool HAS_RES, bool HAS_NL>
 __global__ void __launch_bounds__(BLOCK)
 k_sweep_batch(const SweepJob<T, Coll> *__restrict__ jobs, const int32_t *__restrict__ block_job,
               const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
   const int b = xcd_block_id();
   const int j = block_job[b];
   const int i = block_inst[b];
-  if (j < 0) return;
-  if (active != nullptr && active[i] == 0) return;
+  if (j < 0) return;                               // uniform per block
+  if (active != nullptr && active[i] == 0) return; // uniform per block
   const SweepJob<T, Coll> J = jobs[j];
   const int item = (b - J.block0) * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
-  if (item >= J.g.nitems) return; // uniform per wavefront
-  sweep_item<T, Coll, MODE, HAS_BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
-                                                     J.coarse, J.partials, J.nl, item);
+  if constexpr (BC == 2) {
+    __shared__ BCShared<T, Coll::Q> sh[WAVES_PER_BLOCK];
+    if (item >= J.g.nitems) return;
+    sweep_item<T, Coll, MODE, BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
+                                                   J.coarse, J.partials, J.nl, &sh[threadIdx.x >> 6], item);
+  } else {
+    if (item >= J.g.nitems) return; // uniform per wavefront
+    sweep_item<T, Coll, MODE, BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
+                                                   J.coarse, J.partials, J.nl, nullptr, item);
+  }
 }
 
 template <typename T, class Coll, int MODE>
@@ -339,7 +519,9 @@ int launch_sweep(const SweepGeom &g, const Coll &coll, const void *fin, void *fo
     bc.code_ps = bct->code_ps;
     bc.const_pc = static_cast<const T *>(bct->const_pc);
     bc.const_ps = static_cast<const T *>(bct->const_ps);
+    bc.nslots = bct->nslots;
   }
+  const int bcm = bc_mode(bc.slot, bc.nslots);
   const bool has_res = res != nullptr;
   const dim3 grid(cdiv(g.nitems, WAVES_PER_BLOCK)), block(BLOCK);
   const T omg = T(1.0 - gamma), gm = T(gamma);
@@ -352,19 +534,22 @@ int launch_sweep(const SweepGeom &g, const Coll &coll, const void *fin, void *fo
 #define XB_SWEEP(BC, RES, NL) \
   hipLaunchKernelGGL((k_sweep<T, Coll, MODE, BC, RES, NL>), grid, block, 0, st, g, coll, fi, fo, fp, rs, bc, omg, gm, \
                      co, partials, nl)
+#define XB_SWEEP_BC(RES, NL)                                                                                        \
+  do {                                                                                                             \
+    if (bcm == 2) XB_SWEEP(2, RES, NL);                                                                            \
+    else if (bcm == 1) XB_SWEEP(1, RES, NL);                                                                       \
+    else XB_SWEEP(0, RES, NL);                                                                                     \
+  } while (0)
   if constexpr (Coll::Q == 8) {
     if (nlp != nullptr) {
-      if (has_bc && has_res) XB_SWEEP(true, true, true);
-      else if (has_bc) XB_SWEEP(true, false, true);
-      else if (has_res) XB_SWEEP(false, true, true);
-      else XB_SWEEP(false, false, true);
+      if (has_res) XB_SWEEP_BC(true, true);
+      else XB_SWEEP_BC(false, true);
       return check_launch("sweep kernel");
     }
   }
-  if (has_bc && has_res) XB_SWEEP(true, true, false);
-  else if (has_bc) XB_SWEEP(true, false, false);
-  else if (has_res) XB_SWEEP(false, true, false);
-  else XB_SWEEP(false, false, false);
+  if (has_res) XB_SWEEP_BC(true, false);
+  else XB_SWEEP_BC(false, false);
+#undef XB_SWEEP_BC
 #undef XB_SWEEP
   return check_launch("sweep kernel");
 }
@@ -619,6 +804,7 @@ template <typename T> BCDev<T> bcdev(const xb_bc_table *t) {
     b.code_ps = t->code_ps;
     b.const_pc = static_cast<const T *>(t->const_pc);
     b.const_ps = static_cast<const T *>(t->const_ps);
+    b.nslots = t->nslots;
   }
   return b;
 }
@@ -643,7 +829,7 @@ int elast_geom(const xb_elast_args *a, SweepGeom &g) {
     g = make_geom(a->nx, a->ny, a->ix0, a->iy0, a->ix1 - a->ix0, a->iy1 - a->iy0, a->ix0, a->ix1, a->iy0, a->iy1,
                   a->rows);
   else
-    g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, a->ix0, a->ix1, a->iy0, a->iy1, a->rows);
+    g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, a->ix0, a->ix1, a->iy0, a->iy1, a->rows, STRIP, 1);
   return XB_OK;
 }
 
@@ -710,8 +896,10 @@ int validate_elast(const xb_elast_args *a, const char *who) {
     if (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1))
       return fail(XB_EINVAL, "%s: interior box must have even extents for restriction", who);
   }
-  if (a->rows != 0 && (a->rows < 2 || a->rows > 64 || (a->rows & (a->rows - 1))))
-    return fail(XB_EINVAL, "%s: rows must be 0 or a power of two in [2, 64]", who);
+  if (a->rows != 0 && (a->rows < 1 || a->rows > 64 || (a->rows & (a->rows - 1))))
+    return fail(XB_EINVAL, "%s: rows must be 0 or a power of two in [1, 64]", who);
+  if (a->mode == XB_MODE_RESIDUAL && a->rows == 1)
+    return fail(XB_EINVAL, "%s: the residual sweep restricts row pairs (rows >= 2)", who);
   if (a->mode == XB_MODE_SMOOTH && !(a->gamma >= 0.0 && a->gamma <= 1.0))
     return fail(XB_EINVAL, "%s: gamma must be in [0,1]", who);
   if (int r = check_bc(a->bc, false)) return r;
@@ -879,10 +1067,11 @@ template <typename T> int build_sweep_jobs(const xb_elast_args *args, const int3
       return fail(XB_EINVAL, "xb_batch_prepare: job %d: dtype and mode must agree across the batch", k);
     const int f = (a->bc != nullptr && a->bc->cell_slot != nullptr ? 1 : 0) | (a->resid != nullptr ? 2 : 0) |
                   (a->div_in != nullptr ? 4 : 0);
-    if (k == 0) flags = f;
-    else if (f != flags)
+    if (k == 0) flags = f | 8;
+    else if (f != (flags & 7))
       return fail(XB_EINVAL, "xb_batch_prepare: job %d: boundary table / residual / coupling presence must agree "
                   "across the batch", k);
+    if ((f & 1) && a->bc->nslots > LDS_BC_SLOTS) flags &= ~8;
     SweepJob<T, ElastColl<T>> J{};
     elast_geom(a, J.g);
     elast_coll<T>(a, J.coll, J.nl);
@@ -1040,16 +1229,20 @@ int launch_elast_batch(const xb_batch_plan *p, const int32_t *active, hipStream_
   const dim3 grid(p->nblocks), block(BLOCK);
 #define XB_SB(BC, RES, NL) \
   hipLaunchKernelGGL((k_sweep_batch<T, ElastColl<T>, MODE, BC, RES, NL>), grid, block, 0, st, jobs, bj, bi, active)
-  switch (p->flags) {
-  case 0: XB_SB(false, false, false); break;
-  case 1: XB_SB(true, false, false); break;
-  case 2: XB_SB(false, true, false); break;
-  case 3: XB_SB(true, true, false); break;
-  case 4: XB_SB(false, false, true); break;
-  case 5: XB_SB(true, false, true); break;
-  case 6: XB_SB(false, true, true); break;
-  default: XB_SB(true, true, true); break;
+  const int bcm = (p->flags & 1) ? ((p->flags & 8) ? 2 : 1) : 0;
+#define XB_SB_BC(RES, NL)                                                                                           \
+  do {                                                                                                             \
+    if (bcm == 2) XB_SB(2, RES, NL);                                                                               \
+    else if (bcm == 1) XB_SB(1, RES, NL);                                                                          \
+    else XB_SB(0, RES, NL);                                                                                        \
+  } while (0)
+  switch (p->flags & 6) {
+  case 0: XB_SB_BC(false, false); break;
+  case 2: XB_SB_BC(true, false); break;
+  case 4: XB_SB_BC(false, true); break;
+  default: XB_SB_BC(true, true); break;
   }
+#undef XB_SB_BC
 #undef XB_SB
   return check_launch("batched sweep kernel");
 }
@@ -1299,6 +1492,27 @@ int xb_elast_step(const xb_elast_args *a, void *stream) {
   return a->dtype == XB_F64 ? elast_step_t<double>(a, st) : elast_step_t<float>(a, st);
 }
 
+int xb_elast_step2(const xb_elast_args *a, void *stream) {
+  if (int r = validate_elast(a, "xb_elast_step2")) return r;
+  if (a->mode != XB_MODE_STEP || a->fpc_out != nullptr || a->resid != nullptr || a->div_in != nullptr)
+    return fail(XB_EUNSUPPORTED, "xb_elast_step2: plain steps only (no post-collision output, residual or "
+                "coupling force)");
+  if (a->nx < 2 || a->ny < 2) return fail(XB_EINVAL, "xb_elast_step2: grid must be at least 2 x 2");
+  if (a->f_in == a->f_out) return fail(XB_EINVAL, "xb_elast_step2: f_in and f_out must differ");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny, a->rows, STRIP2, 1);
+  if (a->dtype == XB_F64) {
+    ElastColl<double> c;
+    NLDev<double> nl;
+    elast_coll<double>(a, c, nl);
+    return launch_sweep2<double>(g, c, a->f_in, a->f_out, a->bc, st);
+  }
+  ElastColl<float> c;
+  NLDev<float> nl;
+  elast_coll<float>(a, c, nl);
+  return launch_sweep2<float>(g, c, a->f_in, a->f_out, a->bc, st);
+}
+
 int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const void *coarse, int32_t cnx, int32_t cny,
                          int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1, void *stream) {
   if (int r = check_dtype(dtype)) return r;
@@ -1412,10 +1626,10 @@ int xb_darcy_step(const xb_darcy_args *a, void *stream) {
   if (a->nx <= 0 || a->ny <= 0 || a->f_in == nullptr || a->f_out == nullptr)
     return fail(XB_EINVAL, "xb_darcy_step: bad arguments");
   if (int r = check_bc(a->bc, false)) return r;
-  if (a->rows != 0 && (a->rows < 2 || a->rows > 64 || (a->rows & (a->rows - 1))))
-    return fail(XB_EINVAL, "xb_darcy_step: rows must be 0 or a power of two in [2, 64]");
+  if (a->rows != 0 && (a->rows < 1 || a->rows > 64 || (a->rows & (a->rows - 1))))
+    return fail(XB_EINVAL, "xb_darcy_step: rows must be 0 or a power of two in [1, 64]");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny, a->rows);
+  const SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny, a->rows, STRIP, 1);
   if (a->dtype == XB_F64) {
     DarcyColl<double> c;
     c.omega = a->omega;
@@ -1434,6 +1648,36 @@ int xb_darcy_step(const xb_darcy_args *a, void *stream) {
   c.sb = static_cast<const float *>(a->source_b);
   return launch_sweep<float, DarcyColl<float>, XB_MODE_STEP>(g, c, a->f_in, a->f_out, a->fpc_out, nullptr, a->bc, 1.0,
                                                              nullptr, nullptr, st);
+}
+
+int xb_darcy_step2(const xb_darcy_args *a, void *stream) {
+  if (a == nullptr) return fail(XB_EINVAL, "xb_darcy_step2: null args");
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->nx < 2 || a->ny < 2 || a->f_in == nullptr || a->f_out == nullptr)
+    return fail(XB_EINVAL, "xb_darcy_step2: bad arguments (grid at least 2 x 2)");
+  if (a->f_in == a->f_out) return fail(XB_EINVAL, "xb_darcy_step2: f_in and f_out must differ");
+  if (a->fpc_out != nullptr) return fail(XB_EUNSUPPORTED, "xb_darcy_step2: no post-collision output");
+  if (int r = check_bc(a->bc, false)) return r;
+  if (a->rows != 0 && (a->rows < 1 || a->rows > 64 || (a->rows & (a->rows - 1))))
+    return fail(XB_EINVAL, "xb_darcy_step2: rows must be 0 or a power of two in [1, 64]");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny, a->rows, STRIP2, 1);
+  if (a->dtype == XB_F64) {
+    DarcyColl<double> c;
+    c.omega = a->omega;
+    c.sfac = 1 - 0.5 * a->omega;
+    c.om_field = static_cast<const double *>(a->omega_field);
+    c.sa = static_cast<const double *>(a->source_a);
+    c.sb = static_cast<const double *>(a->source_b);
+    return launch_sweep2<double>(g, c, a->f_in, a->f_out, a->bc, st);
+  }
+  DarcyColl<float> c;
+  c.omega = (float)a->omega;
+  c.sfac = (float)(1 - 0.5 * a->omega);
+  c.om_field = static_cast<const float *>(a->omega_field);
+  c.sa = static_cast<const float *>(a->source_a);
+  c.sb = static_cast<const float *>(a->source_b);
+  return launch_sweep2<float>(g, c, a->f_in, a->f_out, a->bc, st);
 }
 
 int xb_darcy_macro(int32_t nx, int32_t ny, int32_t dtype, double omega, const void *omega_field, const void *f,

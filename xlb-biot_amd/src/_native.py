@@ -92,6 +92,8 @@ SIGNATURES = {
     "xb_abi_version": (c_int32, []),
     "xb_last_error": (ctypes.c_char_p, []),
     "xb_elast_step": (c_int32, [ctypes.POINTER(ElastArgs), c_void_p]),
+    "xb_elast_step2": (c_int32, [ctypes.POINTER(ElastArgs), c_void_p]),
+    "xb_darcy_step2": (c_int32, [ctypes.POINTER(DarcyArgs), c_void_p]),
     "xb_elast_partials_count": (c_int64, [ctypes.POINTER(ElastArgs)]),
     "xb_elast_prolong_add": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_int32,
                                        c_int32, c_int32, c_int32, c_int32, c_void_p]),

@@ -1,7 +1,8 @@
 """Launch-shape autotuner of the sweep kernels.
 
 The sweep kernel's launch shape is ``rows``: how many grid rows one wavefront
-walks (a power of two in [2, 64]).  Few rows give more wavefronts (parallelism,
+walks (a power of two in [1, 64]; at least 2 for the residual sweep, whose
+restriction pairs rows).  Few rows give more wavefronts (parallelism,
 shorter tails) but re-read more halo rows from L2; many rows amortise the halo
 and the pipeline prologue.  The best value depends on grid size, lattice and
 dtype, so it is measured: each candidate runs a few graph-captured launches timed
@@ -22,7 +23,7 @@ import time
 
 import torch
 
-CANDIDATES = (2, 4, 8, 16, 32, 64)
+CANDIDATES = (1, 2, 4, 8, 16, 32, 64)
 _cache = None
 
 

@@ -116,17 +116,28 @@ class GraphedSteps:
 class StepRunner:
     """Persistent ping-pong buffers and one captured graph of ``chunk`` steps of
     ``launch(src, dst)``; :meth:`run` advances a state by any number of steps
-    (graph replays, then single launches) and returns a fresh copy of the result."""
+    (graph replays, then single launches) and returns a fresh copy of the result.
+    With ``launch2(src, dst)`` (two steps in one pass) the graph holds ``chunk / 2``
+    two-step launches and remainders use pairs, plus one single step if odd."""
 
-    def __init__(self, launch, like, chunk):
-        self.launch = launch
+    def __init__(self, launch, like, chunk, launch2=None):
+        self.launch, self.launch2 = launch, launch2
         self.a = torch.empty_like(like)
         self.b = torch.empty_like(like)
         self.a.copy_(like)
         launch(self.a, self.b)  # warm-up outside the capture
+        if launch2 is not None:
+            launch2(self.a, self.b)
+            if chunk % 4:
+                raise ValueError("two-step graphs hold a multiple of 4 steps")
         torch.cuda.synchronize()
         self.chunk = chunk
-        self.graph = GraphedSteps(launch, self.a, self.b, chunk) if chunk >= 2 else None
+        if chunk < 2:
+            self.graph = None
+        elif launch2 is not None:
+            self.graph = GraphedSteps(launch2, self.a, self.b, chunk // 2)
+        else:
+            self.graph = GraphedSteps(launch, self.a, self.b, chunk)
 
     def run(self, f, n):
         self.a.copy_(f)
@@ -135,6 +146,11 @@ class StepRunner:
             while n - done >= self.chunk:
                 self.graph.replay()
                 done += self.chunk
+        if self.launch2 is not None:
+            while n - done >= 2:
+                self.launch2(self.a, self.b)
+                self.a, self.b = self.b, self.a
+                done += 2
         while done < n:
             self.launch(self.a, self.b)
             self.a, self.b = self.b, self.a

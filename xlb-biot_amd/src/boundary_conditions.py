@@ -207,18 +207,35 @@ class RuleTable:
 
     def __init__(self, nx, ny, q, slots, code_pc, const_pc, code_ps, const_ps, dtype, device):
         self.nx, self.ny, self.q = nx, ny, q
-        self.nslots = int(len(slots))
+        self.ncells = int(len(slots))
+        # cells whose rule rows (codes and constants, compared bit for bit) coincide
+        # share one slot: walls with uniform data need only a handful of slots, which
+        # the kernels then keep in LDS
+        cols = []
+        for code, const in ((code_pc, const_pc), (code_ps, const_ps)):
+            if code is not None:
+                cols.append(np.asarray(code, dtype=np.int64))
+                cols.append(np.ascontiguousarray(np.asarray(const, dtype=np.float64)).view(np.int64))
+        if self.ncells and cols:
+            _, first, inverse = np.unique(np.concatenate(cols, axis=1), axis=0, return_index=True,
+                                          return_inverse=True)
+            inverse = inverse.reshape(-1)
+        else:
+            first, inverse = np.arange(self.ncells), np.arange(self.ncells)
+        pick = lambda a: None if a is None else np.asarray(a)[first]
+        code_pc, const_pc, code_ps, const_ps = pick(code_pc), pick(const_pc), pick(code_ps), pick(const_ps)
+        self.nslots = int(len(first))
         cell_slot = np.full(nx * ny, -1, dtype=np.int32)
-        if self.nslots:
-            cell_slot[slots] = np.arange(self.nslots, dtype=np.int32)
+        if self.ncells:
+            cell_slot[slots] = inverse.astype(np.int32)
         self.cell_slot = torch.as_tensor(cell_slot, device=device)
         self.code_pc = None if code_pc is None else torch.as_tensor(code_pc.astype(np.int8).ravel(), device=device)
         self.code_ps = None if code_ps is None else torch.as_tensor(code_ps.astype(np.int8).ravel(), device=device)
         self.const_pc = None if const_pc is None else torch.as_tensor(const_pc.ravel(), dtype=dtype, device=device)
         self.const_ps = None if const_ps is None else torch.as_tensor(const_ps.ravel(), dtype=dtype, device=device)
-        # host copies for inspection and tests
-        self.host = dict(slots=np.asarray(slots), code_pc=code_pc, const_pc=const_pc, code_ps=code_ps,
-                         const_ps=const_ps)
+        # host copies for inspection and tests (rows per slot; cell_slot maps cells to slots)
+        self.host = dict(cells=np.asarray(slots), cell_slot=cell_slot, code_pc=code_pc, const_pc=const_pc,
+                         code_ps=code_ps, const_ps=const_ps)
         self.struct = nat.BCTable(self.nslots, nat.ptr(self.cell_slot) if self.nslots else None,
                                   nat.ptr(self.code_pc), nat.ptr(self.code_ps),
                                   nat.ptr(self.const_pc), nat.ptr(self.const_ps))
@@ -227,7 +244,7 @@ class RuleTable:
 
     def ref(self):
         """``xb_bc_table*`` for the C ABI (None when no cell carries a rule)."""
-        return self._ptr if self.nslots else None
+        return self._ptr if self.ncells else None
 
 
 def compile_rules(bcs, nx, ny, q, stages=("PostCollision", "PostStreaming"), bounds=None):

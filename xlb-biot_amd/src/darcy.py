@@ -124,15 +124,21 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         return view(self._gradient_soa(self._soa(q, 1, nx, ny), scale))
 
     # ---- darcy.py:106-171
-    def _tune_key(self):
+    def _tune_key(self, two=False):
         return (f"darcy/{self.precision}/{self.nx}x{self.ny}/bc{int(self._bc_table is not None)}"
-                f"/omf{int(self._omega_field is not None)}")
+                f"/omf{int(self._omega_field is not None)}" + ("/x2" if two else ""))
 
-    def _args(self, f, out, ext_source, fpc):
+    def _args(self, f, out, ext_source, fpc, two=False):
         table = self._bc_table
         return nat.DarcyArgs(self.nx, self.ny, nat.dtype_code(f.dtype), 0, self.omega, nat.ptr(self._omega_field),
                              nat.ptr(f), nat.ptr(out), nat.ptr(fpc), nat.ptr(self._source_soa), nat.ptr(ext_source),
-                             table.ref() if table is not None else None, autotune.lookup(self._tune_key()), 0)
+                             table.ref() if table is not None else None, autotune.lookup(self._tune_key(two)), 0)
+
+    def _step2_into(self, f, out, ext_source=None):
+        """Two steps in one pass (``xb_darcy_step2``)."""
+        nat.check(nat.lib().xb_darcy_step2(self._args(f, out, ext_source, None, True), nat.stream()),
+                  "xb_darcy_step2")
+        return out
 
     def autotune(self):
         """Measure and cache the best sweep launch shape of this solver's step."""
@@ -145,7 +151,16 @@ class AdvectionDiffusionReactionBGK(LBMBase):
             a.rows = rows
             nat.check(lib.xb_darcy_step(a, nat.stream()), "xb_darcy_step(tune)")
         key = self._tune_key()
-        return {key: autotune.tune(key, run)}
+        out = {key: autotune.tune(key, run)}
+        if min(self.nx, self.ny) >= 2:
+            a2 = self._args(f, o, None, None, True)
+
+            def run2(rows):
+                a2.rows = rows
+                nat.check(lib.xb_darcy_step2(a2, nat.stream()), "xb_darcy_step2(tune)")
+            key2 = self._tune_key(True)
+            out[key2] = autotune.tune(key2, run2)
+        return out
 
     def _step_soa(self, f, ext_source=None, fpc=False, out=None):
         out = torch.empty_like(f) if out is None else out
@@ -157,10 +172,12 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         """``n`` steps without I/O (the body of ``run`` when nothing happens between
         steps), replayed from a captured HIP graph of ``chunk`` steps."""
         fs = self._soa(f, 9)
-        key = ("steps", chunk)
+        two = chunk % 4 == 0 and min(self.nx, self.ny) >= 2
+        key = ("steps", chunk, two)
         runner = self._runners.get(key)
         if runner is None:
-            runner = self._runners[key] = StepRunner(lambda s, d: self._step_soa(s, None, False, out=d), fs, chunk)
+            runner = self._runners[key] = StepRunner(lambda s, d: self._step_soa(s, None, False, out=d), fs, chunk,
+                                                     (lambda s, d: self._step2_into(s, d)) if two else None)
         return view(runner.run(fs, n))
 
     def collision(self, f, external_source=None):

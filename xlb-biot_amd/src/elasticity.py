@@ -330,10 +330,10 @@ class LBMElasticity(object):
         return torch.empty((c, nx, ny), dtype=self._tdt, device=self.device)
 
     # ---- kernels
-    def _tune_key(self, level, mode, has_ext, has_res):
+    def _tune_key(self, level, mode, has_ext, has_res, two=False):
         nx, ny = self.grid_size(level)
         return (f"elast/{self.precision}/{nx}x{ny}/mode{mode}/bc{int(self._tables.get(level) is not None)}"
-                f"/ext{int(has_ext)}/res{int(has_res)}")
+                f"/ext{int(has_ext)}/res{int(has_res)}" + ("/x2" if two else ""))
 
     def _launch(self, f_in, f_out, level, mode, ext=None, resid=None, fpc=None, coarse=None, partials=None,
                 use_force=None):
@@ -380,7 +380,17 @@ class LBMElasticity(object):
                     a.rows = rows
                     nat.check(lib.xb_elast_step(a, nat.stream()), "xb_elast_step(tune)")
                 key = self._tune_key(level, mode, ext is not None, r is not None)
-                out[key] = autotune.tune(key, run)
+                cands = [c for c in autotune.CANDIDATES if c >= 2] if mode == nat.XB_MODE_RESIDUAL \
+                    else autotune.CANDIDATES
+                out[key] = autotune.tune(key, run, cands)
+                if mode == nat.XB_MODE_STEP and min(nx, ny) >= 2:
+                    a2 = self._launch(f, o, level, mode, ext=ext)
+
+                    def run2(rows, a2=a2):
+                        a2.rows = rows
+                        nat.check(lib.xb_elast_step2(a2, nat.stream()), "xb_elast_step2(tune)")
+                    key2 = self._tune_key(level, mode, ext is not None, False, True)
+                    out[key2] = autotune.tune(key2, run2)
         return out
 
     def _run_step(self, f, ext=None, level=0, fpc=False):
@@ -396,17 +406,26 @@ class LBMElasticity(object):
         nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step")
         return out
 
+    def _step2_into(self, f, out, ext=None):
+        """Two single-grid steps in one pass (``xb_elast_step2``)."""
+        a = self._launch(f, out, 0, nat.XB_MODE_STEP, ext=ext)
+        a.rows = autotune.lookup(self._tune_key(0, nat.XB_MODE_STEP, ext is not None, False, True))
+        nat.check(nat.lib().xb_elast_step2(a, nat.stream()), "xb_elast_step2")
+        return out
+
     def run_steps(self, f, n, chunk=64, external_force=None):
         """``n`` single-grid steps with a fixed external force and no I/O, replayed
         from a captured HIP graph of ``chunk`` steps."""
         if self.levels != 1:
             raise ValueError("run_steps is the single-grid loop; multigrid uses step() / run()")
         fs = self._soa(f, 8)
-        key = ("steps", chunk, external_force is None)
+        two = chunk % 4 == 0 and min(self.nx, self.ny) >= 2
+        key = ("steps", chunk, external_force is None, two)
         runner = self._runners.get(key)
         if runner is None or external_force is not None:
             ext = self._soa(external_force, 2)
-            runner = StepRunner(lambda s, d: self._step_into(s, d, ext), fs, chunk)
+            runner = StepRunner(lambda s, d: self._step_into(s, d, ext), fs, chunk,
+                                (lambda s, d: self._step2_into(s, d, ext)) if two else None)
             if external_force is None:
                 self._runners[key] = runner
         return view(runner.run(fs, n))
