@@ -83,7 +83,7 @@ def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True):
     # every population once (compulsory 19 values per cell)
     spl = 2 if chunk % 4 == 0 else 1
     return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 19 * S,
-                steps_per_launch=spl)
+                steps_per_launch=spl, solver=d)
 
 
 def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=32):
@@ -430,6 +430,19 @@ def bench_adjoint(torch, n=1024, steps=1000, K=50, precision="f32/f32"):
     return out
 
 
+def profiled_traffic(kernel, blocks, cells):
+    """L2-to-memory bytes per launch of this kernel configuration from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, calibrated on a copy), or None
+    when that exact configuration was not profiled."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as fh:
+            table = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    rec = table.get(f"{kernel}|blocks={blocks}|cells={cells}")
+    return None if rec is None else rec["read_bytes"] + rec["write_bytes"]
+
+
 def cpu_baseline(n, precision, seconds):
     import torch
     import problems as P
@@ -460,6 +473,14 @@ def only(torch, args):
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"copy: {args.steps} launches of {a.numel() * 8} B")
+    elif args.only == "c2":
+        from src import autotune
+        r = bench_darcy(torch, args.n, "f64/f64", args.steps, 2, chunk=16)
+        d = r["solver"]
+        rows = autotune.lookup(d._tune_key(True))
+        items = -(-d.ny // 60) * -(-d.nx // rows) if rows else None
+        log(f"c2 {d.nx}x{d.ny}: two-step rows {rows}, grid {-(-items // 4) if items else '?'} blocks of 256, "
+            f"{r['kernel_s'] * 1e6:.2f} us/launch")
     elif args.only == "c5":
         print(json.dumps(bench_c5(torch, n_inst=args.instances)), flush=True)
     elif args.only == "adjoint":
@@ -482,7 +503,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched Biot extra")
-    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c5", "adjoint"],
+    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -502,6 +523,16 @@ def main():
     r = bench_darcy(torch, args.n, "f64/f64", args.steps, args.warmup)
     value = r["cells"] * args.steps / r["elapsed"] / 1e6
     achieved = r["bytes"] / r["kernel_s"] / 1e9
+    from src import autotune
+    d = r["solver"]
+    two = r["steps_per_launch"] == 2
+    rows = autotune.lookup(d._tune_key(two))
+    strip = 60 if two else 62
+    rows_eff = rows if rows else None
+    blocks = (-(-(-(-d.ny // strip) * -(-d.nx // rows_eff)) // 4)) if rows_eff else None
+    kname = ("k_sweep2<double, xb::DarcyColl<double>, 2>" if two
+             else "k_sweep<double, xb::DarcyColl<double>, 0, 2, false, false>")
+    traffic = profiled_traffic(kname, blocks, r["cells"]) if blocks else None
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["elapsed"] / args.steps * 1e3, "higher_is_better": True,
@@ -511,9 +542,11 @@ def main():
                                "FP64, p=1 left / p=0 right, no-flux top/bottom",
                    "nx": args.n + 2, "ny": args.n + 2, "precision": "f64/f64", "graph_chunk": 64},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_sweep2<double, DarcyColl, rules in LDS>" if r["steps_per_launch"] == 2
-                     else "k_sweep<double, DarcyColl, STEP, rules>",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and launch grid, "
+                                        "calibrated on a copy (profiles/darcy_c2_514_fp64_hbm_traffic.md)")
+                     if traffic else "not profiled for this launch grid",
+                     "kernel": kname, "launch_blocks": blocks, "rows_per_wavefront": rows,
                      "kernel_us": round(r["kernel_s"] * 1e6, 3), "steps_per_launch": r["steps_per_launch"],
                      "bytes_per_cell_per_launch": 152, "cells": r["cells"],
                      "per_step_equivalent_gbs": round(achieved * r["steps_per_launch"], 1)},
