@@ -118,3 +118,60 @@ def test_biot_batch_matches_sequential_bitwise(scale):
         for i, p in enumerate(seq[:2]):
             assert all(r <= 1e-6 for r in p.inner_residuals), (i, p.inner_residuals, p.inner_counts)
         say("inner residuals", [p.inner_residuals for p in seq], "cycles", [p.inner_counts for p in seq])
+
+
+WALLS = [("neumann", 16, 2), ("neumann", 32, 3), ("dirichlet", 16, 2), ("dirichlet", 32, 2)]
+
+
+def _walled(kind, n, levels):
+    return (P.neumann_elasticity if kind == "neumann" else P.dirichlet_elasticity)("gpu", n=n, levels=levels)
+
+
+def test_elasticity_batch_with_walls_matches_sequential_bitwise():
+    # traction / Dirichlet walls and multigrid wall layers: rule tables read from
+    # global memory (per-cell traction constants) and staged in LDS (coarse levels)
+    from src.batch import ElasticityBatch
+    say("elasticity batch with walls", WALLS)
+    els = [_walled(*c) for c in WALLS]
+    ref, ref_cycles, reached = [], [], []
+    for el in els:
+        vg = el.vcycle_graph()
+        vg.load(torch.zeros((8, el.nx, el.ny), dtype=torch.float64, device="cuda"))
+        c, _, ok = vg.run_to(1e-6, 300)
+        ref.append(vg.state.clone())
+        ref_cycles.append(c)
+        reached.append(ok)
+    tables = [sorted({lvl: el._tables[lvl].nslots for lvl in el._tables if el._tables[lvl] is not None}.items())
+              for el in els]
+    say("rule slots per level", tables)
+    eb = ElasticityBatch([_walled(*c) for c in WALLS], tol=1e-6, max_cycles=300, poll=3, external_force=False)
+    got = eb.solve([torch.zeros((8, el.nx, el.ny), dtype=torch.float64, device="cuda") for el in els])
+    assert list(eb.last_cycles) == ref_cycles
+    assert list(eb.last_status) == [1 if ok else 2 for ok in reached]
+    for i, (a, b) in enumerate(zip(got, ref)):
+        assert torch.equal(a, b), (i, (a - b).abs().max().item())
+
+
+def test_biot_batch_consolidation_walls_matches_sequential_bitwise():
+    from src.batch import BiotBatch
+    say("biot batch, consolidation walls")
+
+    def make(nx):
+        p = P.biot_consolidation("gpu", nx=nx, levels=2, k_smooth=4)
+        p.elasticityTol = 1e-6
+        p.elasticityMaxSteps = 200
+        return p
+    sizes = [16, 32]
+    seq = [make(n) for n in sizes]
+    states = [p.initial_state() for p in seq]
+    bb = BiotBatch([make(n) for n in sizes], poll=2)
+    bstates = bb.initial_state()
+    for t in range(2):
+        ref = [p._step_soa(fe, fd, dde, t) for p, (fe, fd, dde) in zip(seq, states)]
+        got = bb.step(bstates, t)
+        for i in range(len(sizes)):
+            assert bb.problems[i].inner_counts[-1] == seq[i].inner_counts[-1], (t, i)
+            for k, (a, b) in enumerate(zip(got[i], ref[i])):
+                assert torch.equal(a, b), (t, i, k, (a - b).abs().max().item())
+        states = [(o[0], o[1], o[4]) for o in ref]
+        bstates = [(o[0], o[1], o[4]) for o in got]
