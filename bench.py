@@ -205,6 +205,40 @@ def bench_c3(torch, n=256, levels=4, steps=20, tol=1e-8, lam=1.0, mu=1.0, K=1.0,
             "fine_cell_vcycles_per_s_millions": round(n * n / vc_s / 1e6, 1)}
 
 
+def bench_copy(torch, reps=20):
+    """Achievable HBM bandwidth: a plain 1 GiB device-to-device copy (read + write)."""
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    t = event_pair_time(torch, lambda: b.copy_(a), reps)
+    gbs = 2 * a.numel() * 8 / t / 1e9
+    return {"bytes_per_launch": 2 * a.numel() * 8, "us": round(t * 1e6, 1), "gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def bench_biot_vcycle(torch, n=4096, levels=6, reps=10):
+    """One fused-coupling V-cycle of the periodic Biot step (manufactured forcing,
+    lambda = mu = 1, K = 1, k = 2 smoothing steps, FP64) on an n x n grid, replayed
+    from its captured graph; bytes from :func:`vcycle_bytes`."""
+    import problems as P
+    b = P.biot_periodic("gpu", nx=n, levels=levels, steps=1, device_hooks=True)
+    el = b.elasticity
+    el.autotune(external_force=True)
+    fe, fd, dde = b.initial_state()
+    ctx = b._pre_inner(fe, fd, dde, 1)
+    vg = el.vcycle_graph(ctx["force"], ctx["nlf"])
+    vg.load(fe)
+    vg.run(2)
+    t = event_pair_time(torch, vg.replay, reps)
+    byt = vcycle_bytes(n, levels, el.k_smooth)
+    gbs = byt / t / 1e9
+    fine = 2 * el.k_smooth + 1  # smoothing + residual sweeps over the finest grid
+    return {"grid": n, "levels": levels, "k_smooth": el.k_smooth, "vcycle_ms": round(t * 1e3, 3),
+            "algorithmic_gb": round(byt / 1e9, 3), "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "fine_cell_vcycles_per_s_millions": round(n * n / t / 1e6, 1),
+            "finest_level_sweeps_per_cycle": fine}
+
+
 def vcycle_bytes(n, levels, k, S=8):
     """Algorithmic HBM bytes of one fused-coupling V-cycle of a periodic n x n grid:
     level 0 smoothing reads f, the force and the divergence and writes f and the
@@ -485,6 +519,9 @@ def only(torch, args):
         items = -(-d.ny // 60) * -(-d.nx // rows) if rows else None
         log(f"c2 {d.nx}x{d.ny}: two-step rows {rows}, grid {-(-items // 4) if items else '?'} blocks of 256, "
             f"{r['kernel_s'] * 1e6:.2f} us/launch")
+    elif args.only == "biot_vcycle":
+        r = bench_biot_vcycle(torch, reps=args.steps)
+        log(f"biot vcycle: {r}")
     elif args.only == "c5":
         print(json.dumps(bench_c5(torch, n_inst=args.instances)), flush=True)
     elif args.only == "adjoint":
@@ -507,7 +544,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched Biot extra")
-    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint"],
+    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint", "biot_vcycle"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -582,6 +619,10 @@ def main():
         extra("c5_batched_biot_fp64", lambda: bench_c5(torch, n_inst=args.instances))
         log("adjoint: consolidation geometry 1024, FP32, 1000 pseudo-steps, checkpoints every 50")
         extra("adjoint_1024_fp32", lambda: bench_adjoint(torch))
+        log("device copy 1 GiB (achievable HBM bandwidth)")
+        extra("stream_copy", lambda: bench_copy(torch))
+        log("Biot 4096^2 fused-coupling V-cycle (6 levels, k = 2), fp64")
+        extra("biot_vcycle_4096_fp64", lambda: bench_biot_vcycle(torch))
         for n in (4096,):
             log(f"elasticity {n}^2 fp64 (HBM-resident)")
             rl = bench_large_elasticity(torch, n, 200, 20)

@@ -778,19 +778,39 @@ __global__ void __launch_bounds__(256) k_sqdiff(int64_t n, const T *__restrict__
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-// Sum of p[0..n) by one wavefront in a fixed order (lane-strided, then a butterfly).
-__device__ __forceinline__ double wave_sum(const double *__restrict__ p, int n, int lane) {
-  double s = 0.0;
-  for (int i = lane; i < n; i += WAVE) s += p[i];
+// Sum of p[0..n) by one block of NORM_THREADS threads in a fixed order: thread t
+// adds p[t + k * NORM_THREADS] into four interleaved accumulators (k mod 4), which
+// are combined, then the block's values are summed pairwise in LDS.  Every norm of
+// residual partials goes through this one function, so single-instance and batched
+// convergence tests see bit-identical norms.  Call from all threads of the block.
+constexpr int NORM_THREADS = 1024;
+__device__ __forceinline__ double block_sum(const double *__restrict__ p, int n, double *red) {
+  const int t = threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = t;
+  for (; i + 3 * NORM_THREADS < n; i += 4 * NORM_THREADS) {
+    a0 += p[i];
+    a1 += p[i + NORM_THREADS];
+    a2 += p[i + 2 * NORM_THREADS];
+    a3 += p[i + 3 * NORM_THREADS];
+  }
+  for (; i < n; i += NORM_THREADS) a0 += p[i];
+  __syncthreads(); // red may still be read by a previous call
+  red[t] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
 #pragma unroll
-  for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
-  return s;
+  for (int off = NORM_THREADS / 2; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  return red[0];
 }
 
-__global__ void k_norm_finalize(const double *__restrict__ partials, int n, double *__restrict__ out) {
-  const int lane = threadIdx.x;
-  const double s = wave_sum(partials, n, lane);
-  if (lane == 0) {
+__global__ void __launch_bounds__(NORM_THREADS) k_norm_finalize(const double *__restrict__ partials, int n,
+                                                                double *__restrict__ out) {
+  __shared__ double red[NORM_THREADS];
+  const double s = block_sum(partials, n, red);
+  if (threadIdx.x == 0) {
     out[0] = sqrt(s);
     out[1] = (isfinite(s) ? 0.0 : 1.0);
   }
@@ -981,22 +1001,25 @@ struct NormJob {
   int32_t inst;
 };
 
-// Per-instance convergence test after a batched V-cycle: one wavefront per job sums
-// that instance's residual partials (the order of k_norm_finalize), then updates
-// its state and clears its activity flag once converged (r <= tol * s, s the
-// instance's residual scale if given, else r0), out of cycles, or non-finite.  One block; thread 0 finally counts the active instances.
-__global__ void __launch_bounds__(1024)
+// Per-instance convergence test after a batched V-cycle: the block sums each active
+// job's residual partials in turn (block_sum, as k_norm_finalize does), then thread
+// 0 updates that instance's state and clears its activity flag once converged
+// (r <= tol * s, s the instance's residual scale if given, else r0), out of cycles,
+// or non-finite, and finally counts the active instances.  One block.
+__global__ void __launch_bounds__(NORM_THREADS)
 k_batch_converge(const NormJob *__restrict__ jobs, int njobs, int ninst, double tol, int max_cycles,
                  double *__restrict__ r0, double *__restrict__ rlast, int32_t *__restrict__ cycles,
                  int32_t *__restrict__ active, int32_t *__restrict__ status, int32_t *__restrict__ nactive,
                  const double *__restrict__ rref) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  for (int j = wave; j < njobs; j += nwaves) {
+  __shared__ double red[NORM_THREADS];
+  for (int j = 0; j < njobs; ++j) {
     const NormJob J = jobs[j];
-    if (active[J.inst] == 0) continue; // uniform per wavefront
-    const double s = wave_sum(J.partials, J.count, lane);
-    if (lane == 0) {
+    // every thread reads the flag before thread 0 may clear it below
+    const bool on = active[J.inst] != 0;
+    __syncthreads();
+    if (!on) continue; // uniform over the block
+    const double s = block_sum(J.partials, J.count, red);
+    if (threadIdx.x == 0) {
       const int i = J.inst;
       const double r = sqrt(s);
       const int c = cycles[i] + 1;
@@ -1017,8 +1040,8 @@ k_batch_converge(const NormJob *__restrict__ jobs, int njobs, int ninst, double 
         }
       }
     }
+    __syncthreads(); // thread 0's writes are seen by the block before the next job
   }
-  __syncthreads();
   if (threadIdx.x == 0) {
     int n = 0;
     for (int i = 0; i < ninst; ++i) n += active[i] != 0;
@@ -1026,15 +1049,14 @@ k_batch_converge(const NormJob *__restrict__ jobs, int njobs, int ninst, double 
   }
 }
 
-// out[inst] = sqrt(sum of the job's partials), one wavefront per job.
-__global__ void __launch_bounds__(1024)
+// out[inst] = sqrt(sum of the job's partials), the jobs in turn by one block.
+__global__ void __launch_bounds__(NORM_THREADS)
 k_batch_norms(const NormJob *__restrict__ jobs, int njobs, double *__restrict__ out) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  for (int j = wave; j < njobs; j += nwaves) {
+  __shared__ double red[NORM_THREADS];
+  for (int j = 0; j < njobs; ++j) {
     const NormJob J = jobs[j];
-    const double s = wave_sum(J.partials, J.count, lane);
-    if (lane == 0) out[J.inst] = sqrt(s);
+    const double s = block_sum(J.partials, J.count, red);
+    if (threadIdx.x == 0) out[J.inst] = sqrt(s);
   }
 }
 
@@ -1762,7 +1784,7 @@ int xb_sqdiff_partials(int64_t n, int32_t dtype, const void *a, const void *b, d
 
 int xb_norm_finalize(const double *partials, int32_t n, double *out, void *stream) {
   if (partials == nullptr || out == nullptr || n <= 0) return fail(XB_EINVAL, "xb_norm_finalize: bad arguments");
-  hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(WAVE), 0, static_cast<hipStream_t>(stream), partials, n, out);
+  hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(NORM_THREADS), 0, static_cast<hipStream_t>(stream), partials, n, out);
   return check_launch("xb_norm_finalize");
 }
 
@@ -1805,7 +1827,7 @@ int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converg
     return fail(XB_EINVAL, "xb_batch_converge: incomplete state");
   if (ninst <= 0 || max_cycles <= 0 || !(tol >= 0.0)) return fail(XB_EINVAL, "xb_batch_converge: bad arguments");
   const NormJob *jobs = reinterpret_cast<const NormJob *>(static_cast<const char *>(plan->table) + plan->jobs_offset);
-  hipLaunchKernelGGL(k_batch_converge, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs,
+  hipLaunchKernelGGL(k_batch_converge, dim3(1), dim3(NORM_THREADS), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs,
                      ninst, tol, max_cycles, state->r0, state->rlast, state->cycles, state->active, state->status,
                      state->nactive, state->rref);
   return check_launch("xb_batch_converge");
@@ -1876,7 +1898,7 @@ int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream) {
   if (plan == nullptr || plan->kind != XB_BATCH_NORM || plan->table == nullptr || out == nullptr)
     return fail(XB_EINVAL, "xb_batch_norms: needs a norm plan and an output");
   const NormJob *jobs = reinterpret_cast<const NormJob *>(static_cast<const char *>(plan->table) + plan->jobs_offset);
-  hipLaunchKernelGGL(k_batch_norms, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs, out);
+  hipLaunchKernelGGL(k_batch_norms, dim3(1), dim3(NORM_THREADS), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs, out);
   return check_launch("xb_batch_norms");
 }
 
