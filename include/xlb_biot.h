@@ -150,11 +150,12 @@ int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f,
                          void *stream);
 
 /* As xb_elast_prolong_add, and div[interior] = div(f) of the updated populations
- * (the divergence carried by the fused Biot coupling force). */
+ * (the divergence carried by the fused Biot coupling force), with per-cell Lame
+ * fields (N each, both or neither) when given. */
 int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, void *f,
                              const void *coarse, int32_t cnx, int32_t cny,
                              int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1,
-                             void *div, void *stream);
+                             void *div, const void *lam_field, const void *mu_field, void *stream);
 
 /* eta (2,nx,ny), sigma (3,nx,ny) from f and the optional forces.
  * Replaces update_macroscopic (src/elasticity.py:1002-1038). Either output may be null.
@@ -218,6 +219,83 @@ typedef struct xb_darcy_adjoint_args {
 
 int xb_elast_adjoint(const xb_elast_adjoint_args *args, void *stream);
 int xb_darcy_adjoint(const xb_darcy_adjoint_args *args, void *stream);
+
+/* ---- adjoint of the multigrid V-cycle and of the Biot coupler -----------------------
+ *
+ * What jax.grad through _mg_step (src/elasticity.py:1221-1312) and through the coupled
+ * step (src/poroelasticity.py:303-339) would give; the reference has no adjoint.
+ * Every adjoint output below accumulates (+=) unless it says "written". */
+
+enum { XB_ADJ_SMOOTH = 1, XB_ADJ_RESIDUAL = 2 };
+
+/* Transposed sweep of one V-cycle level.  Forward (xb_elast_step modes SMOOTH and
+ * RESIDUAL): o = (1-gamma) f + gamma S(f - r; F) on the whole level, and
+ * c = R(f - S(f - r; F)) with R the 2x2 sum over the interior box, r the level's
+ * residual (interior box; none on the finest level), F the sweep's total force. */
+typedef struct xb_elast_mg_adjoint_args {
+  int32_t nx, ny, dtype, mode;  /* mode: XB_ADJ_SMOOTH or XB_ADJ_RESIDUAL              */
+  double lam, mu, gamma;        /* lam, mu used where the fields are null              */
+  const void *lam_field;        /* optional (nx, ny), with mu_field (finest level)     */
+  const void *mu_field;
+  int32_t ix0, ix1, iy0, iy1;   /* interior box of the level                           */
+  const void *abar_in;          /* SMOOTH: (8, nx, ny) adjoint of o; RESIDUAL: (8, (ix1-ix0)/2, (iy1-iy0)/2) adjoint of c */
+  const void *abar_add;         /* optional (8, nx, ny) added to abar_out              */
+  void *abar_out;               /* written: (8, nx, ny) adjoint of f (!= abar_in)      */
+  void *rbar;                   /* optional (8, ix1-ix0, iy1-iy0): += adjoint of r     */
+  void *force_bar;              /* optional, written: (2, nx, ny) adjoint of F         */
+  const void *f_in;             /* with grad_lam / grad_mu: the sweep's f              */
+  const void *resid;            /* with f_in, where the level subtracts a residual     */
+  double *grad_lam;             /* optional (nx, ny), with grad_mu and the fields      */
+  double *grad_mu;
+  const xb_bc_table *bc;        /* the level's rule table (optional)                   */
+} xb_elast_mg_adjoint_args;
+
+int xb_elast_mg_adjoint(const xb_elast_mg_adjoint_args *args, void *stream);
+
+/* Transposed Biot coupling force F = G_scale(D f) of one sweep (the fused force of
+ * xb_elast_args.div_in): abar[d] += k w2_d (G^T force_bar) with w2 = 1,1,1,1,2,2,2,2
+ * and k the divergence coefficient; with f_in and the fields, the Lame-parameter
+ * derivative of D at f; base_bar (2, nx, ny, FP64) += force_bar when given.
+ * grad_bc: the gradient's wall table for this grid (as xb_elast_args.grad_bc). */
+int xb_elast_coupling_adjoint(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
+                              const void *lam_field, const void *mu_field, const void *force_bar,
+                              double scale, const xb_bc_table *grad_bc, void *abar,
+                              const void *f_in, double *grad_lam, double *grad_mu,
+                              double *base_bar, void *stream);
+
+/* qbar (1, nx, ny) += b * G_scale^T gbar: the transpose of xb_darcy_gradient's G. */
+int xb_darcy_gradient_adjoint(int32_t nx, int32_t ny, int32_t dtype, const void *gbar, double scale,
+                              const xb_bc_table *bc, double b, void *qbar, void *stream);
+
+/* coarse (8, cnx, cny), written: the sum of the four children of every coarse
+ * interior cell, zero on the coarse boundary layers (the transpose of the injection
+ * of xb_elast_prolong_add; arguments as there). */
+int xb_elast_restrict_sum(int32_t nx, int32_t ny, int32_t dtype, const void *f, void *coarse,
+                          int32_t cnx, int32_t cny, int32_t ix0, int32_t ix1, int32_t iy0,
+                          int32_t iy1, void *stream);
+
+/* fbar += (d eta/d f)^T eta_bar + (d sigma/d f)^T sigma_bar (xb_elast_macro; either
+ * bar may be null); with the fields and grad_lam / grad_mu, the stresses'
+ * Lame-parameter derivatives.  The forces' adjoint is eta_bar / 2. */
+int xb_elast_macro_adjoint(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
+                           const void *f, const void *eta_bar, const void *sigma_bar, void *fbar,
+                           const void *lam_field, const void *mu_field, double *grad_lam,
+                           double *grad_mu, void *stream);
+
+/* div = k (m_s(f) - m_s(f_sub)) (xb_elast_divergence): fbar += k w2 div_bar,
+ * fsub_bar -= k w2 div_bar (either may be null); with the fields and grad_lam /
+ * grad_mu (f_sub may be null), the derivative of k. */
+int xb_elast_divergence_adjoint(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu,
+                                const void *f, const void *f_sub, const void *div_bar, void *fbar,
+                                void *fsub_bar, const void *lam_field, const void *mu_field,
+                                double *grad_lam, double *grad_mu, void *stream);
+
+/* xb_darcy_macro transposed: fbar_d += p_bar + (-3 omega) c_d . grad_bar,
+ * source_bar += p_bar / 2 (the adjoint of either source), grad_omega += the
+ * relaxation-rate derivative of the gradient (needs f).  Bars may be null. */
+int xb_darcy_macro_adjoint(int32_t nx, int32_t ny, int32_t dtype, double omega, const void *omega_field,
+                           const void *f, const void *p_bar, const void *grad_bar, void *fbar,
+                           void *source_bar, double *grad_omega, void *stream);
 
 /* ---- Darcy (D2Q9 BGK advection-diffusion-reaction, zero velocity) ------------------ */
 

@@ -49,10 +49,15 @@ class OracleElasticity:
         self.lam_field = kwargs.get("lambda_field")
         self.mu_field = kwargs.get("mu_field")
 
-    def _coef(self):
+    def _fine(self, f):
+        """Per-cell Lame fields apply to arrays of the finest grid; coarse multigrid
+        levels use the scalar lambda, mu."""
+        return self.lam_field is not None and tuple(f.shape[:2]) == (self.nx, self.ny)
+
+    def _coef(self, fields=True):
         """(cs, cd, h, n) of the collision, scalars or per-cell (nx, ny, 1) tensors,
         from elasticity.py:811-813 and :976-979."""
-        if self.lam_field is None:
+        if self.lam_field is None or not fields:
             lm, mu = self.lamda + self.mu, self.mu
             g = 1.0 / (12.0 * lm - 4.0)
             g = 0.0 if abs(g) > 100 else g
@@ -65,7 +70,7 @@ class OracleElasticity:
 
     def _to_f(self, m, h, n):
         # m @ m_to_f with per-cell h, n in the column of m_s
-        if self.lam_field is None:
+        if not torch.is_tensor(h):
             return torch.matmul(m, self.m_to_f)
         base = self.m_to_f.clone()
         base[2, :] = 0.0
@@ -165,7 +170,7 @@ class OracleElasticity:
         f = pp.cast_to_compute(f)
         external_force = pp.cast_to_compute(external_force)
         m = torch.matmul(f, self.f_to_m)
-        cs, cd, h, n = self._coef()
+        cs, cd, h, n = self._coef(self._fine(f))
         if use_force and self.force is not None and external_force is not None:
             force = self.force + external_force
         elif external_force is not None:
@@ -202,7 +207,7 @@ class OracleElasticity:
 
     def update_divergence(self, f):
         # elasticity.py:1040-1058
-        lm = self.lamda + self.mu if self.lam_field is None else (self.lam_field + self.mu_field)[..., None]
+        lm = (self.lam_field + self.mu_field)[..., None] if self._fine(f) else self.lamda + self.mu
         return (-1.5 / (3 * lm + 1)) * torch.matmul(f.to(self.f_to_m.dtype), self.f_to_m[:, 2:3])
 
     # ---- step (elasticity.py:1128-1219)

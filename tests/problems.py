@@ -400,10 +400,13 @@ EL_CORNERS = {(0, 0): (0, 7), (0, 1): (0, 4), (1, 0): (2, 6), (1, 1): (2, 5)}
 
 
 def biot_consolidation(kind, nx=32, levels=2, steps=2, precision="f64/f64", k_smooth=4, lam=2.0, mu=1.0,
-                       alpha=1.0, K=0.5, load=1e-3, coupling="centered"):
+                       alpha=1.0, K=0.5, load=1e-3, coupling="centered", lambda_field=None, mu_field=None,
+                       omega_field=None):
     """C4-like 2D consolidation (examples/poroelasticity/2D.py): fixed bottom
     (full-way bounce-back), traction at the top (corner-fixed anti-bounce-back);
-    undrained bottom, drained top; periodic in x; ``ny = nx + 2``."""
+    undrained bottom, drained top; periodic in x; ``ny = nx + 2``.  Optional
+    per-cell Lame fields and Darcy rate ``(nx, ny)`` (the oracle takes tensors,
+    which may require grad)."""
     m = modules(kind)
     ADR_BB = _corner_fix(m["ADR_BB"], ADR_CORNERS)
     ADR_AntiBB = _corner_fix(m["ADR_ABB"], ADR_CORNERS)
@@ -437,12 +440,17 @@ def biot_consolidation(kind, nx=32, levels=2, steps=2, precision="f64/f64", k_sm
         def __init__(self, **kw):
             super().__init__(El, Darcy, **kw)
 
-    return Consolidation(**{"lattice_elasticity": m["D2Q8"](precision), "lattice_darcy": m["D2Q9"](precision),
-                            "elasticity_steps": steps, "elasticity_MG_levels": levels,
-                            "MG_BCs": {"bottom": True, "top": True}, "MG_n_smoothing": k_smooth,
-                            "coupling_method": coupling, "alpha": alpha, "lambda": lam, "mu": mu,
-                            "epsilon": 1.0 / nx, "omega": 1.0 / (3 * K + 0.5), "nx": nx, "ny": nx + 2, "nz": 0,
-                            "precision": precision, "print_info_rate": 0})
+    kw = {"lattice_elasticity": m["D2Q8"](precision), "lattice_darcy": m["D2Q9"](precision),
+          "elasticity_steps": steps, "elasticity_MG_levels": levels,
+          "MG_BCs": {"bottom": True, "top": True}, "MG_n_smoothing": k_smooth,
+          "coupling_method": coupling, "alpha": alpha, "lambda": lam, "mu": mu,
+          "epsilon": 1.0 / nx, "omega": 1.0 / (3 * K + 0.5), "nx": nx, "ny": nx + 2, "nz": 0,
+          "precision": precision, "print_info_rate": 0}
+    if lambda_field is not None:
+        kw["lambda_field"], kw["mu_field"] = lambda_field, mu_field
+    if omega_field is not None:
+        kw["omega"] = omega_field
+    return Consolidation(**kw)
 
 
 def consolidation_elasticity(kind, nx=16, lam=2.0, mu=1.0, precision="f64/f64", load=1e-3, lambda_field=None,

@@ -35,7 +35,8 @@ def test_library_exports_every_declared_symbol():
 STRUCTS = (("xb_elast_args", nat.ElastArgs), ("xb_darcy_args", nat.DarcyArgs), ("xb_bc_table", nat.BCTable),
            ("xb_prolong_job", nat.ProlongJob), ("xb_fill_job", nat.FillJob), ("xb_norm_job", nat.NormJob),
            ("xb_batch_plan", nat.BatchPlan), ("xb_converge_state", nat.ConvergeState),
-           ("xb_elast_adjoint_args", nat.ElastAdjointArgs), ("xb_darcy_adjoint_args", nat.DarcyAdjointArgs))
+           ("xb_elast_adjoint_args", nat.ElastAdjointArgs), ("xb_darcy_adjoint_args", nat.DarcyAdjointArgs),
+           ("xb_elast_mg_adjoint_args", nat.ElastMGAdjointArgs))
 
 
 def test_struct_sizes_and_offsets_match_the_c_compiler(tmp_path):

@@ -1,0 +1,167 @@
+"""Adjoint of the multigrid V-cycle and of the coupled Biot step, FP64.
+
+Pins: a transposed V-cycle is the exact transpose of the V-cycle's linear part
+(in the populations and in the static force), and its Lame-parameter derivatives
+match central differences; the gradient of a displacement / pressure misfit
+through checkpointed coupled steps matches ``torch.autograd`` through the oracle
+restatement (the stand-in for ``jax.grad`` through the reference, which cannot
+run here) and central finite differences in 16 random directions."""
+import numpy as np
+import pytest
+import torch
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _fields(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    lam = 2.0 + 0.3 * rng.standard_normal((nx, ny))
+    mu = 1.0 + 0.2 * rng.standard_normal((nx, ny))
+    K = 0.5 * np.exp(0.5 * rng.standard_normal((nx, ny)))
+    return lam, mu, 1.0 / (3.0 * K + 0.5)
+
+
+def _problem(nx, levels, k, n_inner, seed=1, alt=False):
+    p = P.biot_consolidation("gpu", nx=nx, levels=levels, steps=n_inner, k_smooth=k)
+    p.alternative_gradient = alt
+    lam, mu, om = _fields(nx, nx + 2, seed)
+    p.elasticity.set_lame_fields(lam, mu)
+    p.darcy.set_omega_field(om)
+    return p
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_vcycle_adjoint_is_the_transpose(levels):
+    from src.biot_adjoint import BiotAdjointSession
+    say("transposed V-cycle, consolidation 16 x 18, levels", levels)
+    p = _problem(16, levels, 2, 3)
+    s = BiotAdjointSession(p, 1, 4)
+    el = p.elasticity
+    nx, ny = el.nx, el.ny
+    g = torch.Generator().manual_seed(0)
+
+    def rnd(*shape, scale=1.0):
+        return (scale * torch.randn(shape, generator=g, dtype=torch.float64)).cuda()
+    f, d = rnd(8, nx, ny, scale=1e-3), rnd(8, nx, ny)
+    base, dF = rnd(2, nx, ny, scale=1e-3), rnd(2, nx, ny)
+    a = rnd(8, nx, ny)
+
+    def V(f, base):
+        out, _, states = s._vcycle_record(f, el._divergence_soa(f), base)
+        return out, states
+    out0, states = V(f, base)
+    acc = {"lam": torch.zeros((1, nx, ny), dtype=torch.float64, device="cuda"),
+           "mu": torch.zeros((1, nx, ny), dtype=torch.float64, device="cuda"),
+           "base": torch.zeros((2, nx, ny), dtype=torch.float64, device="cuda")}
+    ab = s._level_adjoint(0, a, states, None, acc)
+    lhs = ((V(f + d, base)[0] - out0) * a).sum().item()
+    rhs = (d * ab).sum().item()
+    say("populations", lhs, rhs)
+    assert _rel(lhs, rhs) < 1e-10
+    lhs = ((V(f, base + dF)[0] - out0) * a).sum().item()
+    rhs = (dF * acc["base"]).sum().item()
+    say("static force", lhs, rhs)
+    assert _rel(lhs, rhs) < 1e-10
+    lam0, mu0 = el._lam_field.clone().reshape(nx, ny), el._mu_field.clone().reshape(nx, ny)
+    h = 1e-6
+    for name in ("lam", "mu"):
+        dp = rnd(nx, ny)
+        outs = []
+        for sgn in (1.0, -1.0):
+            if name == "lam":
+                el.set_lame_fields(lam0 + sgn * h * dp, mu0)
+            else:
+                el.set_lame_fields(lam0, mu0 + sgn * h * dp)
+            outs.append(V(f, base)[0])
+        el.set_lame_fields(lam0, mu0)
+        fd = ((outs[0] - outs[1]) * a).sum().item() / (2 * h)
+        an = (dp * acc[name].reshape(nx, ny)).sum().item()
+        say(name, fd, an)
+        assert _rel(fd, an) < 1e-6
+
+
+def _misfit_targets(nx, ny, seed):
+    g = torch.Generator().manual_seed(seed)
+    return (1e-4 * torch.randn((2, nx, ny), generator=g, dtype=torch.float64),
+            1e-4 * torch.randn((1, nx, ny), generator=g, dtype=torch.float64))
+
+
+def _misfit(eta, p, targets):
+    te, tp = targets
+    return 0.5 * ((eta - te.to(eta.device)) ** 2).sum() + 0.5 * ((p - tp.to(p.device)) ** 2).sum()
+
+
+@pytest.mark.parametrize("alt", [False, True])
+def test_biot_gradient_matches_autograd_through_oracle(alt):
+    from src.biot_adjoint import biot_solve
+    # three physical steps: a Darcy step from the zero state conserves sum f for any
+    # omega, so the permeability enters the pressure from the second step on
+    nx, levels, k, n_inner, steps = 16, 2, 2, 3, 3
+    ny = nx + 2
+    say("coupled gradient vs autograd through the oracle, alternative_gradient", alt)
+    lam, mu, om = _fields(nx, ny, 2)
+    targets = _misfit_targets(nx, ny, 3)
+    p = P.biot_consolidation("gpu", nx=nx, levels=levels, steps=n_inner, k_smooth=k)
+    p.alternative_gradient = alt
+    ts = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for v in (om, lam, mu)]
+    eta, pr = biot_solve(p, *ts, steps, checkpoint_every=2)
+    J = _misfit(eta, pr, targets)
+    J.backward()
+    to = [torch.tensor(v, dtype=torch.float64, requires_grad=True) for v in (om, lam, mu)]
+    po = P.biot_consolidation("oracle", nx=nx, levels=levels, steps=n_inner, k_smooth=k, lambda_field=to[1],
+                              mu_field=to[2], omega_field=to[0][..., None])
+    po.alternative_gradient = alt
+    fe, fd, dde = po.initial_state()
+    for t in range(steps):
+        fe, fd, eta_o, _, dde, p_o, _ = po.step(fe, fd, dde, t)
+    Jo = _misfit(eta_o.permute(2, 0, 1), p_o.permute(2, 0, 1), targets)
+    Jo.backward()
+    say("J", J.item(), Jo.item())
+    assert _rel(J.item(), Jo.item()) < 1e-12
+    for name, a, b in zip(("omega", "lambda", "mu"), ts, to):
+        ga, gb = a.grad.cpu(), b.grad
+        err = (ga - gb).abs().max().item() / gb.abs().max().item()
+        say(name, "max rel err", err, "norm", gb.norm().item())
+        assert err < 1e-9, name
+
+
+def test_biot_gradient_matches_finite_differences_16_directions():
+    from src.biot_adjoint import biot_solve
+    nx, levels, k, n_inner, steps = 16, 2, 2, 4, 3
+    ny = nx + 2
+    say("coupled gradient vs central differences, 16 directions")
+    lam, mu, om = _fields(nx, ny, 4)
+    targets = _misfit_targets(nx, ny, 5)
+    p = P.biot_consolidation("gpu", nx=nx, levels=levels, steps=n_inner, k_smooth=k)
+    ts = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for v in (om, lam, mu)]
+    eta, pr = biot_solve(p, *ts, steps, checkpoint_every=3)
+    _misfit(eta, pr, targets).backward()
+    grads = [t.grad.clone() for t in ts]
+    base = [t.detach().clone() for t in ts]
+    g = torch.Generator().manual_seed(6)
+    h = 1e-5
+    worst = 0.0
+    with torch.no_grad():
+        for i in range(16):
+            ds = [(b.abs() * torch.randn(b.shape, generator=g, dtype=torch.float64).cuda()) for b in base]
+            Js = []
+            for sgn in (1.0, -1.0):
+                args = [b + sgn * h * d for b, d in zip(base, ds)]
+                e, q = biot_solve(p, *args, steps, checkpoint_every=3)
+                Js.append(_misfit(e, q, targets).item())
+            fd = (Js[0] - Js[1]) / (2 * h)
+            an = sum((gr * d).sum().item() for gr, d in zip(grads, ds))
+            worst = max(worst, _rel(fd, an))
+            say(i, fd, an)
+    say("worst relative difference", worst)
+    assert worst < 1e-6

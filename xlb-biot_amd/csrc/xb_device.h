@@ -139,6 +139,11 @@ template <typename T> struct ElastColl {
     return x;
   }
 
+  // divergence coefficient of the cell: per cell with Lame fields, else k0
+  __device__ __forceinline__ T kdiv(const Aux &x, T k0) const {
+    return lamf != nullptr ? T(-1.5) / (T(3) * (x.lam + x.mu) + T(1)) : k0;
+  }
+
   __device__ __forceinline__ void collide(const T (&f)[8], const Aux &x, T (&o)[8]) const {
     T cs_ = cs, cd_ = cd, h_ = h, n_ = n;
     if (lamf != nullptr) elast_coef<T>(x.lam, x.mu, cs_, cd_, h_, n_);
@@ -224,6 +229,12 @@ template <typename T> struct DarcyColl {
 // ---------------------------------------------------------------------------
 // Shared by the stand-alone kernels and the fused sweep so both paths do the same
 // arithmetic in the same order.
+
+// divergence coefficient k = -1.5 / (3 (lam + mu) + 1) of one cell with per-cell
+// Lame parameters (reference elasticity.py:1040-1058)
+template <typename T> __device__ __forceinline__ T elast_kdiv(T lam, T mu) {
+  return T(-1.5) / (T(3) * (lam + mu) + T(1));
+}
 
 // divergence of D2Q8 populations: k * m_s (reference elasticity.py:1040-1058)
 template <typename T> __device__ __forceinline__ T elast_div_of(const T (&v)[8], T k) {

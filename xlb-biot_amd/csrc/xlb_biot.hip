@@ -192,6 +192,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
   };
 
   T R0[Q], R1[Q], R2[Q], F1[Q];
+  T K1 = nl.kdiv; // HAS_NL: divergence coefficient of the output row (per cell with Lame fields)
   int S1 = -1;
   double nrm = 0.0;
   T acc[Q];
@@ -224,7 +225,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
           for (int d = 0; d < Q; ++d) fpc[d * N + cell] = R1[d];
         }
         if constexpr (HAS_NL && Q == 8) {
-          if (nl.div_out != nullptr) nl.div_out[cell] = elast_div_of<T>(v, nl.kdiv);
+          if (nl.div_out != nullptr) nl.div_out[cell] = elast_div_of<T>(v, K1);
         }
       }
     } else if constexpr (MODE == XB_MODE_SMOOTH) {
@@ -236,7 +237,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
           fout[d * N + cell] = o[d];
         }
         if constexpr (HAS_NL && Q == 8) {
-          if (nl.div_out != nullptr) nl.div_out[cell] = elast_div_of<T>(o, nl.kdiv);
+          if (nl.div_out != nullptr) nl.div_out[cell] = elast_div_of<T>(o, K1);
         }
       }
     } else {
@@ -269,6 +270,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
       F1[d] = cur.f[d];
     }
     S1 = cur.slot;
+    if constexpr (HAS_NL && Q == 8) K1 = coll.kdiv(cur.aux, nl.kdiv);
   };
 
   // rows xs-1 .. xe are read (xe: the halo row after the last output row)
@@ -281,6 +283,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
 #pragma unroll
   for (int d = 0; d < Q; ++d) F1[d] = B.f[d];
   S1 = B.slot;
+  if constexpr (HAS_NL && Q == 8) K1 = coll.kdiv(B.aux, nl.kdiv);
   // A holds row x+1 at even steps, B at odd steps; the other buffer takes row x+2
   for (int x = xs; x < xe; x += 2) {
     if (x + 2 <= xe) load_row(wrapi(x + 2, g.nx), B);
@@ -580,7 +583,8 @@ __global__ void k_prolong_add(int nx, int ny, T *__restrict__ f, const T *__rest
 
 template <typename T>
 __global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__restrict__ c, int cnx, int cny,
-                                  int ix0, int ix1, int iy0, int iy1, T k, T *__restrict__ div) {
+                                  int ix0, int ix1, int iy0, int iy1, T k, T *__restrict__ div,
+                                  const T *__restrict__ lamf, const T *__restrict__ muf) {
   const int nxi = ix1 - ix0, nyi = iy1 - iy0;
   const int64_t n = (int64_t)nxi * nyi;
   const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
@@ -594,7 +598,7 @@ __global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__
       v[d] = f[d * N + cell] + c[d * Nc + cc];
       f[d * N + cell] = v[d];
     }
-    div[cell] = elast_div_of<T>(v, k);
+    div[cell] = elast_div_of<T>(v, lamf != nullptr ? elast_kdiv<T>(lamf[cell], muf[cell]) : k);
   }
 }
 
@@ -638,7 +642,7 @@ __global__ void k_elast_div(int64_t N, T k0, const T *__restrict__ f, const T *_
     T v[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) v[d] = fs != nullptr ? f[d * N + i] - fs[d * N + i] : f[d * N + i];
-    const T k = lamf != nullptr ? T(-1.5) / (T(3) * (lamf[i] + muf[i]) + T(1)) : k0;
+    const T k = lamf != nullptr ? elast_kdiv<T>(lamf[i], muf[i]) : k0;
     out[i] = elast_div_of<T>(v, k);
   }
 }
@@ -930,8 +934,6 @@ int validate_elast(const xb_elast_args *a, const char *who) {
   }
   if ((a->lam_field == nullptr) != (a->mu_field == nullptr))
     return fail(XB_EINVAL, "%s: lam_field and mu_field go together", who);
-  if (a->lam_field != nullptr && a->div_in != nullptr)
-    return fail(XB_EUNSUPPORTED, "%s: per-cell Lame fields with the fused coupling force", who);
   return XB_OK;
 }
 
@@ -1298,14 +1300,15 @@ template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *
 // FP64 accumulators.  One thread per cell, grid-stride; the only cross-cell reads are
 // the 8 (9) neighbours of abar and, with walls, their rule slots.
 
-// adjoint of the post-collision populations of cell i (Stream^T and PS^T)
-template <typename T, int Q, class Lat, bool HAS_BC>
-__device__ __forceinline__ void adj_post_collision(const T *__restrict__ ab, const BCDev<T> &bc, int nx, int ny,
-                                                   int x, int y, size_t i, size_t N, T (&pb)[Q]) {
+// adjoint of the post-collision populations of cell i (Stream^T and PS^T); ab(d, j)
+// is the adjoint of post-streaming population d of cell j
+template <typename T, int Q, class Lat, bool HAS_BC, class AB>
+__device__ __forceinline__ void adj_post_collision(const AB &ab, const BCDev<T> &bc, int nx, int ny, int x, int y,
+                                                   size_t i, T (&pb)[Q]) {
 #pragma unroll
   for (int d = 0; d < Q; ++d) {
     const size_t jn = (size_t)wrapi(x + Lat::cx(d), nx) * ny + wrapi(y + Lat::cy(d), ny);
-    T v = ab[d * N + jn];
+    T v = ab(d, jn);
     if constexpr (HAS_BC) {
       if (bc.code_ps != nullptr) {
         const int s = bc.slot[jn];
@@ -1322,7 +1325,7 @@ __device__ __forceinline__ void adj_post_collision(const T *__restrict__ ab, con
       for (int d = 0; d < Q; ++d) {
         const int c = bc.code_ps[(size_t)s * Q + d];
         if (c != 0 && c != XB_RULE_ZERO) {
-          const T a = ab[d * N + i];
+          const T a = ab(d, i);
           const int k = (c > 0 ? c : -c) - 1;
 #pragma unroll
           for (int e = 0; e < Q; ++e)
@@ -1363,6 +1366,61 @@ __device__ __forceinline__ void adj_pc_rules(const BCDev<T> &bc, size_t i, const
   }
 }
 
+// C^T of the D2Q8 moment collision at one cell: out = C^T ob + fpre.  The
+// adjoints of the post-collision moments (B0..B6) and the sums S03, S47 are kept
+// for the parameter and force derivatives.
+template <typename T> struct ElastMomBar {
+  T S03, S47, B0, B1, B2, B3, B4, B5, B6;
+};
+
+template <typename T>
+__device__ __forceinline__ ElastMomBar<T> elast_collision_adjoint(const T (&ob)[8], const T (&fpre)[8], T cs, T cd,
+                                                                  T h, T n, T (&out)[8]) {
+  ElastMomBar<T> M;
+  M.S03 = ob[0] + ob[1] + ob[2] + ob[3];
+  M.S47 = ob[4] + ob[5] + ob[6] + ob[7];
+  M.B0 = T(0.5) * (ob[0] - ob[2]);
+  M.B1 = T(0.5) * (ob[1] - ob[3]);
+  M.B2 = h * M.S03 + n * M.S47;
+  M.B3 = T(0.25) * (ob[0] - ob[1] + ob[2] - ob[3]);
+  M.B4 = T(0.25) * (ob[4] - ob[5] + ob[6] - ob[7]);
+  M.B5 = T(0.5) * (ob[2] - ob[0]) + T(0.25) * (ob[4] - ob[5] - ob[6] + ob[7]);
+  M.B6 = T(0.5) * (ob[3] - ob[1]) + T(0.25) * (ob[4] + ob[5] - ob[6] - ob[7]);
+  // moments of the populations: m10, m01 (+F), m_s * cs, m_d * cd, m_11 * cd
+  const T b10 = M.B0 + M.B5 / T(3), b01 = M.B1 + M.B6 / T(3);
+  const T bs = cs * M.B2, bd = cd * M.B3, b11 = cd * M.B4;
+  out[0] = b10 + bs + bd + fpre[0];
+  out[1] = b01 + bs - bd + fpre[1];
+  out[2] = -b10 + bs + bd + fpre[2];
+  out[3] = -b01 + bs - bd + fpre[3];
+  out[4] = b10 + b01 + T(2) * bs + b11 + fpre[4];
+  out[5] = -b10 + b01 + T(2) * bs - b11 + fpre[5];
+  out[6] = -b10 - b01 + T(2) * bs + b11 + fpre[6];
+  out[7] = b10 - b01 + T(2) * bs - b11 + fpre[7];
+  return M;
+}
+
+// d(abar . C(f)) / d(lambda, mu) at the pre-collision populations f of one cell
+template <typename T>
+__device__ __forceinline__ void elast_param_bar(const ElastMomBar<T> &M, const T (&f)[8], T lam, T mu, T cs,
+                                                double &g_lam, double &g_mu) {
+  const T ms = f[0] + f[1] + f[2] + f[3] + T(2) * (f[4] + f[5] + f[6] + f[7]);
+  const T md = f[0] - f[1] + f[2] - f[3];
+  const T m11 = f[4] - f[5] + f[6] - f[7];
+  const double M2 = (double)ms * (double)cs;
+  const double bcs = (double)M.B2 * (double)ms;
+  const double bcd = (double)M.B3 * (double)md + (double)M.B4 * (double)m11;
+  const double bh = (double)M.S03 * M2, bn = (double)M.S47 * M2;
+  const double lm = (double)lam + (double)mu;
+  double g = 1.0 / (12.0 * lm - 4.0);
+  const double dg = std::fabs(g) > 100.0 ? 0.0 : -12.0 * g * g;
+  const double dcs = 6.0 / ((3.0 * lm + 1.0) * (3.0 * lm + 1.0));
+  const double dcd = 12.0 / ((6.0 * (double)mu + 1.0) * (6.0 * (double)mu + 1.0));
+  const double bL = bcs * dcs + (0.5 * bh - 0.25 * bn) * dg;
+  g_lam = bL;
+  g_mu = bL + bcd * dcd;
+}
+
 template <typename T> struct ElastAdjDev {
   int nx, ny;
   T lam, mu;
@@ -1379,10 +1437,11 @@ __global__ void __launch_bounds__(256) k_elast_adjoint(const ElastAdjDev<T> A) {
   using C = ElastColl<T>;
   const int nx = A.nx, ny = A.ny;
   const size_t N = (size_t)nx * ny;
+  const auto ab = [&](int d, size_t j) { return A.ab_in[d * N + j]; };
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
     const int x = (int)(i / ny), y = (int)(i % ny);
     T pb[8], ob[8], fpre[8];
-    adj_post_collision<T, 8, C, HAS_BC>(A.ab_in, A.bc, nx, ny, x, y, i, N, pb);
+    adj_post_collision<T, 8, C, HAS_BC>(ab, A.bc, nx, ny, x, y, i, pb);
     adj_pc_rules<T, 8, HAS_BC>(A.bc, i, pb, ob, fpre);
     T lam = A.lam, mu = A.mu;
     if (A.lamf != nullptr) {
@@ -1391,53 +1450,307 @@ __global__ void __launch_bounds__(256) k_elast_adjoint(const ElastAdjDev<T> A) {
     }
     T cs, cd, h, n;
     elast_coef<T>(lam, mu, cs, cd, h, n);
-    // T^T: adjoints of the post-collision moments (M0..M6)
-    const T S03 = ob[0] + ob[1] + ob[2] + ob[3], S47 = ob[4] + ob[5] + ob[6] + ob[7];
-    const T B0 = T(0.5) * (ob[0] - ob[2]);
-    const T B1 = T(0.5) * (ob[1] - ob[3]);
-    const T B2 = h * S03 + n * S47;
-    const T B3 = T(0.25) * (ob[0] - ob[1] + ob[2] - ob[3]);
-    const T B4 = T(0.25) * (ob[4] - ob[5] + ob[6] - ob[7]);
-    const T B5 = T(0.5) * (ob[2] - ob[0]) + T(0.25) * (ob[4] - ob[5] - ob[6] + ob[7]);
-    const T B6 = T(0.5) * (ob[3] - ob[1]) + T(0.25) * (ob[4] + ob[5] - ob[6] - ob[7]);
-    // moments of the populations: m10, m01 (+F), m_s * cs, m_d * cd, m_11 * cd
-    const T b10 = B0 + B5 / T(3), b01 = B1 + B6 / T(3);
-    const T bs = cs * B2, bd = cd * B3, b11 = cd * B4;
     T out[8];
-    out[0] = b10 + bs + bd + fpre[0];
-    out[1] = b01 + bs - bd + fpre[1];
-    out[2] = -b10 + bs + bd + fpre[2];
-    out[3] = -b01 + bs - bd + fpre[3];
-    out[4] = b10 + b01 + T(2) * bs + b11 + fpre[4];
-    out[5] = -b10 + b01 + T(2) * bs - b11 + fpre[5];
-    out[6] = -b10 - b01 + T(2) * bs + b11 + fpre[6];
-    out[7] = b10 - b01 + T(2) * bs - b11 + fpre[7];
+    const ElastMomBar<T> M = elast_collision_adjoint<T>(ob, fpre, cs, cd, h, n, out);
 #pragma unroll
     for (int d = 0; d < 8; ++d) A.ab_out[d * N + i] = A.ab_add != nullptr ? out[d] + A.ab_add[d * N + i] : out[d];
     if (A.g_lam != nullptr || A.g_mu != nullptr) {
       T f[8];
 #pragma unroll
       for (int d = 0; d < 8; ++d) f[d] = A.f_in[d * N + i];
-      const T ms = f[0] + f[1] + f[2] + f[3] + T(2) * (f[4] + f[5] + f[6] + f[7]);
-      const T md = f[0] - f[1] + f[2] - f[3];
-      const T m11 = f[4] - f[5] + f[6] - f[7];
-      const double M2 = (double)ms * (double)cs;
-      const double bcs = (double)B2 * (double)ms;
-      const double bcd = (double)B3 * (double)md + (double)B4 * (double)m11;
-      const double bh = (double)S03 * M2, bn = (double)S47 * M2;
-      const double lm = (double)lam + (double)mu;
-      double g = 1.0 / (12.0 * lm - 4.0);
-      const double dg = std::fabs(g) > 100.0 ? 0.0 : -12.0 * g * g;
-      const double dcs = 6.0 / ((3.0 * lm + 1.0) * (3.0 * lm + 1.0));
-      const double dcd = 12.0 / ((6.0 * (double)mu + 1.0) * (6.0 * (double)mu + 1.0));
-      const double bL = bcs * dcs + (0.5 * bh - 0.25 * bn) * dg;
-      if (A.g_lam != nullptr) A.g_lam[i] += bL;
-      if (A.g_mu != nullptr) A.g_mu[i] += bL + bcd * dcd;
+      double gl, gm;
+      elast_param_bar<T>(M, f, lam, mu, cs, gl, gm);
+      if (A.g_lam != nullptr) A.g_lam[i] += gl;
+      if (A.g_mu != nullptr) A.g_mu[i] += gm;
     }
     if (A.g_force != nullptr) {
-      A.g_force[i] += (double)(B0 + B5 / T(6));
-      A.g_force[N + i] += (double)(B1 + B6 / T(6));
+      A.g_force[i] += (double)(M.B0 + M.B5 / T(6));
+      A.g_force[N + i] += (double)(M.B1 + M.B6 / T(6));
     }
+  }
+}
+
+// Transposed multigrid sweep (smoothing or residual) of one level.  Forward:
+//   SMOOTH    o = (1 - gamma) f + gamma S(f - r; F)                  (all cells)
+//   RESIDUAL  c = R(f - S(f - r; F)), R the 2x2 sum over the interior box
+// with r the level's residual (interior box; none on the finest level) and F the
+// sweep's total force.  Given the adjoint of o (of c), the kernel writes the adjoint
+// of f, adds the adjoint of r, writes the adjoint of F, and adds the Lame-parameter
+// derivatives at f - r (finest level, per-cell fields).
+template <typename T> struct ElastMGAdjDev {
+  int nx, ny, ix0, ix1, iy0, iy1;
+  T lam, mu, gamma;
+  const T *lamf, *muf;
+  const T *ab_in, *ab_add;
+  T *ab_out, *rbar, *fbar;
+  const T *f_in, *resid;
+  double *g_lam, *g_mu;
+  BCDev<T> bc;
+};
+
+template <typename T, int MODE, bool HAS_BC>
+__global__ void __launch_bounds__(256) k_elast_mg_adjoint(const ElastMGAdjDev<T> A) {
+  using C = ElastColl<T>;
+  const int nx = A.nx, ny = A.ny;
+  const size_t N = (size_t)nx * ny;
+  const int nyi = A.iy1 - A.iy0;
+  const size_t Ni = (size_t)(A.ix1 - A.ix0) * nyi;
+  const int cnyi = nyi >> 1;
+  const size_t Nci = Ni >> 2;
+  // adjoint of the residual f - S(f - r) at cell j (zero outside the interior box)
+  const auto res_bar = [&](int d, int xj, int yj) {
+    if (xj < A.ix0 || xj >= A.ix1 || yj < A.iy0 || yj >= A.iy1) return T(0);
+    return A.ab_in[d * Nci + (size_t)((xj - A.ix0) >> 1) * cnyi + ((yj - A.iy0) >> 1)];
+  };
+  // adjoint of S's output
+  const auto vbar = [&](int d, size_t j) {
+    if constexpr (MODE == XB_ADJ_SMOOTH) {
+      return A.gamma * A.ab_in[d * N + j];
+    } else {
+      return -res_bar(d, (int)(j / ny), (int)(j % ny));
+    }
+  };
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    T pb[8], ob[8], fpre[8];
+    adj_post_collision<T, 8, C, HAS_BC>(vbar, A.bc, nx, ny, x, y, i, pb);
+    adj_pc_rules<T, 8, HAS_BC>(A.bc, i, pb, ob, fpre);
+    T lam = A.lam, mu = A.mu;
+    if (A.lamf != nullptr) {
+      lam = A.lamf[i];
+      mu = A.muf[i];
+    }
+    T cs, cd, h, n;
+    elast_coef<T>(lam, mu, cs, cd, h, n);
+    T yv[8];
+    const ElastMomBar<T> M = elast_collision_adjoint<T>(ob, fpre, cs, cd, h, n, yv);
+    const bool in_box = x >= A.ix0 && x < A.ix1 && y >= A.iy0 && y < A.iy1;
+    T o[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      T v = yv[d];
+      if constexpr (MODE == XB_ADJ_SMOOTH) v = v + (T(1) - A.gamma) * A.ab_in[d * N + i];
+      else v = v + res_bar(d, x, y);
+      if (A.ab_add != nullptr) v = v + A.ab_add[d * N + i];
+      o[d] = v;
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) A.ab_out[d * N + i] = o[d];
+    if (A.rbar != nullptr && in_box) {
+      const size_t ci = (size_t)(x - A.ix0) * nyi + (y - A.iy0);
+#pragma unroll
+      for (int d = 0; d < 8; ++d) A.rbar[d * Ni + ci] = A.rbar[d * Ni + ci] - yv[d];
+    }
+    if (A.fbar != nullptr) {
+      A.fbar[i] = M.B0 + M.B5 / T(6);
+      A.fbar[N + i] = M.B1 + M.B6 / T(6);
+    }
+    if (A.g_lam != nullptr) {
+      T f[8];
+#pragma unroll
+      for (int d = 0; d < 8; ++d) f[d] = A.f_in[d * N + i];
+      if (A.resid != nullptr && in_box) {
+        const size_t ci = (size_t)(x - A.ix0) * nyi + (y - A.iy0);
+#pragma unroll
+        for (int d = 0; d < 8; ++d) f[d] = f[d] - A.resid[d * Ni + ci];
+      }
+      double gl, gm;
+      elast_param_bar<T>(M, f, lam, mu, cs, gl, gm);
+      A.g_lam[i] += gl;
+      A.g_mu[i] += gm;
+    }
+  }
+}
+
+// Transposed lattice gradient G_s (xb_darcy_gradient with walls): qbar(x) =
+// sum over the stencil slots d of the cells that read q(x) of (-3 s w_d) c_d . gbar.
+// COUPLE: the transposed Biot coupling force F = G_s(D f) of a sweep instead adds
+// k(x) w2_d qbar(x) to the populations' adjoint (D^T, w2 = 1 1 1 1 2 2 2 2), the
+// Lame-parameter derivative of D at f, and gbar to a FP64 accumulator of the
+// sweep's base force.
+template <typename T> struct GradAdjDev {
+  int nx, ny;
+  T m3s, b, kdiv;
+  const T *gbar;
+  T *qbar;                // !COUPLE: qbar += b * G^T gbar
+  T *fbar;                // COUPLE
+  const T *f_in;          // COUPLE with g_lam
+  const T *lamf, *muf;
+  double *g_lam, *g_mu, *base_bar;
+  BCDev<T> bc;
+};
+
+template <typename T, bool HAS_BC, bool COUPLE>
+__global__ void __launch_bounds__(256) k_grad_adjoint(const GradAdjDev<T> A) {
+  using D = DarcyColl<T>;
+  const int nx = A.nx, ny = A.ny;
+  const size_t N = (size_t)nx * ny;
+  const auto slot_bar = [&](int d, size_t j) {
+    const T cw = A.m3s * T(D::wd(d));
+    T v = T(0);
+    if (D::cx(d) != 0) v = v + (D::cx(d) > 0 ? cw : -cw) * A.gbar[j];
+    if (D::cy(d) != 0) v = v + (D::cy(d) > 0 ? cw : -cw) * A.gbar[N + j];
+    return v;
+  };
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    T pb[9];
+    adj_post_collision<T, 9, D, HAS_BC>(slot_bar, A.bc, nx, ny, x, y, i, pb);
+    T q = pb[0];
+#pragma unroll
+    for (int d = 1; d < 9; ++d) q = q + pb[d];
+    if constexpr (!COUPLE) {
+      A.qbar[i] = A.qbar[i] + A.b * q;
+    } else {
+      T k = A.kdiv;
+      double dk = 0.0;
+      if (A.lamf != nullptr) {
+        const T lm = A.lamf[i] + A.muf[i];
+        k = elast_kdiv<T>(A.lamf[i], A.muf[i]);
+        const double l3 = 3.0 * (double)lm + 1.0;
+        dk = 4.5 / (l3 * l3);
+      }
+      const T kq = k * q;
+#pragma unroll
+      for (int d = 0; d < 8; ++d) A.fbar[d * N + i] = A.fbar[d * N + i] + (d < 4 ? kq : T(2) * kq);
+      if (A.g_lam != nullptr) {
+        T ms = A.f_in[i] + A.f_in[N + i] + A.f_in[2 * N + i] + A.f_in[3 * N + i];
+        ms = ms + T(2) * (A.f_in[4 * N + i] + A.f_in[5 * N + i] + A.f_in[6 * N + i] + A.f_in[7 * N + i]);
+        const double g = (double)q * (double)ms * dk;
+        A.g_lam[i] += g;
+        A.g_mu[i] += g;
+      }
+      if (A.base_bar != nullptr) {
+        A.base_bar[i] += (double)A.gbar[i];
+        A.base_bar[N + i] += (double)A.gbar[N + i];
+      }
+    }
+  }
+}
+
+// coarse = P^T fine: the sum of the four children of each coarse interior cell, zero
+// on the coarse boundary layers (the transpose of the injection of xb_elast_prolong_add)
+template <typename T>
+__global__ void k_restrict_sum(int nx, int ny, const T *__restrict__ f, T *__restrict__ c, int cnx, int cny, int ix0,
+                               int ix1, int iy0, int iy1) {
+  const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
+  const int cx1 = ix0 + ((ix1 - ix0) >> 1), cy1 = iy0 + ((iy1 - iy0) >> 1);
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < Nc; t += (size_t)gridDim.x * blockDim.x) {
+    const int X = (int)(t / cny), Y = (int)(t % cny);
+    const bool in = X >= ix0 && X < cx1 && Y >= iy0 && Y < cy1;
+    const size_t a = in ? (size_t)(ix0 + 2 * (X - ix0)) * ny + (iy0 + 2 * (Y - iy0)) : 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      T v = T(0);
+      if (in) {
+        const T *p = f + d * N + a;
+        v = (p[0] + p[ny]) + (p[1] + p[ny + 1]);
+      }
+      c[d * Nc + t] = v;
+    }
+  }
+}
+
+// Adjoints of the pointwise maps of the Biot coupler; every output accumulates.
+// stresses and displacement: fbar += (d eta / d f)^T eta_bar + (d sigma / d f)^T sigma_bar
+template <typename T>
+__global__ void k_elast_macro_adjoint(int64_t N, T lam, T mu, const T *__restrict__ f, const T *__restrict__ eb,
+                                      const T *__restrict__ sb, T *__restrict__ fbar, const T *__restrict__ lamf,
+                                      const T *__restrict__ muf, double *__restrict__ g_lam,
+                                      double *__restrict__ g_mu) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    T o[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) o[d] = T(0);
+    if (eb != nullptr) {
+      const T e0 = eb[i], e1 = eb[N + i];
+#pragma unroll
+      for (int d = 0; d < 8; ++d) o[d] = T(ElastColl<T>::cx(d)) * e0 + T(ElastColl<T>::cy(d)) * e1;
+    }
+    if (sb != nullptr) {
+      const T l = lamf != nullptr ? lamf[i] : lam, m = lamf != nullptr ? muf[i] : mu;
+      const T lm = l + m;
+      const T cs = T(-1.5) * lm / (T(3) * lm + T(1));
+      const T cd = T(-3) * m / (T(6) * m + T(1));
+      const T s0 = sb[i], s1 = sb[N + i], s2 = sb[2 * N + i];
+      const T as = cs * (s0 + s2), ad = cd * (s0 - s2), a11 = T(2) * cd * s1;
+      o[0] = o[0] + as + ad;
+      o[1] = o[1] + as - ad;
+      o[2] = o[2] + as + ad;
+      o[3] = o[3] + as - ad;
+      o[4] = o[4] + T(2) * as + a11;
+      o[5] = o[5] + T(2) * as - a11;
+      o[6] = o[6] + T(2) * as + a11;
+      o[7] = o[7] + T(2) * as - a11;
+      if (g_lam != nullptr) {
+        T v[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) v[d] = f[d * N + i];
+        const double ms = (double)(v[0] + v[1] + v[2] + v[3] + T(2) * (v[4] + v[5] + v[6] + v[7]));
+        const double md = (double)(v[0] - v[1] + v[2] - v[3]);
+        const double m11 = (double)(v[4] - v[5] + v[6] - v[7]);
+        const double l3 = 3.0 * (double)lm + 1.0, m6 = 6.0 * (double)m + 1.0;
+        const double gL = ((double)s0 + (double)s2) * ms * (-1.5 / (l3 * l3));
+        const double gM = (((double)s0 - (double)s2) * md + 2.0 * (double)s1 * m11) * (-3.0 / (m6 * m6));
+        g_lam[i] += gL;
+        g_mu[i] += gL + gM;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) fbar[d * N + i] = fbar[d * N + i] + o[d];
+  }
+}
+
+// div = k (m_s(f) - m_s(f_sub)): fbar += k w2 dbar, fsub_bar -= k w2 dbar
+template <typename T>
+__global__ void k_elast_div_adjoint(int64_t N, T k0, const T *__restrict__ f, const T *__restrict__ fs,
+                                    const T *__restrict__ db, T *__restrict__ fbar, T *__restrict__ fsbar,
+                                    const T *__restrict__ lamf, const T *__restrict__ muf, double *__restrict__ g_lam,
+                                    double *__restrict__ g_mu) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const T k = lamf != nullptr ? elast_kdiv<T>(lamf[i], muf[i]) : k0;
+    const T a = k * db[i];
+    if (fbar != nullptr) {
+#pragma unroll
+      for (int d = 0; d < 8; ++d) fbar[d * N + i] = fbar[d * N + i] + (d < 4 ? a : T(2) * a);
+    }
+    if (fsbar != nullptr) {
+#pragma unroll
+      for (int d = 0; d < 8; ++d) fsbar[d * N + i] = fsbar[d * N + i] - (d < 4 ? a : T(2) * a);
+    }
+    if (g_lam != nullptr) {
+      T v[8];
+#pragma unroll
+      for (int d = 0; d < 8; ++d) v[d] = fs != nullptr ? f[d * N + i] - fs[d * N + i] : f[d * N + i];
+      const double ms = (double)(v[0] + v[1] + v[2] + v[3] + T(2) * (v[4] + v[5] + v[6] + v[7]));
+      const double l3 = 3.0 * ((double)lamf[i] + (double)muf[i]) + 1.0;
+      const double g = (double)db[i] * ms * (4.5 / (l3 * l3));
+      g_lam[i] += g;
+      g_mu[i] += g;
+    }
+  }
+}
+
+// p = sum f + S/2, grad = -3 omega sum c f: fbar_d += pbar + (-3 omega) c_d . gbar,
+// sbar += pbar / 2, grad_omega += -3 sum_d (c_d . gbar) f_d
+template <typename T>
+__global__ void k_darcy_macro_adjoint(int64_t N, T omega, const T *__restrict__ omf, const T *__restrict__ f,
+                                      const T *__restrict__ pb, const T *__restrict__ gb, T *__restrict__ fbar,
+                                      T *__restrict__ sbar, double *__restrict__ g_omega) {
+  using D = DarcyColl<T>;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const T p = pb != nullptr ? pb[i] : T(0);
+    const T g0 = gb != nullptr ? gb[i] : T(0), g1 = gb != nullptr ? gb[N + i] : T(0);
+    const T om = omf != nullptr ? omf[i] : omega;
+    double go = 0.0;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const T cg = T(D::cx(d)) * g0 + T(D::cy(d)) * g1;
+      if (fbar != nullptr) fbar[d * N + i] = fbar[d * N + i] + (p + T(-3) * om * cg);
+      if (g_omega != nullptr && gb != nullptr) go += -3.0 * (double)cg * (double)f[d * N + i];
+    }
+    if (sbar != nullptr && pb != nullptr) sbar[i] = sbar[i] + T(0.5) * p;
+    if (g_omega != nullptr && gb != nullptr) g_omega[i] += go;
   }
 }
 
@@ -1457,10 +1770,11 @@ __global__ void __launch_bounds__(256) k_darcy_adjoint(const DarcyAdjDev<T> A) {
   using C = DarcyColl<T>;
   const int nx = A.nx, ny = A.ny;
   const size_t N = (size_t)nx * ny;
+  const auto ab = [&](int d, size_t j) { return A.ab_in[d * N + j]; };
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
     const int x = (int)(i / ny), y = (int)(i % ny);
     T pb[9], ob[9], fpre[9];
-    adj_post_collision<T, 9, C, HAS_BC>(A.ab_in, A.bc, nx, ny, x, y, i, N, pb);
+    adj_post_collision<T, 9, C, HAS_BC>(ab, A.bc, nx, ny, x, y, i, pb);
     adj_pc_rules<T, 9, HAS_BC>(A.bc, i, pb, ob, fpre);
     const T om = A.omf != nullptr ? A.omf[i] : A.omega;
     T Sw = T(0);
@@ -1490,6 +1804,66 @@ __global__ void __launch_bounds__(256) k_darcy_adjoint(const DarcyAdjDev<T> A) {
     }
     if (A.g_source != nullptr) A.g_source[i] += (double)Sw;
   }
+}
+
+template <typename T> int elast_mg_adjoint_t(const xb_elast_mg_adjoint_args *a, hipStream_t st) {
+  ElastMGAdjDev<T> A{};
+  A.nx = a->nx; A.ny = a->ny;
+  A.ix0 = a->ix0; A.ix1 = a->ix1; A.iy0 = a->iy0; A.iy1 = a->iy1;
+  A.lam = (T)a->lam; A.mu = (T)a->mu; A.gamma = (T)a->gamma;
+  A.lamf = static_cast<const T *>(a->lam_field);
+  A.muf = static_cast<const T *>(a->mu_field);
+  A.ab_in = static_cast<const T *>(a->abar_in);
+  A.ab_add = static_cast<const T *>(a->abar_add);
+  A.ab_out = static_cast<T *>(a->abar_out);
+  A.rbar = static_cast<T *>(a->rbar);
+  A.fbar = static_cast<T *>(a->force_bar);
+  A.f_in = static_cast<const T *>(a->f_in);
+  A.resid = static_cast<const T *>(a->resid);
+  A.g_lam = a->grad_lam;
+  A.g_mu = a->grad_mu;
+  A.bc = bcdev<T>(a->bc);
+  const bool has_bc = A.bc.slot != nullptr;
+  const dim3 gr = ew_grid((int64_t)a->nx * a->ny);
+  if (a->mode == XB_ADJ_SMOOTH) {
+    if (has_bc) hipLaunchKernelGGL((k_elast_mg_adjoint<T, XB_ADJ_SMOOTH, true>), gr, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_elast_mg_adjoint<T, XB_ADJ_SMOOTH, false>), gr, dim3(256), 0, st, A);
+  } else {
+    if (has_bc) hipLaunchKernelGGL((k_elast_mg_adjoint<T, XB_ADJ_RESIDUAL, true>), gr, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_elast_mg_adjoint<T, XB_ADJ_RESIDUAL, false>), gr, dim3(256), 0, st, A);
+  }
+  return check_launch("xb_elast_mg_adjoint");
+}
+
+template <typename T>
+int grad_adjoint_t(int nx, int ny, double scale, double b, double kdiv, const void *gbar, void *qbar, void *fbar,
+                   const void *f_in, const void *lamf, const void *muf, double *g_lam, double *g_mu,
+                   double *base_bar, const xb_bc_table *bct, bool couple, hipStream_t st) {
+  GradAdjDev<T> A{};
+  A.nx = nx; A.ny = ny;
+  A.m3s = T(-3 * scale);
+  A.b = T(b);
+  A.kdiv = T(kdiv);
+  A.gbar = static_cast<const T *>(gbar);
+  A.qbar = static_cast<T *>(qbar);
+  A.fbar = static_cast<T *>(fbar);
+  A.f_in = static_cast<const T *>(f_in);
+  A.lamf = static_cast<const T *>(lamf);
+  A.muf = static_cast<const T *>(muf);
+  A.g_lam = g_lam;
+  A.g_mu = g_mu;
+  A.base_bar = base_bar;
+  A.bc = bcdev<T>(bct);
+  const bool has_bc = A.bc.slot != nullptr;
+  const dim3 gr = ew_grid((int64_t)nx * ny);
+  if (couple) {
+    if (has_bc) hipLaunchKernelGGL((k_grad_adjoint<T, true, true>), gr, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_grad_adjoint<T, false, true>), gr, dim3(256), 0, st, A);
+  } else {
+    if (has_bc) hipLaunchKernelGGL((k_grad_adjoint<T, true, false>), gr, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_grad_adjoint<T, false, false>), gr, dim3(256), 0, st, A);
+  }
+  return check_launch(couple ? "xb_elast_coupling_adjoint" : "xb_darcy_gradient_adjoint");
 }
 
 } // namespace
@@ -1556,10 +1930,12 @@ int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const v
 
 int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, void *f,
                              const void *coarse, int32_t cnx, int32_t cny, int32_t ix0, int32_t ix1, int32_t iy0,
-                             int32_t iy1, void *div, void *stream) {
+                             int32_t iy1, void *div, const void *lam_field, const void *mu_field, void *stream) {
   if (int r = check_dtype(dtype)) return r;
   if (f == nullptr || coarse == nullptr || div == nullptr)
     return fail(XB_EINVAL, "xb_elast_prolong_add_div: null pointer");
+  if ((lam_field == nullptr) != (mu_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_prolong_add_div: lam_field and mu_field go together");
   if (ix0 < 0 || ix1 > nx || iy0 < 0 || iy1 > ny || ix0 >= ix1 || iy0 >= iy1 || ((ix1 - ix0) & 1) || ((iy1 - iy0) & 1))
     return fail(XB_EINVAL, "xb_elast_prolong_add_div: bad interior box");
   if (cnx != ix0 + (ix1 - ix0) / 2 + (nx - ix1) || cny != iy0 + (iy1 - iy0) / 2 + (ny - iy1))
@@ -1569,10 +1945,12 @@ int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, 
   const double k = -1.5 / (3 * (lam + mu) + 1);
   if (dtype == XB_F64)
     hipLaunchKernelGGL(k_prolong_add_div<double>, gr, dim3(256), 0, st, nx, ny, (double *)f, (const double *)coarse,
-                       cnx, cny, ix0, ix1, iy0, iy1, k, (double *)div);
+                       cnx, cny, ix0, ix1, iy0, iy1, k, (double *)div, (const double *)lam_field,
+                       (const double *)mu_field);
   else
     hipLaunchKernelGGL(k_prolong_add_div<float>, gr, dim3(256), 0, st, nx, ny, (float *)f, (const float *)coarse, cnx,
-                       cny, ix0, ix1, iy0, iy1, (float)k, (float *)div);
+                       cny, ix0, ix1, iy0, iy1, (float)k, (float *)div, (const float *)lam_field,
+                       (const float *)mu_field);
   return check_launch("xb_elast_prolong_add_div");
 }
 
@@ -1900,6 +2278,154 @@ int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream) {
   const NormJob *jobs = reinterpret_cast<const NormJob *>(static_cast<const char *>(plan->table) + plan->jobs_offset);
   hipLaunchKernelGGL(k_batch_norms, dim3(1), dim3(NORM_THREADS), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs, out);
   return check_launch("xb_batch_norms");
+}
+
+int xb_elast_mg_adjoint(const xb_elast_mg_adjoint_args *a, void *stream) {
+  if (a == nullptr) return fail(XB_EINVAL, "xb_elast_mg_adjoint: null args");
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->mode != XB_ADJ_SMOOTH && a->mode != XB_ADJ_RESIDUAL)
+    return fail(XB_EINVAL, "xb_elast_mg_adjoint: unknown mode %d", a->mode);
+  if (a->nx <= 0 || a->ny <= 0 || a->abar_in == nullptr || a->abar_out == nullptr)
+    return fail(XB_EINVAL, "xb_elast_mg_adjoint: bad arguments");
+  if (a->abar_in == a->abar_out || (a->abar_add != nullptr && a->abar_add == a->abar_out))
+    return fail(XB_EINVAL, "xb_elast_mg_adjoint: abar_out must differ from abar_in and abar_add");
+  if (a->ix0 < 0 || a->ix1 > a->nx || a->iy0 < 0 || a->iy1 > a->ny || a->ix0 >= a->ix1 || a->iy0 >= a->iy1)
+    return fail(XB_EINVAL, "xb_elast_mg_adjoint: bad interior box");
+  if (a->mode == XB_ADJ_RESIDUAL && (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1)))
+    return fail(XB_EINVAL, "xb_elast_mg_adjoint: the restricted box must have even sides");
+  if ((a->lam_field == nullptr) != (a->mu_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_mg_adjoint: lam_field and mu_field go together");
+  if ((a->grad_lam == nullptr) != (a->grad_mu == nullptr))
+    return fail(XB_EINVAL, "xb_elast_mg_adjoint: grad_lam and grad_mu go together");
+  if (a->grad_lam != nullptr && (a->f_in == nullptr || a->lam_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_mg_adjoint: parameter gradients need f_in and the Lame fields");
+  if (int r = check_bc(a->bc, false)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (a->dtype == XB_F64) return elast_mg_adjoint_t<double>(a, st);
+  return elast_mg_adjoint_t<float>(a, st);
+}
+
+int xb_elast_coupling_adjoint(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *lam_field,
+                              const void *mu_field, const void *force_bar, double scale, const xb_bc_table *grad_bc,
+                              void *abar, const void *f_in, double *grad_lam, double *grad_mu, double *base_bar,
+                              void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || force_bar == nullptr || abar == nullptr)
+    return fail(XB_EINVAL, "xb_elast_coupling_adjoint: bad arguments");
+  if ((lam_field == nullptr) != (mu_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_coupling_adjoint: lam_field and mu_field go together");
+  if ((grad_lam == nullptr) != (grad_mu == nullptr))
+    return fail(XB_EINVAL, "xb_elast_coupling_adjoint: grad_lam and grad_mu go together");
+  if (grad_lam != nullptr && (f_in == nullptr || lam_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_coupling_adjoint: parameter gradients need f_in and the Lame fields");
+  if (int r = check_bc(grad_bc, true)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const double k = -1.5 / (3 * (lam + mu) + 1);
+  if (dtype == XB_F64)
+    return grad_adjoint_t<double>(nx, ny, scale, 1.0, k, force_bar, nullptr, abar, f_in, lam_field, mu_field,
+                                  grad_lam, grad_mu, base_bar, grad_bc, true, st);
+  return grad_adjoint_t<float>(nx, ny, scale, 1.0, k, force_bar, nullptr, abar, f_in, lam_field, mu_field, grad_lam,
+                               grad_mu, base_bar, grad_bc, true, st);
+}
+
+int xb_darcy_gradient_adjoint(int32_t nx, int32_t ny, int32_t dtype, const void *gbar, double scale,
+                              const xb_bc_table *bc, double b, void *qbar, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || gbar == nullptr || qbar == nullptr)
+    return fail(XB_EINVAL, "xb_darcy_gradient_adjoint: bad arguments");
+  if (int r = check_bc(bc, true)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    return grad_adjoint_t<double>(nx, ny, scale, b, 0.0, gbar, qbar, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, bc, false, st);
+  return grad_adjoint_t<float>(nx, ny, scale, b, 0.0, gbar, qbar, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, nullptr, bc, false, st);
+}
+
+int xb_elast_restrict_sum(int32_t nx, int32_t ny, int32_t dtype, const void *f, void *coarse, int32_t cnx,
+                          int32_t cny, int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (f == nullptr || coarse == nullptr) return fail(XB_EINVAL, "xb_elast_restrict_sum: null pointer");
+  if (ix0 < 0 || ix1 > nx || iy0 < 0 || iy1 > ny || ix0 >= ix1 || iy0 >= iy1 || ((ix1 - ix0) & 1) || ((iy1 - iy0) & 1))
+    return fail(XB_EINVAL, "xb_elast_restrict_sum: bad interior box");
+  if (cnx != ix0 + (ix1 - ix0) / 2 + (nx - ix1) || cny != iy0 + (iy1 - iy0) / 2 + (ny - iy1))
+    return fail(XB_EINVAL, "xb_elast_restrict_sum: coarse grid %d x %d does not match the fine box", cnx, cny);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 gr = ew_grid((int64_t)cnx * cny);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_restrict_sum<double>, gr, dim3(256), 0, st, nx, ny, (const double *)f, (double *)coarse, cnx,
+                       cny, ix0, ix1, iy0, iy1);
+  else
+    hipLaunchKernelGGL(k_restrict_sum<float>, gr, dim3(256), 0, st, nx, ny, (const float *)f, (float *)coarse, cnx,
+                       cny, ix0, ix1, iy0, iy1);
+  return check_launch("xb_elast_restrict_sum");
+}
+
+int xb_elast_macro_adjoint(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *f,
+                           const void *eta_bar, const void *sigma_bar, void *fbar, const void *lam_field,
+                           const void *mu_field, double *grad_lam, double *grad_mu, void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || fbar == nullptr) return fail(XB_EINVAL, "xb_elast_macro_adjoint: bad arguments");
+  if ((lam_field == nullptr) != (mu_field == nullptr) || (grad_lam == nullptr) != (grad_mu == nullptr))
+    return fail(XB_EINVAL, "xb_elast_macro_adjoint: fields and gradients go in pairs");
+  if (grad_lam != nullptr && (f == nullptr || lam_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_macro_adjoint: parameter gradients need f and the Lame fields");
+  const int64_t N = (int64_t)nx * ny;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double *gl = sigma_bar != nullptr ? grad_lam : nullptr, *gm = sigma_bar != nullptr ? grad_mu : nullptr;
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_elast_macro_adjoint<double>, ew_grid(N), dim3(256), 0, st, N, lam, mu, (const double *)f,
+                       (const double *)eta_bar, (const double *)sigma_bar, (double *)fbar,
+                       (const double *)lam_field, (const double *)mu_field, gl, gm);
+  else
+    hipLaunchKernelGGL(k_elast_macro_adjoint<float>, ew_grid(N), dim3(256), 0, st, N, (float)lam, (float)mu,
+                       (const float *)f, (const float *)eta_bar, (const float *)sigma_bar, (float *)fbar,
+                       (const float *)lam_field, (const float *)mu_field, gl, gm);
+  return check_launch("xb_elast_macro_adjoint");
+}
+
+int xb_elast_divergence_adjoint(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *f,
+                                const void *f_sub, const void *div_bar, void *fbar, void *fsub_bar,
+                                const void *lam_field, const void *mu_field, double *grad_lam, double *grad_mu,
+                                void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0 || div_bar == nullptr) return fail(XB_EINVAL, "xb_elast_divergence_adjoint: bad arguments");
+  if ((lam_field == nullptr) != (mu_field == nullptr) || (grad_lam == nullptr) != (grad_mu == nullptr))
+    return fail(XB_EINVAL, "xb_elast_divergence_adjoint: fields and gradients go in pairs");
+  if (grad_lam != nullptr && (f == nullptr || lam_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_divergence_adjoint: parameter gradients need f and the Lame fields");
+  const int64_t N = (int64_t)nx * ny;
+  const double k = -1.5 / (3 * (lam + mu) + 1);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_elast_div_adjoint<double>, ew_grid(N), dim3(256), 0, st, N, k, (const double *)f,
+                       (const double *)f_sub, (const double *)div_bar, (double *)fbar, (double *)fsub_bar,
+                       (const double *)lam_field, (const double *)mu_field, grad_lam, grad_mu);
+  else
+    hipLaunchKernelGGL(k_elast_div_adjoint<float>, ew_grid(N), dim3(256), 0, st, N, (float)k, (const float *)f,
+                       (const float *)f_sub, (const float *)div_bar, (float *)fbar, (float *)fsub_bar,
+                       (const float *)lam_field, (const float *)mu_field, grad_lam, grad_mu);
+  return check_launch("xb_elast_divergence_adjoint");
+}
+
+int xb_darcy_macro_adjoint(int32_t nx, int32_t ny, int32_t dtype, double omega, const void *omega_field, const void *f,
+                           const void *p_bar, const void *grad_bar, void *fbar, void *source_bar, double *grad_omega,
+                           void *stream) {
+  if (int r = check_dtype(dtype)) return r;
+  if (nx <= 0 || ny <= 0) return fail(XB_EINVAL, "xb_darcy_macro_adjoint: bad arguments");
+  if (grad_omega != nullptr && grad_bar != nullptr && f == nullptr)
+    return fail(XB_EINVAL, "xb_darcy_macro_adjoint: the relaxation-rate gradient needs f");
+  const int64_t N = (int64_t)nx * ny;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == XB_F64)
+    hipLaunchKernelGGL(k_darcy_macro_adjoint<double>, ew_grid(N), dim3(256), 0, st, N, omega,
+                       (const double *)omega_field, (const double *)f, (const double *)p_bar,
+                       (const double *)grad_bar, (double *)fbar, (double *)source_bar, grad_omega);
+  else
+    hipLaunchKernelGGL(k_darcy_macro_adjoint<float>, ew_grid(N), dim3(256), 0, st, N, (float)omega,
+                       (const float *)omega_field, (const float *)f, (const float *)p_bar, (const float *)grad_bar,
+                       (float *)fbar, (float *)source_bar, grad_omega);
+  return check_launch("xb_darcy_macro_adjoint");
 }
 
 } // extern "C"

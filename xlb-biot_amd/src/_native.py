@@ -59,6 +59,18 @@ class DarcyAdjointArgs(ctypes.Structure):
                 ("grad_omega", c_void_p), ("grad_source", c_void_p), ("bc", ctypes.POINTER(BCTable))]
 
 
+XB_ADJ_SMOOTH, XB_ADJ_RESIDUAL = 1, 2
+
+
+class ElastMGAdjointArgs(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("dtype", c_int32), ("mode", c_int32),
+                ("lam", c_double), ("mu", c_double), ("gamma", c_double), ("lam_field", c_void_p),
+                ("mu_field", c_void_p), ("ix0", c_int32), ("ix1", c_int32), ("iy0", c_int32), ("iy1", c_int32),
+                ("abar_in", c_void_p), ("abar_add", c_void_p), ("abar_out", c_void_p), ("rbar", c_void_p),
+                ("force_bar", c_void_p), ("f_in", c_void_p), ("resid", c_void_p), ("grad_lam", c_void_p),
+                ("grad_mu", c_void_p), ("bc", ctypes.POINTER(BCTable))]
+
+
 XB_BATCH_ELAST_STEP, XB_BATCH_PROLONG_DIV, XB_BATCH_FILL, XB_BATCH_NORM = 1, 2, 3, 4
 
 
@@ -99,7 +111,7 @@ SIGNATURES = {
                                        c_int32, c_int32, c_int32, c_int32, c_void_p]),
     "xb_elast_prolong_add_div": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
                                            c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, c_void_p,
-                                           c_void_p]),
+                                           c_void_p, c_void_p, c_void_p]),
     "xb_elast_macro": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "xb_elast_divergence": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
@@ -108,6 +120,21 @@ SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "xb_elast_adjoint": (c_int32, [ctypes.POINTER(ElastAdjointArgs), c_void_p]),
     "xb_darcy_adjoint": (c_int32, [ctypes.POINTER(DarcyAdjointArgs), c_void_p]),
+    "xb_elast_mg_adjoint": (c_int32, [ctypes.POINTER(ElastMGAdjointArgs), c_void_p]),
+    "xb_elast_coupling_adjoint": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
+                                            c_void_p, c_double, ctypes.POINTER(BCTable), c_void_p, c_void_p,
+                                            c_void_p, c_void_p, c_void_p, c_void_p]),
+    "xb_darcy_gradient_adjoint": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_double, ctypes.POINTER(BCTable),
+                                            c_double, c_void_p, c_void_p]),
+    "xb_elast_restrict_sum": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                        c_int32, c_int32, c_int32, c_void_p]),
+    "xb_elast_macro_adjoint": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "xb_elast_divergence_adjoint": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p]),
+    "xb_darcy_macro_adjoint": (c_int32, [c_int32, c_int32, c_int32, c_double, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "xb_darcy_step": (c_int32, [ctypes.POINTER(DarcyArgs), c_void_p]),
     "xb_darcy_macro": (c_int32, [c_int32, c_int32, c_int32, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),

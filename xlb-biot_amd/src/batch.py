@@ -131,6 +131,8 @@ class ElasticityBatch:
                 raise ValueError("all instances of a batch share one precision")
             if el.k_smooth != ks:
                 raise ValueError("all instances of a batch share MG_n_smoothing")
+            if el._lam_field is not None:
+                raise NotImplementedError("batched V-cycles take scalar Lame parameters per instance")
         fused = [getattr(n, "fused", False) for n in nlfs]
         if any(n is not None for n in nlfs) and not all(fused):
             raise ValueError("batched coupling forces must all be fused BiotNonlinearForce objects")

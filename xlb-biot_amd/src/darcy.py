@@ -56,6 +56,16 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         """Per-cell relaxation rate ``(nx, ny, 1)`` or None."""
         return None if self._omega_field is None else view(self._omega_field)
 
+    def set_omega_field(self, omega_field):
+        """Per-cell relaxation rate ``(nx, ny[, 1])`` (copied; a later call overwrites
+        the field in place, so captured graphs keep reading it)."""
+        field = as_soa(omega_field, self.nx, self.ny, 1, self._tdt, self.device)
+        if self._omega_field is not None:
+            self._omega_field.copy_(field)
+            return
+        self._omega_field = field.clone()
+        self._runners = {}
+
     def _soa(self, x, c, nx=None, ny=None):
         if x is None:
             return None

@@ -57,15 +57,15 @@ class LBMElasticity(object):
         force = self.get_force()
         self._force_soa = None if force is None else as_soa(force, self.nx, self.ny, 2, self._tdt, self.device)
         self.force = None if self._force_soa is None else view(self._force_soa)
-        # extension: per-cell Lame parameters (nx, ny[, 1]) on the single grid; they
-        # replace lambda / mu in the collision, stresses, divergence and equilibrium
+        # extension: per-cell Lame parameters (nx, ny[, 1]) of the finest grid; they
+        # replace lambda / mu in its collision, stresses, divergence and equilibrium.
+        # Coarse multigrid levels keep the scalar lambda / mu (their operator only
+        # preconditions the fine-grid fixed point).
         self._lam_field = self._mu_field = None
         lf, mf = kwargs.get("lambda_field"), kwargs.get("mu_field")
         if (lf is None) != (mf is None):
             raise ValueError("lambda_field and mu_field go together")
         if lf is not None:
-            if self.levels != 1:
-                raise NotImplementedError("per-cell Lame fields are supported on the single grid (MG_levels = 1)")
             self.set_lame_fields(lf, mf)
         self._norm_ws = {}
         self._runners = {}
@@ -73,10 +73,16 @@ class LBMElasticity(object):
         self.last_residual = None
 
     def set_lame_fields(self, lam_field, mu_field):
-        """Per-cell Lame parameters ``(nx, ny[, 1])`` (arrays or device tensors; a
-        tensor of the solver's dtype and layout is used without a copy)."""
-        self._lam_field = as_soa(lam_field, self.nx, self.ny, 1, self._tdt, self.device)
-        self._mu_field = as_soa(mu_field, self.nx, self.ny, 1, self._tdt, self.device)
+        """Per-cell Lame parameters ``(nx, ny[, 1])`` of the finest grid (arrays or
+        device tensors, copied; a second call overwrites the first fields in place)."""
+        lf = as_soa(lam_field, self.nx, self.ny, 1, self._tdt, self.device)
+        mf = as_soa(mu_field, self.nx, self.ny, 1, self._tdt, self.device)
+        if self._lam_field is not None:
+            # in place: captured step and V-cycle graphs hold these addresses
+            self._lam_field.copy_(lf)
+            self._mu_field.copy_(mf)
+            return
+        self._lam_field, self._mu_field = lf.clone(), mf.clone()
         self._runners = {}
 
     def _fields(self):
@@ -592,7 +598,8 @@ class LBMElasticity(object):
             x0, x1, y0, y1 = self.interior_box(level)
             nat.check(nat.lib().xb_elast_prolong_add_div(nx, ny, nat.dtype_code(f.dtype), self.lamda, self.mu,
                                                          nat.ptr(f), nat.ptr(f_c), nxc, nyc, x0, x1, y0, y1,
-                                                         nat.ptr(div), nat.stream()), "xb_elast_prolong_add_div")
+                                                         nat.ptr(div), *(self._fields() if level == 0 else (None, None)),
+                                                         nat.stream()), "xb_elast_prolong_add_div")
         for k in range(self.k_smooth):
             last = k == self.k_smooth - 1
             f, div = self._smooth_nl(f, div, level, residual, base, nlf, out if last else None,
@@ -782,7 +789,7 @@ class VCycleGraph:
             el = self.el
             nat.check(nat.lib().xb_elast_divergence(el.nx, el.ny, nat.dtype_code(f_soa.dtype), el.lamda, el.mu,
                                                     nat.ptr(self.buf[self.cur]), None, nat.ptr(self.div[self.cur]),
-                                                    None, None, nat.stream()), "xb_elast_divergence")
+                                                    *el._fields(), nat.stream()), "xb_elast_divergence")
 
     def set_force(self, force_soa):
         if self.ext is None:
