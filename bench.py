@@ -468,6 +468,69 @@ def bench_adjoint(torch, n=1024, steps=1000, K=50, precision="f32/f32"):
     return out
 
 
+def bench_biot_adjoint(torch, n=1024, steps=50, ne=2, levels=None, K=50, precision="f32/f32", k_smooth=8):
+    """Gradient of a displacement / pressure misfit through the coupled multigrid
+    consolidation solve (2-D consolidation: n x (n + 2) cells, fixed undrained bottom,
+    traction on the drained top; multigrid depth by the rule of
+    examples/poroelasticity/2D.py:216-219 (10 levels for n = 1024), ``ne`` V-cycles
+    per physical step as there).  ``k_smooth`` 8: with lambda = 2, mu = 1 the V-cycle
+    with the driver's 4 smoothing steps diverges from 3 levels on (residual x 1e3 in
+    100 cycles at 1024, 4 levels, FP32), with 8 it converges at every depth.
+    Log-normal K (variance 2, correlation 32, seed 1),
+    lambda = 2 and mu = 1 held as fields; target (eta*, p*) from the forward solve with
+    K perturbed by a log-normal factor (variance 0.1, seed 2).  ``steps`` physical
+    steps, checkpoints every ``K`` V-cycles; state FP32, accumulators FP64."""
+    import problems as P
+    from src.biot_adjoint import biot_solve
+    dt = torch.float64 if precision == "f64/f64" else torch.float32
+    if levels is None:
+        # examples/poroelasticity/2D.py:216-219
+        levels = int(n).bit_length() - 1
+        while n % 2 ** (levels - 1):
+            levels -= 1
+    prob = P.biot_consolidation("gpu", nx=n, levels=levels, steps=ne, k_smooth=k_smooth, precision=precision)
+    nx, ny = prob.elasticity.nx, prob.elasticity.ny
+    Kf = 0.5 * torch.as_tensor(P.lognormal_field(nx, ny, 2.0, 32.0, 1), dtype=torch.float64)
+    Kt = Kf * torch.as_tensor(P.lognormal_field(nx, ny, 0.1, 32.0, 2), dtype=torch.float64)
+    lam = torch.full((nx, ny), 2.0, dtype=dt, device="cuda")
+    mu = torch.full((nx, ny), 1.0, dtype=dt, device="cuda")
+
+    def omega(k):
+        return (1.0 / (3.0 * k + 0.5)).to(device="cuda", dtype=dt)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    with torch.no_grad():
+        te, tp = biot_solve(prob, omega(Kt), lam, mu, steps, checkpoint_every=K)
+        te, tp = te.clone(), tp.clone()
+    Kg = Kf.to(device="cuda", dtype=dt).requires_grad_(True)
+    mug = mu.clone().requires_grad_(True)
+
+    def misfit():
+        e, q = biot_solve(prob, 1.0 / (3.0 * Kg + 0.5), lam, mug, steps, checkpoint_every=K)
+        return 0.5 * ((e - te) ** 2).sum(dtype=torch.float64) + 0.5 * ((q - tp) ** 2).sum(dtype=torch.float64)
+    misfit().backward()  # warm-up: session buffers and graphs
+    Kg.grad = mug.grad = None
+    torch.cuda.reset_peak_memory_stats()
+    J, t_fwd = timed(misfit)
+    _, t_bwd = timed(lambda: J.backward())
+    finite = bool(torch.isfinite(Kg.grad).all() and torch.isfinite(mug.grad).all())
+    s = next(iter(prob._biot_adjoint_sessions.values()))
+    return {"grid": [nx, ny], "levels": levels, "k_smooth": k_smooth, "vcycles_per_step": ne, "physical_steps": steps,
+            "checkpoint_every_vcycles": K, "checkpointed_steps": len(s.ckpt_step), "precision": precision,
+            "forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4),
+            "backward_over_forward": round(t_bwd / t_fwd, 3),
+            "gradient_over_forward": round((t_fwd + t_bwd) / t_fwd, 3),
+            "peak_memory_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2),
+            "misfit": float(J), "grad_K_norm": float(Kg.grad.double().norm()),
+            "grad_mu_norm": float(mug.grad.double().norm()), "finite": finite}
+
+
 def profiled_traffic(kernel, blocks, cells):
     """L2-to-memory bytes per launch of this kernel configuration from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, calibrated on a copy), or None
@@ -526,6 +589,8 @@ def only(torch, args):
         print(json.dumps(bench_c5(torch, n_inst=args.instances)), flush=True)
     elif args.only == "adjoint":
         print(json.dumps(bench_adjoint(torch)), flush=True)
+    elif args.only == "biot_adjoint":
+        print(json.dumps(bench_biot_adjoint(torch, n=args.n, steps=args.steps)), flush=True)
     elif args.only == "large_elast":
         r = bench_large_elasticity(torch, n, args.steps, 2)
         log(f"elasticity {n}^2: {r['kernel_s'] * 1e6:.1f} us/launch, {r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
@@ -544,7 +609,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched Biot extra")
-    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint", "biot_vcycle"],
+    ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint", "biot_vcycle",
+                             "biot_adjoint"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:

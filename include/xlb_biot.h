@@ -97,7 +97,7 @@ typedef struct xb_elast_args {
   double nl_scale;
   const xb_bc_table *grad_bc;
   /* Optional per-cell Lame parameters (nx, ny) each, both or neither; they replace
-   * lam, mu (not with the fused coupling force). */
+   * lam, mu (also in the divergence carried by the fused coupling force). */
   const void *lam_field;
   const void *mu_field;
 } xb_elast_args;

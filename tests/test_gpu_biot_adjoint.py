@@ -114,7 +114,9 @@ def test_biot_gradient_matches_autograd_through_oracle(alt):
     p = P.biot_consolidation("gpu", nx=nx, levels=levels, steps=n_inner, k_smooth=k)
     p.alternative_gradient = alt
     ts = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for v in (om, lam, mu)]
-    eta, pr = biot_solve(p, *ts, steps, checkpoint_every=2)
+    # K = 2 < 3 V-cycles per step: V-cycle checkpoints inside each step; K = 6:
+    # checkpoints every second step, segments of two steps recomputed
+    eta, pr = biot_solve(p, *ts, steps, checkpoint_every=6 if alt else 2)
     J = _misfit(eta, pr, targets)
     J.backward()
     to = [torch.tensor(v, dtype=torch.float64, requires_grad=True) for v in (om, lam, mu)]

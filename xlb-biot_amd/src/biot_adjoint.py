@@ -12,11 +12,13 @@ of the finest elasticity grid.
 Backward is the discrete adjoint of exactly that computation, what ``jax.grad``
 through the reference's steps would give:
 
-* the inputs of every physical step are checkpointed by the forward pass, and the
-  inputs of its V-cycles every ``checkpoint_every`` V-cycles (pseudo-steps);
-* physical steps are reversed last first; the V-cycles of a step are recomputed
-  segment by segment from their checkpoints (a recomputed V-cycle keeps the inputs
-  of its finest-level sweeps), then reversed;
+* the forward pass checkpoints the inputs of a physical step every
+  ``checkpoint_every`` V-cycles (pseudo-steps) - every ``checkpoint_every //
+  elasticity_steps`` steps, and inside steps of more V-cycles than that, their
+  V-cycle inputs every ``checkpoint_every`` V-cycles;
+* backward recomputes one segment at a time from its checkpoint (a recomputed
+  V-cycle keeps the inputs of its finest-level sweeps), then reverses it, physical
+  step by physical step and V-cycle by V-cycle;
 * a V-cycle is reversed level by level: transposed smoothing and residual sweeps
   (``xb_elast_mg_adjoint``; the residual's 2x2 restriction is transposed inside),
   the transposed coupling force of every sweep (``xb_elast_coupling_adjoint``) and
@@ -56,6 +58,9 @@ class BiotAdjointSession:
             raise ValueError("need steps >= 1 and a checkpoint interval >= 1")
         self.p, self.el, self.da = p, el, da
         self.steps, self.K, self.n = int(steps), int(K), p.elasticitySteps
+        # physical steps per checkpoint: K V-cycles' worth (at least one step; a step
+        # of more than K V-cycles also checkpoints its V-cycles every K)
+        self.P = max(1, self.K // self.n)
         self.dtype, self.device = el._tdt, el.device
         nx, ny = el.nx, el.ny
         if el._lam_field is None:
@@ -83,7 +88,8 @@ class BiotAdjointSession:
         self.ckpt_step, self.ckpt_cycle = [], []
         out = None
         for t in range(self.steps):
-            self.ckpt_step.append((fe.clone(), fd.clone(), dde.clone()))
+            if t % self.P == 0:
+                self.ckpt_step.append((fe.clone(), fd.clone(), dde.clone()))
             ctx = p._pre_inner(fe, fd, dde, t)
             vg.set_force(ctx["force"])
             vg.load(fe)
@@ -195,6 +201,24 @@ class BiotAdjointSession:
                 ab = self._level_adjoint(0, ab, states, None, acc)
         return ab
 
+    def _recompute_segment(self, t0, t1):
+        """Physical steps t0 .. t1-1 recomputed from the checkpoint of step t0: per
+        step its inputs, the coupler's pre-inner context, the records of its last
+        V-cycle segment and the V-cycles' output (a list, last step at the end)."""
+        p = self.p
+        fe, fd, dde = self.ckpt_step[t0 // self.P]
+        out = []
+        for t in range(t0, t1):
+            ctx = p._pre_inner(fe, fd, dde, t)
+            starts = [fe] + self.ckpt_cycle[t]
+            m_last = self.n - (len(starts) - 1) * self.K
+            recs, fe_new = self._recompute(starts[-1], m_last, ctx["force"])
+            out.append((fe, fd, dde, ctx, recs, fe_new))
+            if t + 1 < t1:
+                o = p._post_inner(ctx, fe_new)
+                fe, fd, dde = o[0], o[1], o[4]
+        return out
+
     # ---- transposed coupler
     def backward(self, eta_bar, p_bar, want_lame, want_omega):
         p, el, da = self.p, self.el, self.da
@@ -222,13 +246,11 @@ class BiotAdjointSession:
                                      nat.stream()), "xb_lincomb")
 
         for t in range(self.steps - 1, -1, -1):
+            if t == self.steps - 1 or (t + 1) % self.P == 0:
+                segment = self._recompute_segment(t - t % self.P, t + 1)
+            fe_prev, fd, dde, ctx, last_recs, fe_new = segment.pop()
             last = t == self.steps - 1
-            fe_prev, fd, dde = self.ckpt_step[t]
-            ctx = p._pre_inner(fe_prev, fd, dde, t)
             base = ctx["force"]
-            starts = [fe_prev] + self.ckpt_cycle[t]
-            m_last = self.n - (len(starts) - 1) * self.K
-            last_recs, fe_new = self._recompute(starts[-1], m_last, base)
             source = self._lincomb_new(ctx["source_base"], -(th * al / eps), el._divergence_soa(fe_new, fe_prev))
             acc["base"] = _zeros((2, nx, ny), torch.float64, dev)
             # Darcy step (poroelasticity.py:339)
