@@ -178,6 +178,45 @@ int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, doub
                          const void *force_b, void *f_out, const void *lam_field,
                          const void *mu_field, void *stream);
 
+/* ---- coarse tail of a V-cycle in one workgroup ------------------------------------
+ *
+ * Below a few thousand cells a level's sweep is bound by launch and load latency.
+ * xb_vcycle_tail runs the V-cycle of the levels L0 .. L0+nlevels-1 (the recursion of
+ * _mg_step below the level L0-1 that calls it, src/elasticity.py:1221-1312) from the
+ * zero state in one workgroup: per sweep every cell collides, then every cell pulls
+ * and applies its rules, with the per-cell arithmetic of xb_elast_step (bit-identical
+ * results).  The first level's resid is the input (the restricted residual of level
+ * L0-1); the result is in the first level's f0, for xb_elast_prolong_add(_div). */
+typedef struct xb_tail_level {
+  int32_t nx, ny, ix0, ix1, iy0, iy1; /* grid and interior box of the level           */
+  const xb_bc_table *bc;        /* the level's rules (may be null)                     */
+  const xb_bc_table *grad_bc;   /* coupling force: the gradient's wall rules (or null) */
+  void *f0, *f1;                /* (8, nx, ny) states; the result is in f0             */
+  void *div0, *div1;            /* (nx, ny) divergence carry of the coupling, or null  */
+  void *resid;                  /* (8, ix1-ix0, iy1-iy0) the level's residual          */
+  void *scratch;                /* (8, nx, ny) post-collision populations              */
+} xb_tail_level;
+
+typedef struct xb_tail_plan {   /* filled by xb_vcycle_tail_prepare                    */
+  int32_t dtype, nlevels, k_smooth, coupled;
+  double lam, mu, gamma, nl_scale;
+  void *table;                  /* device copy of the level descriptors                */
+} xb_tail_plan;
+
+/* Device bytes of the level descriptors of a tail of nlevels (<= 16) levels. */
+int64_t xb_vcycle_tail_bytes(int32_t nlevels);
+
+/* Checks the levels (each coarsens the previous one) and copies their descriptors
+ * into table (device memory of xb_vcycle_tail_bytes; synchronous).  coupled: the
+ * Biot coupling force G_scale(D f) with the divergence carry (nl_scale as
+ * xb_elast_args.nl_scale), else no force (coarse levels carry none). */
+int xb_vcycle_tail_prepare(int32_t dtype, double lam, double mu, double gamma, int32_t k_smooth,
+                           double nl_scale, int32_t coupled, const xb_tail_level *levels,
+                           int32_t nlevels, void *table, int64_t table_bytes, xb_tail_plan *plan);
+
+/* The tail V-cycle: one workgroup, graph-capturable. */
+int xb_vcycle_tail(const xb_tail_plan *plan, void *stream);
+
 /* ---- adjoint steps ---------------------------------------------------------------
  * No reference counterpart as kernels: they give what jax.grad through the jitted
  * _step (src/elasticity.py:1169-1219) / AdvectionDiffusionReactionBGK.step

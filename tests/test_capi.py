@@ -36,7 +36,8 @@ STRUCTS = (("xb_elast_args", nat.ElastArgs), ("xb_darcy_args", nat.DarcyArgs), (
            ("xb_prolong_job", nat.ProlongJob), ("xb_fill_job", nat.FillJob), ("xb_norm_job", nat.NormJob),
            ("xb_batch_plan", nat.BatchPlan), ("xb_converge_state", nat.ConvergeState),
            ("xb_elast_adjoint_args", nat.ElastAdjointArgs), ("xb_darcy_adjoint_args", nat.DarcyAdjointArgs),
-           ("xb_elast_mg_adjoint_args", nat.ElastMGAdjointArgs))
+           ("xb_elast_mg_adjoint_args", nat.ElastMGAdjointArgs), ("xb_tail_level", nat.TailLevel),
+           ("xb_tail_plan", nat.TailPlan))
 
 
 def test_struct_sizes_and_offsets_match_the_c_compiler(tmp_path):

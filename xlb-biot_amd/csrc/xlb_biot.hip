@@ -1866,6 +1866,225 @@ int grad_adjoint_t(int nx, int ny, double scale, double b, double kdiv, const vo
   return check_launch(couple ? "xb_elast_coupling_adjoint" : "xb_darcy_gradient_adjoint");
 }
 
+// ---------------------------------------------------------------------------
+// Coarse tail of a V-cycle in one workgroup.  Below a few thousand cells a level's
+// sweep is bound by launch and load latency (a wavefront walks its rows one after
+// another), so the levels L0.. of a V-cycle run as phases of one block separated by
+// barriers: per sweep, every cell collides (phase A, post-collision populations into
+// a scratch array), then every cell pulls from its neighbours and applies its rules
+// (phase B).  The per-cell arithmetic is that of sweep_item, in the same order, so
+// the result is bit-identical to the level-by-level launches.
+constexpr int TAIL_THREADS = 512;
+constexpr int TAIL_MAX_LEVELS = 16;
+
+template <typename T> struct TailLevelDev {
+  int nx, ny, ix0, ix1, iy0, iy1;
+  BCDev<T> bc, gbc;
+  T *f[2], *div[2];
+  T *resid, *pc;
+};
+
+template <typename T> struct TailDev {
+  T cs, cd, h, n, kdiv, gamma, omg, m3s;
+};
+
+// phase A of a sweep of level L from state s: pc = PC(C(f - r; F)), F the coupling force
+template <typename T, bool COUPLED>
+__device__ void tail_collide(const TailLevelDev<T> &L, const TailDev<T> &P, int s, bool has_r) {
+  using D = DarcyColl<T>;
+  ElastColl<T> coll;
+  coll.cs = P.cs; coll.cd = P.cd; coll.h = P.h; coll.n = P.n;
+  coll.has_b = COUPLED;
+  const int nx = L.nx, ny = L.ny, N = nx * ny;
+  const int nyi = L.iy1 - L.iy0;
+  const int Ni = (L.ix1 - L.ix0) * nyi;
+  const T *f = L.f[s];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int x = i / ny, y = i % ny;
+    T pre[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) pre[d] = f[d * N + i];
+    if (has_r && x >= L.ix0 && x < L.ix1 && y >= L.iy0 && y < L.iy1) {
+      const int ci = (x - L.ix0) * nyi + (y - L.iy0);
+#pragma unroll
+      for (int d = 0; d < 8; ++d) pre[d] = pre[d] - L.resid[d * Ni + ci];
+    }
+    typename ElastColl<T>::Aux ax{T(0), T(0), T(0), T(0), T(0), T(0)};
+    if constexpr (COUPLED) {
+      const T *dv = L.div[s];
+      T qs[9];
+#pragma unroll
+      for (int d = 0; d < 9; ++d) qs[d] = dv[wrapi(x - D::cx(d), nx) * ny + wrapi(y - D::cy(d), ny)];
+      const int gs = L.gbc.slot != nullptr ? L.gbc.slot[i] : -1;
+      if (gs >= 0) {
+        T src[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) src[d] = qs[0];
+        apply_rules<T, 9>(qs, src, L.gbc.code_ps + (size_t)gs * 9, L.gbc.const_ps + (size_t)gs * 9);
+      }
+      stencil_grad<T>(qs, P.m3s, ax.b0, ax.b1);
+    }
+    T pc[8];
+    coll.collide(pre, ax, pc);
+    const int sl = L.bc.slot != nullptr ? L.bc.slot[i] : -1;
+    if (sl >= 0 && L.bc.code_pc != nullptr)
+      apply_rules<T, 8>(pc, pre, L.bc.code_pc + (size_t)sl * 8, L.bc.const_pc + (size_t)sl * 8);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) L.pc[d * N + i] = pc[d];
+  }
+}
+
+// the streamed, post-streaming-ruled populations of cell (x, y)
+template <typename T>
+__device__ __forceinline__ void tail_pull(const TailLevelDev<T> &L, int x, int y, T (&v)[8]) {
+  using C = ElastColl<T>;
+  const int nx = L.nx, ny = L.ny, N = nx * ny;
+  const int i = x * ny + y;
+#pragma unroll
+  for (int d = 0; d < 8; ++d) v[d] = L.pc[d * N + wrapi(x - C::cx(d), nx) * ny + wrapi(y - C::cy(d), ny)];
+  const int sl = L.bc.slot != nullptr ? L.bc.slot[i] : -1;
+  if (sl >= 0 && L.bc.code_ps != nullptr) {
+    T own[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) own[d] = L.pc[d * N + i];
+    apply_rules<T, 8>(v, own, L.bc.code_ps + (size_t)sl * 8, L.bc.const_ps + (size_t)sl * 8);
+  }
+}
+
+// damped smoothing sweep of level L from state s into state 1 - s
+template <typename T, bool COUPLED>
+__device__ void tail_smooth(const TailLevelDev<T> &L, const TailDev<T> &P, int s, bool has_r) {
+  tail_collide<T, COUPLED>(L, P, s, has_r);
+  __syncthreads();
+  const int N = L.nx * L.ny;
+  const T *f = L.f[s];
+  T *o = L.f[1 - s];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    T v[8], w[8];
+    tail_pull<T>(L, i / L.ny, i % L.ny, v);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      w[d] = P.omg * f[d * N + i] + P.gamma * v[d];
+      o[d * N + i] = w[d];
+    }
+    if constexpr (COUPLED) L.div[1 - s][i] = elast_div_of<T>(w, P.kdiv);
+  }
+  __syncthreads();
+}
+
+// residual f - S(f - r) of level L, summed over 2 x 2 blocks of its interior box into
+// the residual of level Lc (pairs along y first, as the sweep)
+template <typename T, bool COUPLED>
+__device__ void tail_residual(const TailLevelDev<T> &L, const TailLevelDev<T> &Lc, const TailDev<T> &P, int s,
+                              bool has_r) {
+  tail_collide<T, COUPLED>(L, P, s, has_r);
+  __syncthreads();
+  const int N = L.nx * L.ny;
+  const T *f = L.f[s];
+  const int cnyi = (L.iy1 - L.iy0) >> 1;
+  const int Nc = ((L.ix1 - L.ix0) >> 1) * cnyi;
+  for (int t = threadIdx.x; t < Nc; t += blockDim.x) {
+    const int x0 = L.ix0 + 2 * (t / cnyi), y0 = L.iy0 + 2 * (t % cnyi);
+    T pr[2][8];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      T r0[8], r1[8];
+      tail_pull<T>(L, x0 + a, y0, r0);
+      tail_pull<T>(L, x0 + a, y0 + 1, r1);
+      const int i0 = (x0 + a) * L.ny + y0;
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const T ra = f[d * N + i0] - r0[d];
+        const T rb = f[d * N + i0 + 1] - r1[d];
+        pr[a][d] = ra + rb;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) Lc.resid[d * Nc + t] = pr[0][d] + pr[1][d];
+  }
+  __syncthreads();
+}
+
+template <typename T, bool COUPLED>
+__device__ void tail_fill_zero(const TailLevelDev<T> &L) {
+  const int N = L.nx * L.ny;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < 8; ++d) L.f[0][d * N + i] = T(0);
+    if constexpr (COUPLED) L.div[0][i] = T(0);
+  }
+}
+
+// f(L) += injection of f(Lc) over L's interior box; the carried divergence follows
+template <typename T, bool COUPLED>
+__device__ void tail_prolong(const TailLevelDev<T> &L, const TailLevelDev<T> &Lc, const TailDev<T> &P, int s) {
+  const int N = L.nx * L.ny, Nc = Lc.nx * Lc.ny;
+  const int nyi = L.iy1 - L.iy0;
+  const int n = (L.ix1 - L.ix0) * nyi;
+  T *f = L.f[s];
+  const T *c = Lc.f[0];
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int xi = t / nyi, yi = t % nyi;
+    const int cell = (L.ix0 + xi) * L.ny + (L.iy0 + yi);
+    const int cc = (L.ix0 + (xi >> 1)) * Lc.ny + (L.iy0 + (yi >> 1));
+    T v[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      v[d] = f[d * N + cell] + c[d * Nc + cc];
+      f[d * N + cell] = v[d];
+    }
+    if constexpr (COUPLED) L.div[s][cell] = elast_div_of<T>(v, P.kdiv);
+  }
+  __syncthreads();
+}
+
+// V-cycle of levels 0 .. nlev-1 of the table (level 0 takes its residual as input and
+// starts from zero); the result is level 0's state 0
+template <typename T, bool COUPLED>
+__global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(const TailLevelDev<T> *__restrict__ lv, int nlev, int k,
+                                                              TailDev<T> P) {
+  unsigned cur = 0; // bit l: the state buffer of level l
+  tail_fill_zero<T, COUPLED>(lv[0]);
+  __syncthreads();
+  for (int l = 0; l < nlev; ++l) {
+    const TailLevelDev<T> &L = lv[l];
+    for (int j = 0; j < k; ++j) {
+      tail_smooth<T, COUPLED>(L, P, (cur >> l) & 1, true);
+      cur ^= 1u << l;
+    }
+    if (l + 1 < nlev) {
+      tail_residual<T, COUPLED>(L, lv[l + 1], P, (cur >> l) & 1, true);
+      tail_fill_zero<T, COUPLED>(lv[l + 1]);
+      cur &= ~(1u << (l + 1));
+      __syncthreads();
+    }
+  }
+  for (int l = nlev - 1; l >= 0; --l) {
+    const TailLevelDev<T> &L = lv[l];
+    if (l + 1 < nlev) tail_prolong<T, COUPLED>(L, lv[l + 1], P, (cur >> l) & 1);
+    for (int j = 0; j < k; ++j) {
+      tail_smooth<T, COUPLED>(L, P, (cur >> l) & 1, true);
+      cur ^= 1u << l;
+    }
+  }
+}
+
+template <typename T> TailDev<T> tail_params(const xb_tail_plan *p) {
+  const double lm = p->lam + p->mu;
+  double g = 1.0 / (12.0 * lm - 4.0);
+  if (std::fabs(g) > 100.0) g = 0.0;
+  TailDev<T> P;
+  P.cs = T((3 * lm - 1) / (3 * lm + 1));
+  P.cd = T((6 * p->mu - 1) / (6 * p->mu + 1));
+  P.h = T(0.25 + 0.5 * g);
+  P.n = T(-0.25 * g);
+  P.kdiv = T(-1.5 / (3 * (p->lam + p->mu) + 1));
+  P.gamma = T(p->gamma);
+  P.omg = T(1.0 - p->gamma);
+  P.m3s = T(-3 * p->nl_scale);
+  return P;
+}
+
 } // namespace
 
 // ============================================================================
@@ -2426,6 +2645,97 @@ int xb_darcy_macro_adjoint(int32_t nx, int32_t ny, int32_t dtype, double omega, 
                        (const float *)omega_field, (const float *)f, (const float *)p_bar, (const float *)grad_bar,
                        (float *)fbar, (float *)source_bar, grad_omega);
   return check_launch("xb_darcy_macro_adjoint");
+}
+
+int64_t xb_vcycle_tail_bytes(int32_t nlevels) {
+  if (nlevels < 1 || nlevels > TAIL_MAX_LEVELS) return 0;
+  return (int64_t)sizeof(TailLevelDev<double>) * nlevels;
+}
+
+int xb_vcycle_tail_prepare(int32_t dtype, double lam, double mu, double gamma, int32_t k_smooth, double nl_scale,
+                           int32_t coupled, const xb_tail_level *levels, int32_t nlevels, void *table,
+                           int64_t table_bytes, xb_tail_plan *plan) {
+  if (int r = check_dtype(dtype)) return r;
+  if (levels == nullptr || plan == nullptr || table == nullptr || nlevels < 1 || nlevels > TAIL_MAX_LEVELS ||
+      k_smooth < 1)
+    return fail(XB_EINVAL, "xb_vcycle_tail_prepare: bad arguments");
+  if (table_bytes < xb_vcycle_tail_bytes(nlevels))
+    return fail(XB_EINVAL, "xb_vcycle_tail_prepare: table of %lld bytes, %lld needed", (long long)table_bytes,
+                (long long)xb_vcycle_tail_bytes(nlevels));
+  for (int l = 0; l < nlevels; ++l) {
+    const xb_tail_level &L = levels[l];
+    if (L.nx <= 0 || L.ny <= 0 || L.f0 == nullptr || L.f1 == nullptr || L.resid == nullptr || L.scratch == nullptr)
+      return fail(XB_EINVAL, "xb_vcycle_tail_prepare: level %d: bad arguments", l);
+    if (coupled && (L.div0 == nullptr || L.div1 == nullptr))
+      return fail(XB_EINVAL, "xb_vcycle_tail_prepare: level %d: the coupling force needs div0 and div1", l);
+    if (L.ix0 < 0 || L.ix1 > L.nx || L.iy0 < 0 || L.iy1 > L.ny || L.ix0 >= L.ix1 || L.iy0 >= L.iy1)
+      return fail(XB_EINVAL, "xb_vcycle_tail_prepare: level %d: bad interior box", l);
+    if (int r = check_bc(L.bc, false)) return r;
+    if (int r = check_bc(L.grad_bc, true)) return r;
+    if (l + 1 < nlevels) {
+      const xb_tail_level &C = levels[l + 1];
+      if (((L.ix1 - L.ix0) & 1) || ((L.iy1 - L.iy0) & 1) || C.ix1 - C.ix0 != (L.ix1 - L.ix0) / 2 ||
+          C.iy1 - C.iy0 != (L.iy1 - L.iy0) / 2 || C.ix0 != L.ix0 || C.iy0 != L.iy0 ||
+          C.nx != L.ix0 + (L.ix1 - L.ix0) / 2 + (L.nx - L.ix1) || C.ny != L.iy0 + (L.iy1 - L.iy0) / 2 + (L.ny - L.iy1))
+        return fail(XB_EINVAL, "xb_vcycle_tail_prepare: level %d does not coarsen level %d", l + 1, l);
+    }
+  }
+  auto fill = [&](auto tag) {
+    using T = decltype(tag);
+    std::vector<TailLevelDev<T>> v((size_t)nlevels);
+    for (int l = 0; l < nlevels; ++l) {
+      const xb_tail_level &L = levels[l];
+      TailLevelDev<T> &D = v[(size_t)l];
+      D.nx = L.nx; D.ny = L.ny; D.ix0 = L.ix0; D.ix1 = L.ix1; D.iy0 = L.iy0; D.iy1 = L.iy1;
+      D.bc = bcdev<T>(L.bc);
+      D.gbc = bcdev<T>(L.grad_bc);
+      D.f[0] = static_cast<T *>(L.f0);
+      D.f[1] = static_cast<T *>(L.f1);
+      D.div[0] = static_cast<T *>(L.div0);
+      D.div[1] = static_cast<T *>(L.div1);
+      D.resid = static_cast<T *>(L.resid);
+      D.pc = static_cast<T *>(L.scratch);
+    }
+    return hipMemcpy(table, v.data(), sizeof(TailLevelDev<T>) * v.size(), hipMemcpyHostToDevice);
+  };
+  const hipError_t e = dtype == XB_F64 ? fill(double(0)) : fill(float(0));
+  if (e != hipSuccess) return fail(XB_EHIP, "xb_vcycle_tail_prepare: %s", hipGetErrorString(e));
+  plan->dtype = dtype;
+  plan->nlevels = nlevels;
+  plan->k_smooth = k_smooth;
+  plan->coupled = coupled != 0;
+  plan->lam = lam;
+  plan->mu = mu;
+  plan->gamma = gamma;
+  plan->nl_scale = nl_scale;
+  plan->table = table;
+  return XB_OK;
+}
+
+int xb_vcycle_tail(const xb_tail_plan *plan, void *stream) {
+  if (plan == nullptr || plan->table == nullptr || plan->nlevels < 1 || plan->nlevels > TAIL_MAX_LEVELS)
+    return fail(XB_EINVAL, "xb_vcycle_tail: needs a prepared plan");
+  if (int r = check_dtype(plan->dtype)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 gr(1), bl(TAIL_THREADS);
+  if (plan->dtype == XB_F64) {
+    const auto *t = static_cast<const TailLevelDev<double> *>(plan->table);
+    if (plan->coupled)
+      hipLaunchKernelGGL((k_vcycle_tail<double, true>), gr, bl, 0, st, t, plan->nlevels, plan->k_smooth,
+                         tail_params<double>(plan));
+    else
+      hipLaunchKernelGGL((k_vcycle_tail<double, false>), gr, bl, 0, st, t, plan->nlevels, plan->k_smooth,
+                         tail_params<double>(plan));
+  } else {
+    const auto *t = static_cast<const TailLevelDev<float> *>(plan->table);
+    if (plan->coupled)
+      hipLaunchKernelGGL((k_vcycle_tail<float, true>), gr, bl, 0, st, t, plan->nlevels, plan->k_smooth,
+                         tail_params<float>(plan));
+    else
+      hipLaunchKernelGGL((k_vcycle_tail<float, false>), gr, bl, 0, st, t, plan->nlevels, plan->k_smooth,
+                         tail_params<float>(plan));
+  }
+  return check_launch("xb_vcycle_tail");
 }
 
 } // extern "C"

@@ -62,6 +62,19 @@ class DarcyAdjointArgs(ctypes.Structure):
 XB_ADJ_SMOOTH, XB_ADJ_RESIDUAL = 1, 2
 
 
+class TailLevel(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("ix0", c_int32), ("ix1", c_int32), ("iy0", c_int32),
+                ("iy1", c_int32), ("bc", ctypes.POINTER(BCTable)), ("grad_bc", ctypes.POINTER(BCTable)),
+                ("f0", c_void_p), ("f1", c_void_p), ("div0", c_void_p), ("div1", c_void_p), ("resid", c_void_p),
+                ("scratch", c_void_p)]
+
+
+class TailPlan(ctypes.Structure):
+    _fields_ = [("dtype", c_int32), ("nlevels", c_int32), ("k_smooth", c_int32), ("coupled", c_int32),
+                ("lam", c_double), ("mu", c_double), ("gamma", c_double), ("nl_scale", c_double),
+                ("table", c_void_p)]
+
+
 class ElastMGAdjointArgs(ctypes.Structure):
     _fields_ = [("nx", c_int32), ("ny", c_int32), ("dtype", c_int32), ("mode", c_int32),
                 ("lam", c_double), ("mu", c_double), ("gamma", c_double), ("lam_field", c_void_p),
@@ -121,6 +134,11 @@ SIGNATURES = {
     "xb_elast_adjoint": (c_int32, [ctypes.POINTER(ElastAdjointArgs), c_void_p]),
     "xb_darcy_adjoint": (c_int32, [ctypes.POINTER(DarcyAdjointArgs), c_void_p]),
     "xb_elast_mg_adjoint": (c_int32, [ctypes.POINTER(ElastMGAdjointArgs), c_void_p]),
+    "xb_vcycle_tail_bytes": (c_int64, [c_int32]),
+    "xb_vcycle_tail_prepare": (c_int32, [c_int32, c_double, c_double, c_double, c_int32, c_double, c_int32,
+                                         ctypes.POINTER(TailLevel), c_int32, c_void_p, c_int64,
+                                         ctypes.POINTER(TailPlan)]),
+    "xb_vcycle_tail": (c_int32, [ctypes.POINTER(TailPlan), c_void_p]),
     "xb_elast_coupling_adjoint": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
                                             c_void_p, c_double, ctypes.POINTER(BCTable), c_void_p, c_void_p,
                                             c_void_p, c_void_p, c_void_p, c_void_p]),

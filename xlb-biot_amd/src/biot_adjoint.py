@@ -111,11 +111,8 @@ class BiotAdjointSession:
             states.append(f)
             f, div = el._smooth_nl(f, div, 0, None, base, nlf)
         states.append(f)
-        res_c, _, _ = el._residual_nl(f, div, 0, None, base, nlf)
+        f_c, _ = el._coarse_solve_nl(f, div, 0, None, base, nlf)
         nxc, nyc = el.grid_size(1)
-        f_c = nat.fill(torch.empty((8, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
-        d_c = nat.fill(torch.empty((1, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
-        f_c, d_c = el._vcycle_nl(f_c, d_c, 1, res_c, None, nlf)
         f, div = f.clone(), div.clone()
         nx, ny = el.grid_size(0)
         x0, x1, y0, y1 = el.interior_box(0)

@@ -7,6 +7,7 @@ validation errors, hooks (``set_boundary_conditions``, ``get_force``,
 signatures and return values.  Every step runs on the GPU through the C ABI of
 ``include/xlb_biot.h``; arrays are torch tensors shaped like the reference's.
 """
+import ctypes
 import time
 
 import numpy as np
@@ -442,11 +443,12 @@ class LBMElasticity(object):
         nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step(smooth)")
         return out
 
-    def _residual(self, f, level, resid, ext):
+    def _residual(self, f, level, resid, ext, coarse=None):
         """Restricted residual (coarse interior, SoA) and the workspace holding its
         squared-norm partials."""
         x0, x1, y0, y1 = self.interior_box(level)
-        coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=f.dtype, device=f.device)
+        if coarse is None:
+            coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=f.dtype, device=f.device)
         lib = nat.lib()
         a = self._launch(f, None, level, nat.XB_MODE_RESIDUAL, ext=ext, resid=resid, coarse=coarse)
         n = int(lib.xb_elast_partials_count(a))
@@ -536,6 +538,27 @@ class LBMElasticity(object):
         return self._mg_step(f_poststreaming, timestep, external_force=external_force,
                              nonlinear_force=nonlinear_force)
 
+    # ---- coarse levels run by the single-workgroup tail kernel
+    def _tail_start(self):
+        """First multigrid level run by the single-workgroup tail (None: no tail)."""
+        if not getattr(self, "use_tail", True):
+            return None
+        for lvl in range(1, self.levels):
+            nx, ny = self.grid_size(lvl)
+            if nx * ny <= TAIL_CELLS:
+                return lvl
+        return None
+
+    def _tail(self, level, nlf):
+        """The tail that solves level ``level`` and below, if it starts there."""
+        if self._tail_start() != level:
+            return None
+        key = (level, None if nlf is None else float(nlf.scale))
+        tails = self.__dict__.setdefault("_tails", {})
+        if key not in tails:
+            tails[key] = VCycleTail(self, level, nlf)
+        return tails[key]
+
     # ---- fused Biot coupling force: the sweep forms base + grad(div f) itself from a
     # carried divergence field (every sweep writes div of its output)
     @staticmethod
@@ -566,9 +589,10 @@ class LBMElasticity(object):
         nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step(coupled smooth)")
         return out, out_div
 
-    def _residual_nl(self, f, div, level, resid, base, nlf):
+    def _residual_nl(self, f, div, level, resid, base, nlf, coarse=None):
         x0, x1, y0, y1 = self.interior_box(level)
-        coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=f.dtype, device=f.device)
+        if coarse is None:
+            coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=f.dtype, device=f.device)
         lib = nat.lib()
         a = self._nl_args(self._launch(f, None, level, nat.XB_MODE_RESIDUAL, ext=base, resid=resid,
                                        coarse=coarse), div, None, nlf, level)
@@ -580,6 +604,22 @@ class LBMElasticity(object):
         nat.check(lib.xb_elast_step(a, nat.stream()), "xb_elast_step(coupled residual)")
         return coarse, ws, n
 
+    def _coarse_solve_nl(self, f, div, level, residual, base, nlf):
+        """Residual of ``f`` restricted to level ``level + 1`` and the coarse V-cycle
+        from zero on it (the tail kernel where it starts there): ``(f_c, norm)``,
+        norm (level 0 only) the residual's norm as a device scalar."""
+        tail = self._tail(level + 1, nlf)
+        res_c, ws, n = self._residual_nl(f, div, level, residual, base, nlf,
+                                         coarse=tail.resid if tail is not None else None)
+        norm = ws.finalize(n)[0] if level == 0 else None
+        if tail is not None:
+            return tail.launch(), norm
+        nxc, nyc = self.grid_size(level + 1)
+        f_c = nat.fill(torch.empty((8, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
+        d_c = nat.fill(torch.empty((1, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
+        f_c, _ = self._vcycle_nl(f_c, d_c, level + 1, res_c, None, nlf)
+        return f_c, norm
+
     def _vcycle_nl(self, f, div, level, residual, base, nlf, out=None, out_div=None):
         """``_vcycle`` with the Biot coupling force fused into the sweeps; ``div`` is
         the divergence of ``f`` and is carried along.  Returns ``(f, div[, norm])``."""
@@ -587,13 +627,8 @@ class LBMElasticity(object):
             f, div = self._smooth_nl(f, div, level, residual, base, nlf)
         norm = None
         if level < self.levels - 1:
-            res_c, ws, n = self._residual_nl(f, div, level, residual, base, nlf)
-            if level == 0:
-                norm = ws.finalize(n)[0]
+            f_c, norm = self._coarse_solve_nl(f, div, level, residual, base, nlf)
             nxc, nyc = self.grid_size(level + 1)
-            f_c = nat.fill(torch.empty((8, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
-            d_c = nat.fill(torch.empty((1, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
-            f_c, d_c = self._vcycle_nl(f_c, d_c, level + 1, res_c, None, nlf)
             nx, ny = self.grid_size(level)
             x0, x1, y0, y1 = self.interior_box(level)
             nat.check(nat.lib().xb_elast_prolong_add_div(nx, ny, nat.dtype_code(f.dtype), self.lamda, self.mu,
@@ -649,12 +684,16 @@ class LBMElasticity(object):
         norm = None
         if level < self.levels - 1:
             ext = self._ext_soa(external_force, nonlinear_force, f, level)
-            res_c, ws, n = self._residual(f, level, residual, ext)
+            tail = self._tail(level + 1, None) if nonlinear_force is None else None
+            res_c, ws, n = self._residual(f, level, residual, ext, coarse=tail.resid if tail is not None else None)
             if level == 0:
                 norm = ws.finalize(n)[0]
-            nxc, nyc = self.grid_size(level + 1)
-            f_c = nat.fill(torch.empty((8, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
-            f_c = self._vcycle(f_c, level + 1, res_c, None, nonlinear_force)
+            if tail is not None:
+                f_c = tail.launch()
+            else:
+                nxc, nyc = self.grid_size(level + 1)
+                f_c = nat.fill(torch.empty((8, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
+                f_c = self._vcycle(f_c, level + 1, res_c, None, nonlinear_force)
             self._prolong_add(f, f_c, level)
         for k in range(self.k_smooth):
             ext = self._ext_soa(external_force, nonlinear_force, f, level)
@@ -737,6 +776,60 @@ class LBMElasticity(object):
     def handle_io_timestep(self, timestep, f, fstar, eta, sigma, eta_prev, sigma_prev):
         self.output_data(timestep=timestep, eta=eta, eta_prev=eta_prev, sigma=sigma, sigma_prev=sigma_prev,
                          f_poststreaming=f, f_postcollision=fstar)
+
+
+# Multigrid levels of at most this many cells run as one single-workgroup kernel
+# (xb_vcycle_tail): their sweeps are bound by launch and load latency.  Measured on
+# MI355X (V-cycle time, threshold 0 / 512 / 2048 / 8192 cells): 1024 x 1026 with 10
+# levels FP32 3.10 / 2.59 / 2.72 / 3.61 ms, 256^2 with 8 levels FP64 298 / 238 / 243
+# / 371 us; above about a thousand cells one workgroup is slower than a launch.
+TAIL_CELLS = 512
+
+
+class VCycleTail:
+    """The levels ``start`` .. of the V-cycles of ``el`` (below the level that calls
+    them) as one single-workgroup launch from the zero state (``xb_vcycle_tail``):
+    bit-identical to the level-by-level launches.  The restricted residual of level
+    ``start - 1`` is written into :attr:`resid`; the result is :attr:`result`."""
+
+    def __init__(self, el, start, nlf):
+        lib = nat.lib()
+        coupled = nlf is not None
+        self.levels = list(range(start, el.levels))
+        n = len(self.levels)
+        dt, dev = el._tdt, el.device
+        arr = (nat.TailLevel * n)()
+        self._keep = []
+        for i, lvl in enumerate(self.levels):
+            nx, ny = el.grid_size(lvl)
+            x0, x1, y0, y1 = el.interior_box(lvl)
+            b = {"f0": torch.zeros((8, nx, ny), dtype=dt, device=dev),
+                 "f1": torch.zeros((8, nx, ny), dtype=dt, device=dev),
+                 "resid": torch.zeros((8, x1 - x0, y1 - y0), dtype=dt, device=dev),
+                 "scratch": torch.zeros((8, nx, ny), dtype=dt, device=dev)}
+            if coupled:
+                b["div0"] = torch.zeros((1, nx, ny), dtype=dt, device=dev)
+                b["div1"] = torch.zeros((1, nx, ny), dtype=dt, device=dev)
+            table = el._tables.get(lvl)
+            gt = nlf.darcy._grad_table(nx, ny) if coupled else None
+            arr[i] = nat.TailLevel(nx, ny, x0, x1, y0, y1, table.ref() if table is not None else None,
+                                   gt.ref() if gt is not None else None, nat.ptr(b["f0"]), nat.ptr(b["f1"]),
+                                   nat.ptr(b.get("div0")), nat.ptr(b.get("div1")), nat.ptr(b["resid"]),
+                                   nat.ptr(b["scratch"]))
+            self._keep.append(b)
+        nbytes = int(lib.xb_vcycle_tail_bytes(n))
+        self.table = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+        self.plan = nat.TailPlan()
+        scale = float(nlf.scale) if coupled else 0.0
+        nat.check(lib.xb_vcycle_tail_prepare(nat.dtype_code(dt), el.lamda, el.mu, el.gamma, el.k_smooth, scale,
+                                             int(coupled), arr, n, nat.ptr(self.table), nbytes,
+                                             ctypes.byref(self.plan)), "xb_vcycle_tail_prepare")
+        self.resid = self._keep[0]["resid"]
+        self.result = self._keep[0]["f0"]
+
+    def launch(self):
+        nat.check(nat.lib().xb_vcycle_tail(ctypes.byref(self.plan), nat.stream()), "xb_vcycle_tail")
+        return self.result
 
 
 class VCycleGraph:
