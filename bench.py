@@ -78,12 +78,15 @@ def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True):
     f, wall, gpu = timed_steps(torch, lambda f, k: d.run_steps(f, k, chunk), f, steps)
     S = 8 if precision == "f64/f64" else 4
     cells = d.nx * d.ny
-    # the timed graphs hold only sweep launches of two steps each (chunk % 4 == 0):
-    # GPU time / (steps / 2) = average launch duration; a launch reads and writes
-    # every population once (compulsory 19 values per cell)
-    spl = 2 if chunk % 4 == 0 else 1
+    # the timed graphs hold only launches of the tuned kind (m steps each: one or two
+    # per sweep, or m on LDS tiles): GPU time / (steps / m) = average launch
+    # duration; a launch reads and writes every population once (compulsory 19
+    # values per cell)
+    from src import autotune
+    choice, spl, _ = d._multi()
+    kind = "tile" if choice > autotune.TILE_BASE else ("sweep2" if choice == autotune.SWEEP2 else "sweep")
     return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 19 * S,
-                steps_per_launch=spl, solver=d)
+                steps_per_launch=spl, launch_kind=kind, solver=d)
 
 
 def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=32):
@@ -96,7 +99,7 @@ def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=3
     f, wall, gpu = timed_steps(torch, lambda f, k: el.run_steps(f, k, chunk), f, steps)
     S = 8 if precision == "f64/f64" else 4
     cells = n * n
-    spl = 2 if chunk % 4 == 0 else 1
+    _, spl, _ = el._multi(None)
     return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 18 * S,
                 steps_per_launch=spl)
 
@@ -578,10 +581,8 @@ def only(torch, args):
         from src import autotune
         r = bench_darcy(torch, args.n, "f64/f64", args.steps, 2, chunk=16)
         d = r["solver"]
-        rows = autotune.lookup(d._tune_key(True))
-        items = -(-d.ny // 60) * -(-d.nx // rows) if rows else None
-        log(f"c2 {d.nx}x{d.ny}: two-step rows {rows}, grid {-(-items // 4) if items else '?'} blocks of 256, "
-            f"{r['kernel_s'] * 1e6:.2f} us/launch")
+        log(f"c2 {d.nx}x{d.ny}: {r['launch_kind']} launches of {r['steps_per_launch']} steps, "
+            f"{r['kernel_s'] * 1e6:.2f} us/launch, {r['kernel_s'] * 1e6 / r['steps_per_launch']:.2f} us/step")
     elif args.only == "biot_vcycle":
         r = bench_biot_vcycle(torch, reps=args.steps)
         log(f"biot vcycle: {r}")
@@ -632,13 +633,19 @@ def main():
     achieved = r["bytes"] / r["kernel_s"] / 1e9
     from src import autotune
     d = r["solver"]
-    two = r["steps_per_launch"] == 2
-    rows = autotune.lookup(d._tune_key(two))
-    strip = 60 if two else 62
-    rows_eff = rows if rows else None
-    blocks = (-(-(-(-d.ny // strip) * -(-d.nx // rows_eff)) // 4)) if rows_eff else None
-    kname = ("k_sweep2<double, xb::DarcyColl<double>, 2>" if two
-             else "k_sweep<double, xb::DarcyColl<double>, 0, 2, false, false>")
+    kind, m = r["launch_kind"], r["steps_per_launch"]
+    if kind == "tile":
+        rows = None
+        inner = 32 - 2 * m
+        blocks = -(-d.nx // inner) * -(-d.ny // inner)
+        kname = "k_tile_steps<double, xb::DarcyColl<double>, 2>"
+    else:
+        two = kind == "sweep2"
+        rows = autotune.lookup(d._tune_key(two))
+        strip = 60 if two else 62
+        blocks = (-(-(-(-d.ny // strip) * -(-d.nx // rows)) // 4)) if rows else None
+        kname = ("k_sweep2<double, xb::DarcyColl<double>, 2>" if two
+                 else "k_sweep<double, xb::DarcyColl<double>, 0, 2, false, false>")
     traffic = profiled_traffic(kname, blocks, r["cells"]) if blocks else None
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
@@ -653,7 +660,7 @@ def main():
                      "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and launch grid, "
                                         "calibrated on a copy (profiles/darcy_c2_514_fp64_hbm_traffic.md)")
                      if traffic else "not profiled for this launch grid",
-                     "kernel": kname, "launch_blocks": blocks, "rows_per_wavefront": rows,
+                     "kernel": kname, "launch_kind": kind, "launch_blocks": blocks, "rows_per_wavefront": rows,
                      "kernel_us": round(r["kernel_s"] * 1e6, 3), "steps_per_launch": r["steps_per_launch"],
                      "bytes_per_cell_per_launch": 152, "cells": r["cells"],
                      "per_step_equivalent_gbs": round(achieved * r["steps_per_launch"], 1)},

@@ -138,6 +138,11 @@ int xb_elast_step(const xb_elast_args *args, void *stream);
  * xb_elast_step calls).  STEP mode only: no fpc_out, resid or div_in; grid >= 2 x 2. */
 int xb_elast_step2(const xb_elast_args *args, void *stream);
 
+/* `steps` (1..8) consecutive plain steps in one launch on 32 x 32 LDS tiles
+ * (temporal blocking; bit-identical to `steps` XB_MODE_STEP calls).  The same
+ * restrictions as xb_elast_step2. */
+int xb_elast_stepn(const xb_elast_args *args, int32_t steps, void *stream);
+
 /* Number of doubles in norm_partials for a RESIDUAL call with these args. */
 int64_t xb_elast_partials_count(const xb_elast_args *args);
 
@@ -344,6 +349,10 @@ int xb_darcy_step(const xb_darcy_args *args, void *stream);
 
 /* Two Darcy steps in one pass (as xb_elast_step2; no fpc_out). */
 int xb_darcy_step2(const xb_darcy_args *args, void *stream);
+
+/* `steps` (1..8) consecutive Darcy steps in one launch on 32 x 32 LDS tiles
+ * (temporal blocking; bit-identical to `steps` xb_darcy_step calls; no fpc_out). */
+int xb_darcy_stepn(const xb_darcy_args *args, int32_t steps, void *stream);
 
 /* p = sum_d f_d + (s_a + s_b)/2 ; grad = sum_d f_d (-3 omega) c_d.
  * Replaces update_macroscopic (src/darcy.py:36-73). Either output may be null. */

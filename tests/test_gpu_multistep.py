@@ -1,0 +1,70 @@
+"""Several plain steps per launch on LDS tiles (xb_darcy_stepn / xb_elast_stepn)
+against the same number of single-step launches: bit-identical populations, with
+walls, per-cell parameters, forces and sources, on grids that are and are not
+multiples of the tile, FP64 and FP32."""
+import numpy as np
+import pytest
+import torch
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _single(step, f, n):
+    a = f.clone()
+    b = torch.empty_like(a)
+    for _ in range(n):
+        step(a, b)
+        a, b = b, a
+    return a
+
+
+@pytest.mark.parametrize("n,precision,hetero", [(514, "f64/f64", True), (70, "f32/f32", True), (16, "f64/f64", False),
+                                               (5, "f64/f64", False)])
+@pytest.mark.parametrize("m", [1, 3, 4, 8])
+def test_darcy_stepn_matches_single_steps(n, precision, hetero, m):
+    say("darcy", n, precision, "hetero", hetero, "steps per launch", m)
+    d = P.darcy_c2("gpu", n=n - 2, precision=precision, heterogeneous=hetero, seed=0, variance=1.0, corr=4.0)
+    g = torch.Generator().manual_seed(1)
+    f = (1.0 + 0.1 * torch.randn((9, d.nx, d.ny), generator=g, dtype=torch.float64)).to(d._tdt).cuda()
+    src = (1e-3 * torch.randn((1, d.nx, d.ny), generator=g, dtype=torch.float64)).to(d._tdt).cuda()
+    ref = _single(lambda a, b: d._step_soa(a, src, False, out=b), f, m)
+    got = d._stepn_into(f, torch.empty_like(f), m, src)
+    assert torch.equal(got, ref), (got - ref).abs().max().item()
+
+
+@pytest.mark.parametrize("case", ["periodic", "walls", "fields"])
+@pytest.mark.parametrize("m", [2, 5, 8])
+def test_elast_stepn_matches_single_steps(case, m):
+    say("elasticity", case, "steps per launch", m)
+    g = torch.Generator().manual_seed(2)
+    if case == "periodic":
+        el = P.periodic_elasticity("gpu", nx=64)
+    elif case == "walls":
+        el = P.consolidation_elasticity("gpu", nx=40)
+    else:
+        rng = np.random.default_rng(3)
+        el = P.consolidation_elasticity("gpu", nx=33, lambda_field=rng.uniform(1.0, 3.0, (33, 35)),
+                                        mu_field=rng.uniform(0.5, 1.5, (33, 35)))
+    f = (1e-3 * torch.randn((8, el.nx, el.ny), generator=g, dtype=torch.float64)).to(el._tdt).cuda()
+    ext = (1e-4 * torch.randn((2, el.nx, el.ny), generator=g, dtype=torch.float64)).to(el._tdt).cuda()
+    ref = _single(lambda a, b: el._step_into(a, b, ext), f, m)
+    got = el._stepn_into(f, torch.empty_like(f), m, ext)
+    assert torch.equal(got, ref), (got - ref).abs().max().item()
+
+
+def test_run_steps_with_tuned_multistep_matches_single_steps():
+    say("darcy run_steps after tuning the steps per launch")
+    d = P.darcy_c2("gpu", n=126, precision="f64/f64", heterogeneous=True, seed=0, variance=1.0, corr=4.0)
+    d.autotune()
+    choice, m, _ = d._multi()
+    say("chosen launch kind", choice, "steps per launch", m)
+    f0 = d._soa(d.assign_fields_sharded(), 9)
+    ref = _single(lambda a, b: d._step_soa(a, None, False, out=b), f0, 77)
+    got = d._soa(d.run_steps(f0.permute(1, 2, 0), 77, chunk=32), 9)
+    assert torch.equal(got, ref)

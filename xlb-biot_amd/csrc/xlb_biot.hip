@@ -833,6 +833,174 @@ template <typename T> BCDev<T> bcdev(const xb_bc_table *t) {
   return b;
 }
 
+// ---------------------------------------------------------------------------
+// Several plain steps per launch on LDS tiles (temporal blocking).  A workgroup owns
+// a TILE x TILE region of cells (periodic wrap at the grid edges), keeps the
+// populations of its cells in registers and runs `steps` steps on them: every thread
+// collides its cells and applies their post-collision rules, writes the
+// post-collision populations to LDS and, after a barrier, pulls its cells' streamed
+// populations from the neighbours' LDS slots and applies the post-streaming rules.
+// A cell on the region's edge pulls from itself where its neighbour lies outside the
+// region; that error moves inward one cell per step, so after `steps` steps the inner
+// (TILE - 2 steps)^2 cells are exact, and those are written.  Each population is read
+// and written once per launch; per cell the arithmetic is that of `steps` k_sweep
+// STEP launches (bit-identical).  A grid resident in the 256 MB last-level cache
+// (the 514^2 Darcy grid is 19 MB) is bound by the latency of its launches, not by
+// HBM: fewer, larger launches are the speed-up.
+constexpr int TILE = 32;
+constexpr int TILE_THREADS = 256;
+constexpr int TILE_CPT = TILE * TILE / TILE_THREADS;
+constexpr int TILE_MAX_STEPS = 8;
+
+__device__ __forceinline__ int wrapm(int i, int n) {
+  const int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// Rules of one cell without branches (the tile kernel unrolls its cells, and the
+// branchy form of apply_rules would not fit the instruction cache): the same values
+// as apply_rules, with the rule sources read from the cell's LDS slots (source
+// population d at own[d * TT]) instead of selected among registers.
+template <typename T, int Q, int TT>
+__device__ __forceinline__ void rules_lds(T (&v)[Q], const T *own, const int8_t *code, const T *cst) {
+  T out[Q];
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    const int c = code[d];
+    const int k = min(max((c > 0 ? c : -c) - 1, 0), Q - 1);
+    const T s = own[k * TT];
+    T base = c > 0 ? s : -s;
+    base = c == XB_RULE_ZERO ? T(0) : base;
+    base = c == 0 ? v[d] : base;
+    out[d] = base + cst[d];
+  }
+#pragma unroll
+  for (int d = 0; d < Q; ++d) v[d] = out[d];
+}
+
+template <typename T, class Coll, int BC>
+__device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int tiles_y, const Coll &coll,
+                                                const T *__restrict__ fin, T *__restrict__ fout, const BCDev<T> &bc,
+                                                T *lds, BCShared<T, Coll::Q> *sh) {
+  constexpr int Q = Coll::Q;
+  constexpr bool HAS_BC = BC != 0;
+  constexpr int TT = TILE * TILE;
+  using Aux = typename Coll::Aux;
+  const int inner = TILE - 2 * steps;
+  const int bx = blockIdx.x / tiles_y, by = blockIdx.x % tiles_y;
+  const int x0 = bx * inner - steps, y0 = by * inner - steps;
+  const size_t N = (size_t)nx * ny;
+  if constexpr (BC == 2) {
+    // the (small, deduplicated) rule table into LDS by the whole block
+    const int n = bc.nslots * Q;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      sh->code_pc[k] = bc.code_pc != nullptr ? bc.code_pc[k] : int8_t(0);
+      sh->const_pc[k] = bc.const_pc != nullptr ? bc.const_pc[k] : T(0);
+      sh->code_ps[k] = bc.code_ps != nullptr ? bc.code_ps[k] : int8_t(0);
+      sh->const_ps[k] = bc.const_ps != nullptr ? bc.const_ps[k] : T(0);
+    }
+  }
+  T f[TILE_CPT][Q];
+  Aux aux[TILE_CPT];
+  int slot[TILE_CPT];
+#pragma unroll
+  for (int k = 0; k < TILE_CPT; ++k) {
+    const int c = threadIdx.x + k * TILE_THREADS;
+    const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
+    const size_t cell = (size_t)gx * ny + gy;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) f[k][d] = fin[d * N + cell];
+    aux[k] = coll.load_aux(cell);
+    slot[k] = HAS_BC ? bc.slot[cell] : -1;
+  }
+  if constexpr (BC == 2) __syncthreads();
+  for (int s = 0; s < steps; ++s) {
+#pragma unroll
+    for (int k = 0; k < TILE_CPT; ++k) {
+      const int c = threadIdx.x + k * TILE_THREADS;
+      T pc[Q];
+      coll.collide(f[k], aux[k], pc);
+      if constexpr (HAS_BC) {
+        const int sl = slot[k];
+        if (sl >= 0 && bc.code_pc != nullptr) {
+          // the rules' sources, the cell's pre-collision populations, go through its own
+          // LDS slots (only this thread touches them before the barrier)
+#pragma unroll
+          for (int d = 0; d < Q; ++d) lds[d * TT + c] = f[k][d];
+          if constexpr (BC == 2) rules_lds<T, Q, TT>(pc, lds + c, sh->code_pc + sl * Q, sh->const_pc + sl * Q);
+          else rules_lds<T, Q, TT>(pc, lds + c, bc.code_pc + (size_t)sl * Q, bc.const_pc + (size_t)sl * Q);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < Q; ++d) lds[d * TT + c] = pc[d];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TILE_CPT; ++k) {
+      const int c = threadIdx.x + k * TILE_THREADS;
+      const int rx = c / TILE, ry = c % TILE;
+      T v[Q];
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        const int sx = rx - Coll::cx(d), sy = ry - Coll::cy(d);
+        const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
+        v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
+      }
+      if constexpr (HAS_BC) {
+        const int sl = slot[k];
+        if (sl >= 0 && bc.code_ps != nullptr) {
+          if constexpr (BC == 2) rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + sl * Q, sh->const_ps + sl * Q);
+          else rules_lds<T, Q, TT>(v, lds + c, bc.code_ps + (size_t)sl * Q, bc.const_ps + (size_t)sl * Q);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < Q; ++d) f[k][d] = v[d];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < TILE_CPT; ++k) {
+    const int c = threadIdx.x + k * TILE_THREADS;
+    const int rx = c / TILE, ry = c % TILE;
+    if (rx < steps || rx >= TILE - steps || ry < steps || ry >= TILE - steps) continue;
+    const int gx = x0 + rx, gy = y0 + ry;
+    if (gx >= nx || gy >= ny) continue;
+    const size_t cell = (size_t)gx * ny + gy;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) fout[d * N + cell] = f[k][d];
+  }
+}
+
+template <typename T, class Coll, int BC>
+__global__ void __launch_bounds__(TILE_THREADS)
+k_tile_steps(int nx, int ny, int steps, int tiles_y, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
+             const BCDev<T> bc) {
+  __shared__ T lds[Coll::Q * TILE * TILE];
+  if constexpr (BC == 2) {
+    __shared__ BCShared<T, Coll::Q> sh;
+    tile_steps_body<T, Coll, BC>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, &sh);
+  } else {
+    tile_steps_body<T, Coll, BC>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, nullptr);
+  }
+}
+
+template <typename T, class Coll>
+int launch_tile_steps(int nx, int ny, int steps, const Coll &coll, const void *fin, void *fout,
+                      const xb_bc_table *bct, hipStream_t st) {
+  const BCDev<T> bc = bcdev<T>(bct);
+  const int inner = TILE - 2 * steps;
+  const int tx = cdiv(nx, inner), ty = cdiv(ny, inner);
+  const dim3 grid((unsigned)(tx * ty)), block(TILE_THREADS);
+  const T *fi = static_cast<const T *>(fin);
+  T *fo = static_cast<T *>(fout);
+  switch (bc_mode(bc.slot, bc.nslots)) {
+  case 2: hipLaunchKernelGGL((k_tile_steps<T, Coll, 2>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
+  case 1: hipLaunchKernelGGL((k_tile_steps<T, Coll, 1>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
+  default: hipLaunchKernelGGL((k_tile_steps<T, Coll, 0>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
+  }
+  return check_launch("multi-step tile kernel");
+}
+
 int check_dtype(int32_t dtype) {
   if (dtype != XB_F64 && dtype != XB_F32) return fail(XB_EUNSUPPORTED, "unsupported dtype %d", dtype);
   return XB_OK;
@@ -2736,6 +2904,54 @@ int xb_vcycle_tail(const xb_tail_plan *plan, void *stream) {
                          tail_params<float>(plan));
   }
   return check_launch("xb_vcycle_tail");
+}
+
+int xb_darcy_stepn(const xb_darcy_args *a, int32_t steps, void *stream) {
+  if (a == nullptr) return fail(XB_EINVAL, "xb_darcy_stepn: null args");
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->nx < 1 || a->ny < 1 || a->f_in == nullptr || a->f_out == nullptr)
+    return fail(XB_EINVAL, "xb_darcy_stepn: bad arguments");
+  if (steps < 1 || steps > TILE_MAX_STEPS) return fail(XB_EINVAL, "xb_darcy_stepn: steps must be in [1, %d]", TILE_MAX_STEPS);
+  if (a->f_in == a->f_out) return fail(XB_EINVAL, "xb_darcy_stepn: f_in and f_out must differ");
+  if (a->fpc_out != nullptr) return fail(XB_EUNSUPPORTED, "xb_darcy_stepn: no post-collision output");
+  if (int r = check_bc(a->bc, false)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (a->dtype == XB_F64) {
+    DarcyColl<double> c;
+    c.omega = a->omega;
+    c.sfac = 1 - 0.5 * a->omega;
+    c.om_field = static_cast<const double *>(a->omega_field);
+    c.sa = static_cast<const double *>(a->source_a);
+    c.sb = static_cast<const double *>(a->source_b);
+    return launch_tile_steps<double>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, st);
+  }
+  DarcyColl<float> c;
+  c.omega = (float)a->omega;
+  c.sfac = (float)(1 - 0.5 * a->omega);
+  c.om_field = static_cast<const float *>(a->omega_field);
+  c.sa = static_cast<const float *>(a->source_a);
+  c.sb = static_cast<const float *>(a->source_b);
+  return launch_tile_steps<float>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, st);
+}
+
+int xb_elast_stepn(const xb_elast_args *a, int32_t steps, void *stream) {
+  if (int r = validate_elast(a, "xb_elast_stepn")) return r;
+  if (a->mode != XB_MODE_STEP || a->fpc_out != nullptr || a->resid != nullptr || a->div_in != nullptr)
+    return fail(XB_EUNSUPPORTED, "xb_elast_stepn: plain steps only (no post-collision output, residual or "
+                "coupling force)");
+  if (steps < 1 || steps > TILE_MAX_STEPS) return fail(XB_EINVAL, "xb_elast_stepn: steps must be in [1, %d]", TILE_MAX_STEPS);
+  if (a->f_in == a->f_out) return fail(XB_EINVAL, "xb_elast_stepn: f_in and f_out must differ");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (a->dtype == XB_F64) {
+    ElastColl<double> c;
+    NLDev<double> nl;
+    elast_coll<double>(a, c, nl);
+    return launch_tile_steps<double>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, st);
+  }
+  ElastColl<float> c;
+  NLDev<float> nl;
+  elast_coll<float>(a, c, nl);
+  return launch_tile_steps<float>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, st);
 }
 
 } // extern "C"

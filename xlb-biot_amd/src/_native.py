@@ -134,6 +134,8 @@ SIGNATURES = {
     "xb_elast_adjoint": (c_int32, [ctypes.POINTER(ElastAdjointArgs), c_void_p]),
     "xb_darcy_adjoint": (c_int32, [ctypes.POINTER(DarcyAdjointArgs), c_void_p]),
     "xb_elast_mg_adjoint": (c_int32, [ctypes.POINTER(ElastMGAdjointArgs), c_void_p]),
+    "xb_darcy_stepn": (c_int32, [ctypes.POINTER(DarcyArgs), c_int32, c_void_p]),
+    "xb_elast_stepn": (c_int32, [ctypes.POINTER(ElastArgs), c_int32, c_void_p]),
     "xb_vcycle_tail_bytes": (c_int64, [c_int32]),
     "xb_vcycle_tail_prepare": (c_int32, [c_int32, c_double, c_double, c_double, c_int32, c_double, c_int32,
                                          ctypes.POINTER(TailLevel), c_int32, c_void_p, c_int64,

@@ -80,6 +80,31 @@ def tune(key, launch_with_rows, candidates=CANDIDATES, reps=20):
     return best, timings
 
 
+# Plain-step launch kinds (the value cached under a solver's "/multi" key): one step
+# per sweep, two per sweep (register temporal blocking), or m steps on LDS tiles.
+SWEEP1, SWEEP2 = 1, 2
+TILE_BASE = 100
+TILE_STEPS = (2, 3, 4, 6, 8)
+
+
+def tile_choice(m):
+    return TILE_BASE + int(m)
+
+
+def steps_per_launch(choice):
+    """Steps one launch of ``choice`` advances."""
+    return choice - TILE_BASE if choice > TILE_BASE else choice
+
+
+def tune_multi(key, launches, reps=20):
+    """``launches``: {choice: launch()}; times each, keeps the choice with the least
+    time per step and caches it under ``key`` (see :func:`lookup`)."""
+    timings = {c: time_launch(fn, reps) / steps_per_launch(c) for c, fn in launches.items()}
+    best = min(timings, key=timings.get)
+    store(key, best, timings)
+    return best, timings
+
+
 def cycles_to_tol(norms, tol, tail=20):
     """Cycles a residual history needs to reach ``tol`` times its first value: the
     measured count when it gets there, else an extrapolation at the geometric rate

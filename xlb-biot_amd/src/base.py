@@ -114,43 +114,40 @@ class GraphedSteps:
 
 
 class StepRunner:
-    """Persistent ping-pong buffers and one captured graph of ``chunk`` steps of
-    ``launch(src, dst)``; :meth:`run` advances a state by any number of steps
-    (graph replays, then single launches) and returns a fresh copy of the result.
-    With ``launch2(src, dst)`` (two steps in one pass) the graph holds ``chunk / 2``
-    two-step launches and remainders use pairs, plus one single step if odd."""
+    """Persistent ping-pong buffers and one captured graph of about ``chunk`` steps;
+    :meth:`run` advances a state by any number of steps (graph replays, then
+    multi-step launches, then single launches) and returns a fresh copy of the
+    result.  ``launch(src, dst)`` is one step; ``multi(src, dst)``, when given, does
+    ``m`` steps in one launch, and the graph then holds an even number of those."""
 
-    def __init__(self, launch, like, chunk, launch2=None):
-        self.launch, self.launch2 = launch, launch2
+    def __init__(self, launch, like, chunk, multi=None, m=1):
+        if multi is None:
+            m = 1
+        self.launch, self.multi, self.m = launch, multi, m
         self.a = torch.empty_like(like)
         self.b = torch.empty_like(like)
         self.a.copy_(like)
         launch(self.a, self.b)  # warm-up outside the capture
-        if launch2 is not None:
-            launch2(self.a, self.b)
-            if chunk % 4:
-                raise ValueError("two-step graphs hold a multiple of 4 steps")
+        if multi is not None:
+            multi(self.a, self.b)
         torch.cuda.synchronize()
-        self.chunk = chunk
-        if chunk < 2:
-            self.graph = None
-        elif launch2 is not None:
-            self.graph = GraphedSteps(launch2, self.a, self.b, chunk // 2)
-        else:
-            self.graph = GraphedSteps(launch, self.a, self.b, chunk)
+        step = multi if multi is not None else launch
+        n = 2 * (chunk // (2 * m))
+        self.graph = GraphedSteps(step, self.a, self.b, n) if n >= 2 else None
+        self.gsteps = n * m
 
     def run(self, f, n):
         self.a.copy_(f)
         done = 0
         if self.graph is not None:
-            while n - done >= self.chunk:
+            while n - done >= self.gsteps:
                 self.graph.replay()
-                done += self.chunk
-        if self.launch2 is not None:
-            while n - done >= 2:
-                self.launch2(self.a, self.b)
+                done += self.gsteps
+        if self.multi is not None:
+            while n - done >= self.m:
+                self.multi(self.a, self.b)
                 self.a, self.b = self.b, self.a
-                done += 2
+                done += self.m
         while done < n:
             self.launch(self.a, self.b)
             self.a, self.b = self.b, self.a

@@ -151,7 +151,9 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         return out
 
     def autotune(self):
-        """Measure and cache the best sweep launch shape of this solver's step."""
+        """Measure and cache the best sweep launch shape of this solver's step, then
+        the best number of steps per launch for step loops: one or two per sweep, or
+        2 .. 8 on LDS tiles (``xb_darcy_stepn``)."""
         lib = nat.lib()
         f = torch.zeros((9, self.nx, self.ny), dtype=self._tdt, device=self.device)
         o = torch.empty_like(f)
@@ -162,6 +164,7 @@ class AdvectionDiffusionReactionBGK(LBMBase):
             nat.check(lib.xb_darcy_step(a, nat.stream()), "xb_darcy_step(tune)")
         key = self._tune_key()
         out = {key: autotune.tune(key, run)}
+        launches = {autotune.SWEEP1: lambda: self._step_soa(f, None, False, out=o)}
         if min(self.nx, self.ny) >= 2:
             a2 = self._args(f, o, None, None, True)
 
@@ -170,7 +173,30 @@ class AdvectionDiffusionReactionBGK(LBMBase):
                 nat.check(lib.xb_darcy_step2(a2, nat.stream()), "xb_darcy_step2(tune)")
             key2 = self._tune_key(True)
             out[key2] = autotune.tune(key2, run2)
+            launches[autotune.SWEEP2] = lambda: self._step2_into(f, o)
+        for m in autotune.TILE_STEPS:
+            launches[autotune.tile_choice(m)] = (lambda m=m: self._stepn_into(f, o, m))
+        km = self._tune_key() + "/multi"
+        out[km] = autotune.tune_multi(km, launches)
         return out
+
+    def _stepn_into(self, f, out, m, ext_source=None):
+        """``m`` steps in one launch on LDS tiles (``xb_darcy_stepn``)."""
+        nat.check(nat.lib().xb_darcy_stepn(self._args(f, out, ext_source, None), int(m), nat.stream()),
+                  "xb_darcy_stepn")
+        return out
+
+    def _multi(self):
+        """The tuned plain-step launch (choice, steps per launch, launch(src, dst))."""
+        choice = autotune.lookup(self._tune_key() + "/multi")
+        if choice == 0:
+            choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
+        m = autotune.steps_per_launch(choice)
+        if choice > autotune.TILE_BASE:
+            return choice, m, (lambda s, d: self._stepn_into(s, d, m))
+        if choice == autotune.SWEEP2:
+            return choice, 2, (lambda s, d: self._step2_into(s, d))
+        return choice, 1, None
 
     def _step_soa(self, f, ext_source=None, fpc=False, out=None):
         out = torch.empty_like(f) if out is None else out
@@ -182,12 +208,12 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         """``n`` steps without I/O (the body of ``run`` when nothing happens between
         steps), replayed from a captured HIP graph of ``chunk`` steps."""
         fs = self._soa(f, 9)
-        two = chunk % 4 == 0 and min(self.nx, self.ny) >= 2
-        key = ("steps", chunk, two)
+        choice, m, multi = self._multi()
+        key = ("steps", chunk, choice)
         runner = self._runners.get(key)
         if runner is None:
             runner = self._runners[key] = StepRunner(lambda s, d: self._step_soa(s, None, False, out=d), fs, chunk,
-                                                     (lambda s, d: self._step2_into(s, d)) if two else None)
+                                                     multi, m)
         return view(runner.run(fs, n))
 
     def collision(self, f, external_source=None):

@@ -398,6 +398,8 @@ class LBMElasticity(object):
                         nat.check(lib.xb_elast_step2(a2, nat.stream()), "xb_elast_step2(tune)")
                     key2 = self._tune_key(level, mode, ext is not None, False, True)
                     out[key2] = autotune.tune(key2, run2)
+        if self.levels == 1 and (levels is None or 0 in levels):
+            out.update(self.tune_step_loops(external_force))
         return out
 
     def _run_step(self, f, ext=None, level=0, fpc=False):
@@ -420,19 +422,53 @@ class LBMElasticity(object):
         nat.check(nat.lib().xb_elast_step2(a, nat.stream()), "xb_elast_step2")
         return out
 
+    def _stepn_into(self, f, out, m, ext=None):
+        """``m`` single-grid steps in one launch on LDS tiles (``xb_elast_stepn``)."""
+        a = self._launch(f, out, 0, nat.XB_MODE_STEP, ext=ext)
+        nat.check(nat.lib().xb_elast_stepn(a, int(m), nat.stream()), "xb_elast_stepn")
+        return out
+
+    def _multi_key(self, has_ext):
+        return self._tune_key(0, nat.XB_MODE_STEP, has_ext, False) + "/multi"
+
+    def tune_step_loops(self, external_force=False):
+        """Measure and cache the best number of single-grid steps per launch for step
+        loops: one or two per sweep, or 2 .. 8 on LDS tiles."""
+        f = torch.zeros((8, self.nx, self.ny), dtype=self._tdt, device=self.device)
+        o = torch.empty_like(f)
+        ext = torch.zeros((2, self.nx, self.ny), dtype=self._tdt, device=self.device) if external_force else None
+        launches = {autotune.SWEEP1: lambda: self._step_into(f, o, ext)}
+        if min(self.nx, self.ny) >= 2:
+            launches[autotune.SWEEP2] = lambda: self._step2_into(f, o, ext)
+        for m in autotune.TILE_STEPS:
+            launches[autotune.tile_choice(m)] = (lambda m=m: self._stepn_into(f, o, m, ext))
+        key = self._multi_key(external_force)
+        return {key: autotune.tune_multi(key, launches)}
+
+    def _multi(self, ext):
+        choice = autotune.lookup(self._multi_key(ext is not None))
+        if choice == 0:
+            choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
+        m = autotune.steps_per_launch(choice)
+        if choice > autotune.TILE_BASE:
+            return choice, m, (lambda s, d: self._stepn_into(s, d, m, ext))
+        if choice == autotune.SWEEP2:
+            return choice, 2, (lambda s, d: self._step2_into(s, d, ext))
+        return choice, 1, None
+
     def run_steps(self, f, n, chunk=64, external_force=None):
         """``n`` single-grid steps with a fixed external force and no I/O, replayed
-        from a captured HIP graph of ``chunk`` steps."""
+        from a captured HIP graph of about ``chunk`` steps (the tuned number of steps
+        per launch, see :meth:`tune_step_loops`)."""
         if self.levels != 1:
             raise ValueError("run_steps is the single-grid loop; multigrid uses step() / run()")
         fs = self._soa(f, 8)
-        two = chunk % 4 == 0 and min(self.nx, self.ny) >= 2
-        key = ("steps", chunk, external_force is None, two)
+        ext = self._soa(external_force, 2)
+        choice, m, multi = self._multi(ext)
+        key = ("steps", chunk, external_force is None, choice)
         runner = self._runners.get(key)
         if runner is None or external_force is not None:
-            ext = self._soa(external_force, 2)
-            runner = StepRunner(lambda s, d: self._step_into(s, d, ext), fs, chunk,
-                                (lambda s, d: self._step2_into(s, d, ext)) if two else None)
+            runner = StepRunner(lambda s, d: self._step_into(s, d, ext), fs, chunk, multi, m)
             if external_force is None:
                 self._runners[key] = runner
         return view(runner.run(fs, n))
