@@ -878,10 +878,19 @@ __device__ __forceinline__ void rules_lds(T (&v)[Q], const T *own, const int8_t 
   for (int d = 0; d < Q; ++d) v[d] = out[d];
 }
 
+// The rule table of a tile in LDS, with one more row of identity rules (codes 0,
+// constants 0) for the cells without rules: a tile holding any wall cell applies the
+// rules of all its cells in straight-line code, so that the four cells of a thread
+// share their LDS round trips (per-cell branches would serialize them).
+template <typename T, int Q> struct TileRules {
+  int8_t code_pc[(LDS_BC_SLOTS + 1) * Q], code_ps[(LDS_BC_SLOTS + 1) * Q];
+  T const_pc[(LDS_BC_SLOTS + 1) * Q], const_ps[(LDS_BC_SLOTS + 1) * Q];
+};
+
 template <typename T, class Coll, int BC>
 __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int tiles_y, const Coll &coll,
                                                 const T *__restrict__ fin, T *__restrict__ fout, const BCDev<T> &bc,
-                                                T *lds, BCShared<T, Coll::Q> *sh) {
+                                                T *lds, TileRules<T, Coll::Q> *sh) {
   constexpr int Q = Coll::Q;
   constexpr bool HAS_BC = BC != 0;
   constexpr int TT = TILE * TILE;
@@ -891,13 +900,15 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
   const int x0 = bx * inner - steps, y0 = by * inner - steps;
   const size_t N = (size_t)nx * ny;
   if constexpr (BC == 2) {
-    // the (small, deduplicated) rule table into LDS by the whole block
+    // the (small, deduplicated) rule table into LDS by the whole block, then the
+    // identity row
     const int n = bc.nslots * Q;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-      sh->code_pc[k] = bc.code_pc != nullptr ? bc.code_pc[k] : int8_t(0);
-      sh->const_pc[k] = bc.const_pc != nullptr ? bc.const_pc[k] : T(0);
-      sh->code_ps[k] = bc.code_ps != nullptr ? bc.code_ps[k] : int8_t(0);
-      sh->const_ps[k] = bc.const_ps != nullptr ? bc.const_ps[k] : T(0);
+    for (int k = threadIdx.x; k < n + Q; k += blockDim.x) {
+      const bool id = k >= n;
+      sh->code_pc[k] = (!id && bc.code_pc != nullptr) ? bc.code_pc[k] : int8_t(0);
+      sh->const_pc[k] = (!id && bc.const_pc != nullptr) ? bc.const_pc[k] : T(0);
+      sh->code_ps[k] = (!id && bc.code_ps != nullptr) ? bc.code_ps[k] : int8_t(0);
+      sh->const_ps[k] = (!id && bc.const_ps != nullptr) ? bc.const_ps[k] : T(0);
     }
   }
   T f[TILE_CPT][Q];
@@ -913,22 +924,36 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
     aux[k] = coll.load_aux(cell);
     slot[k] = HAS_BC ? bc.slot[cell] : -1;
   }
-  if constexpr (BC == 2) __syncthreads();
+  bool walls = false; // BC == 2: this tile holds a cell with rules (uniform over the block)
+  if constexpr (BC == 2) {
+    bool mine = false;
+#pragma unroll
+    for (int k = 0; k < TILE_CPT; ++k) mine = mine || slot[k] >= 0;
+    walls = __syncthreads_or(mine) != 0;
+#pragma unroll
+    for (int k = 0; k < TILE_CPT; ++k) slot[k] = slot[k] >= 0 ? slot[k] : bc.nslots;
+  }
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
     for (int k = 0; k < TILE_CPT; ++k) {
       const int c = threadIdx.x + k * TILE_THREADS;
       T pc[Q];
       coll.collide(f[k], aux[k], pc);
-      if constexpr (HAS_BC) {
-        const int sl = slot[k];
-        if (sl >= 0 && bc.code_pc != nullptr) {
-          // the rules' sources, the cell's pre-collision populations, go through its own
-          // LDS slots (only this thread touches them before the barrier)
+      // post-collision rules: their sources, the cell's pre-collision populations, go
+      // through its own LDS slots (only this thread touches them before the barrier)
+      if constexpr (BC == 2) {
+        if (walls && bc.code_pc != nullptr) {
+          const int sl = slot[k];
 #pragma unroll
           for (int d = 0; d < Q; ++d) lds[d * TT + c] = f[k][d];
-          if constexpr (BC == 2) rules_lds<T, Q, TT>(pc, lds + c, sh->code_pc + sl * Q, sh->const_pc + sl * Q);
-          else rules_lds<T, Q, TT>(pc, lds + c, bc.code_pc + (size_t)sl * Q, bc.const_pc + (size_t)sl * Q);
+          rules_lds<T, Q, TT>(pc, lds + c, sh->code_pc + sl * Q, sh->const_pc + sl * Q);
+        }
+      } else if constexpr (HAS_BC) {
+        const int sl = slot[k];
+        if (sl >= 0 && bc.code_pc != nullptr) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) lds[d * TT + c] = f[k][d];
+          rules_lds<T, Q, TT>(pc, lds + c, bc.code_pc + (size_t)sl * Q, bc.const_pc + (size_t)sl * Q);
         }
       }
 #pragma unroll
@@ -946,12 +971,15 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
         const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
         v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
       }
-      if constexpr (HAS_BC) {
-        const int sl = slot[k];
-        if (sl >= 0 && bc.code_ps != nullptr) {
-          if constexpr (BC == 2) rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + sl * Q, sh->const_ps + sl * Q);
-          else rules_lds<T, Q, TT>(v, lds + c, bc.code_ps + (size_t)sl * Q, bc.const_ps + (size_t)sl * Q);
+      if constexpr (BC == 2) {
+        if (walls && bc.code_ps != nullptr) {
+          const int sl = slot[k];
+          rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + sl * Q, sh->const_ps + sl * Q);
         }
+      } else if constexpr (HAS_BC) {
+        const int sl = slot[k];
+        if (sl >= 0 && bc.code_ps != nullptr)
+          rules_lds<T, Q, TT>(v, lds + c, bc.code_ps + (size_t)sl * Q, bc.const_ps + (size_t)sl * Q);
       }
 #pragma unroll
       for (int d = 0; d < Q; ++d) f[k][d] = v[d];
@@ -977,7 +1005,7 @@ k_tile_steps(int nx, int ny, int steps, int tiles_y, const Coll coll, const T *_
              const BCDev<T> bc) {
   __shared__ T lds[Coll::Q * TILE * TILE];
   if constexpr (BC == 2) {
-    __shared__ BCShared<T, Coll::Q> sh;
+    __shared__ TileRules<T, Coll::Q> sh;
     tile_steps_body<T, Coll, BC>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, &sh);
   } else {
     tile_steps_body<T, Coll, BC>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, nullptr);
