@@ -69,10 +69,13 @@ def timed_steps(torch, run, f, steps):
     return f, time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3
 
 
-def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True):
+def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True, tile_steps=0):
     d = darcy_workload(n, precision)
     if tune:
         d.autotune()
+    if tile_steps:
+        from src import autotune
+        d.step_launch = autotune.tile_choice(tile_steps)
     f = d.assign_fields_sharded()
     f = d.run_steps(f, warmup, chunk)
     f, wall, gpu = timed_steps(torch, lambda f, k: d.run_steps(f, k, chunk), f, steps)
@@ -579,7 +582,7 @@ def only(torch, args):
         log(f"copy: {args.steps} launches of {a.numel() * 8} B")
     elif args.only == "c2":
         from src import autotune
-        r = bench_darcy(torch, args.n, "f64/f64", args.steps, 2, chunk=16)
+        r = bench_darcy(torch, args.n, "f64/f64", args.steps, 2, chunk=16, tile_steps=args.tile_steps)
         d = r["solver"]
         log(f"c2 {d.nx}x{d.ny}: {r['launch_kind']} launches of {r['steps_per_launch']} steps, "
             f"{r['kernel_s'] * 1e6:.2f} us/launch, {r['kernel_s'] * 1e6 / r['steps_per_launch']:.2f} us/step")
@@ -610,6 +613,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched Biot extra")
+    ap.add_argument("--tile-steps", type=int, default=0,
+                    help="with --only c2: steps per launch of the LDS-tile kernel instead of the tuned launch")
     ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint", "biot_vcycle",
                              "biot_adjoint"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
@@ -692,6 +697,8 @@ def main():
         extra("c5_batched_biot_fp64", lambda: bench_c5(torch, n_inst=args.instances))
         log("adjoint: consolidation geometry 1024, FP32, 1000 pseudo-steps, checkpoints every 50")
         extra("adjoint_1024_fp32", lambda: bench_adjoint(torch))
+        log("C4: gradient through the coupled multigrid consolidation, 1024 x 1026, FP32, 50 steps")
+        extra("c4_biot_adjoint_1024_fp32", lambda: bench_biot_adjoint(torch))
         log("device copy 1 GiB (achievable HBM bandwidth)")
         extra("stream_copy", lambda: bench_copy(torch))
         log("Biot 4096^2 fused-coupling V-cycle (6 levels, k = 2), fp64")

@@ -187,8 +187,10 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         return out
 
     def _multi(self):
-        """The tuned plain-step launch (choice, steps per launch, launch(src, dst))."""
-        choice = autotune.lookup(self._tune_key() + "/multi")
+        """The plain-step launch of step loops (choice, steps per launch,
+        launch(src, dst)): ``self.step_launch`` when set (an ``autotune`` choice),
+        else the tuned one."""
+        choice = getattr(self, "step_launch", None) or autotune.lookup(self._tune_key() + "/multi")
         if choice == 0:
             choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
         m = autotune.steps_per_launch(choice)
