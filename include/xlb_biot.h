@@ -143,6 +143,13 @@ int xb_elast_step2(const xb_elast_args *args, void *stream);
  * restrictions as xb_elast_step2. */
 int xb_elast_stepn(const xb_elast_args *args, int32_t steps, void *stream);
 
+/* `steps` consecutive XB_MODE_SMOOTH sweeps of one multigrid level in one launch on
+ * 32 x 32 LDS tiles (residual, external force, fused coupling force with div_out,
+ * per-cell Lame fields as xb_elast_step; bit-identical to `steps` calls).  steps <= 6
+ * (<= 3 with the coupling force, which reaches two cells per step).  A gradient wall
+ * table of more than 32 slots: XB_EUNSUPPORTED (use the sweeps). */
+int xb_elast_smoothn(const xb_elast_args *args, int32_t steps, void *stream);
+
 /* Number of doubles in norm_partials for a RESIDUAL call with these args. */
 int64_t xb_elast_partials_count(const xb_elast_args *args);
 

@@ -68,3 +68,40 @@ def test_run_steps_with_tuned_multistep_matches_single_steps():
     ref = _single(lambda a, b: d._step_soa(a, None, False, out=b), f0, 77)
     got = d._soa(d.run_steps(f0.permute(1, 2, 0), 77, chunk=32), 9)
     assert torch.equal(got, ref)
+
+
+def _smooth_cases(case, precision):
+    import numpy as np
+    g = torch.Generator().manual_seed(4)
+    if case == "periodic_biot":
+        p = P.biot_periodic("gpu", nx=96, levels=3, precision=precision)
+        level = 0
+    else:
+        rng = np.random.default_rng(5)
+        p = P.biot_consolidation("gpu", nx=64, levels=3, k_smooth=4, precision=precision)
+        if case == "fields":
+            p.elasticity.set_lame_fields(rng.uniform(1.5, 2.5, (64, 66)), rng.uniform(0.7, 1.3, (64, 66)))
+        level = 1 if case == "coarse" else 0
+    el = p.elasticity
+    nx, ny = el.grid_size(level)
+    x0, x1, y0, y1 = el.interior_box(level)
+    dt = el._tdt
+    f = (1e-3 * torch.randn((8, nx, ny), generator=g, dtype=torch.float64)).to(dt).cuda()
+    r = (1e-4 * torch.randn((8, x1 - x0, y1 - y0), generator=g, dtype=torch.float64)).to(dt).cuda() \
+        if level > 0 else None
+    base = (1e-4 * torch.randn((2, nx, ny), generator=g, dtype=torch.float64)).to(dt).cuda() if level == 0 else None
+    return el, p.coupling_force(), level, f, r, base
+
+
+@pytest.mark.parametrize("case", ["periodic_biot", "consolidation", "fields", "coarse"])
+@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (3, "f64/f64"), (2, "f32/f32")])
+def test_coupled_smoothn_matches_sweeps(case, m, precision):
+    say("coupled smoothing on tiles", case, m, precision)
+    el, nlf, level, f, r, base = _smooth_cases(case, precision)
+    div = el._divergence_soa(f)
+    a, da = f.clone(), div.clone()
+    for _ in range(m):
+        a, da = el._smooth_nl(a, da, level, r, base, nlf)
+    b, db = el._smoothn_nl(f, div, level, r, base, nlf, m)
+    assert torch.equal(a, b), (a - b).abs().max().item()
+    assert torch.equal(da, db), (da - db).abs().max().item()

@@ -60,6 +60,8 @@ class DarcyAdjointArgs(ctypes.Structure):
 
 
 XB_ADJ_SMOOTH, XB_ADJ_RESIDUAL = 1, 2
+# rule tables of at most this many slots are staged in LDS (xb_device.h LDS_BC_SLOTS)
+LDS_BC_SLOTS = 32
 
 
 class TailLevel(ctypes.Structure):
@@ -136,6 +138,7 @@ SIGNATURES = {
     "xb_elast_mg_adjoint": (c_int32, [ctypes.POINTER(ElastMGAdjointArgs), c_void_p]),
     "xb_darcy_stepn": (c_int32, [ctypes.POINTER(DarcyArgs), c_int32, c_void_p]),
     "xb_elast_stepn": (c_int32, [ctypes.POINTER(ElastArgs), c_int32, c_void_p]),
+    "xb_elast_smoothn": (c_int32, [ctypes.POINTER(ElastArgs), c_int32, c_void_p]),
     "xb_vcycle_tail_bytes": (c_int64, [c_int32]),
     "xb_vcycle_tail_prepare": (c_int32, [c_int32, c_double, c_double, c_double, c_int32, c_double, c_int32,
                                          ctypes.POINTER(TailLevel), c_int32, c_void_p, c_int64,

@@ -656,11 +656,49 @@ class LBMElasticity(object):
         f_c, _ = self._vcycle_nl(f_c, d_c, level + 1, res_c, None, nlf)
         return f_c, norm
 
+    def _smooth_tile_steps(self, level, nlf):
+        """Smoothing steps per launch on LDS tiles at ``level`` (1: one sweep per
+        step): ``self.smooth_tile_steps`` when set, else 2, where the coupling
+        force's wall table fits the tile kernel's LDS table."""
+        m = getattr(self, "smooth_tile_steps", 2)
+        if m <= 1:
+            return 1
+        nx, ny = self.grid_size(level)
+        if nlf is not None:
+            gt = nlf.darcy._grad_table(nx, ny)
+            if gt is not None and gt.nslots > nat.LDS_BC_SLOTS:
+                return 1
+        return min(m, 3 if nlf is not None else 6)
+
+    def _smoothn_nl(self, f, div, level, resid, base, nlf, m, out=None, out_div=None):
+        """``m`` coupled smoothing sweeps in one launch (``xb_elast_smoothn``)."""
+        out = torch.empty_like(f) if out is None else out
+        out_div = torch.empty_like(div) if out_div is None else out_div
+        a = self._nl_args(self._launch(f, out, level, nat.XB_MODE_SMOOTH, ext=base, resid=resid), div, out_div,
+                          nlf, level)
+        nat.check(nat.lib().xb_elast_smoothn(a, int(m), nat.stream()), "xb_elast_smoothn")
+        return out, out_div
+
+    def _smooth_seq_nl(self, f, div, level, residual, base, nlf, k, out=None, out_div=None):
+        """``k`` coupled smoothing sweeps: tile launches of the tuned number of steps,
+        the remainder as single sweeps; the last one writes ``out`` when given."""
+        m = self._smooth_tile_steps(level, nlf)
+        done = 0
+        while done < k:
+            step = min(m, k - done) if m > 1 else 1
+            last = done + step == k
+            o, od = (out, out_div) if last else (None, None)
+            if step > 1:
+                f, div = self._smoothn_nl(f, div, level, residual, base, nlf, step, o, od)
+            else:
+                f, div = self._smooth_nl(f, div, level, residual, base, nlf, o, od)
+            done += step
+        return f, div
+
     def _vcycle_nl(self, f, div, level, residual, base, nlf, out=None, out_div=None):
         """``_vcycle`` with the Biot coupling force fused into the sweeps; ``div`` is
         the divergence of ``f`` and is carried along.  Returns ``(f, div[, norm])``."""
-        for _ in range(self.k_smooth):
-            f, div = self._smooth_nl(f, div, level, residual, base, nlf)
+        f, div = self._smooth_seq_nl(f, div, level, residual, base, nlf, self.k_smooth)
         norm = None
         if level < self.levels - 1:
             f_c, norm = self._coarse_solve_nl(f, div, level, residual, base, nlf)
@@ -671,10 +709,7 @@ class LBMElasticity(object):
                                                          nat.ptr(f), nat.ptr(f_c), nxc, nyc, x0, x1, y0, y1,
                                                          nat.ptr(div), *(self._fields() if level == 0 else (None, None)),
                                                          nat.stream()), "xb_elast_prolong_add_div")
-        for k in range(self.k_smooth):
-            last = k == self.k_smooth - 1
-            f, div = self._smooth_nl(f, div, level, residual, base, nlf, out if last else None,
-                                     out_div if last else None)
+        f, div = self._smooth_seq_nl(f, div, level, residual, base, nlf, self.k_smooth, out, out_div)
         if level == 0:
             return f, div, norm
         return f, div
