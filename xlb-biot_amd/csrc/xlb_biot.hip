@@ -1314,6 +1314,372 @@ template <typename T> int build_sweep_jobs(const xb_elast_args *args, const int3
   return XB_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Damped smoothing sweeps of a multigrid level, several per launch on LDS tiles:
+// f <- (1 - gamma) f + gamma S(f - r; F(f)), r the level's residual on its interior
+// box, F the body force (finest level) + the external force + the fused coupling
+// force G_s(D f).  Per step every cell writes its divergence D f to LDS (coupled
+// solves); barrier; every cell forms the coupling force from its neighbours'
+// divergences (the gradient's wall rules included), collides f - r, applies its
+// post-collision rules and writes the post-collision populations to LDS; barrier;
+// every cell pulls, applies the post-streaming rules and damps.  The coupling force
+// reaches two cells per step (divergence, gradient) and the pull one more, so the
+// exact core shrinks by R = 2 cells per side and step with the coupling (1 without):
+// after m steps the inner (32 - 2 R m)^2 cells are exact and are written, with the
+// divergence of the result.  Per cell the arithmetic is that of m XB_MODE_SMOOTH
+// sweeps (bit-identical).  Rule tables of at most LDS_BC_SLOTS slots only (staged in
+// LDS with an identity row, as k_tile_steps); larger ones stay on the sweeps.
+template <typename T> struct TileSmoothDev {
+  int nx, ny, steps, tiles_y, ix0, ix1, iy0, iy1;
+  T gamma, omg;
+  ElastColl<T> coll;
+  NLDev<T> nl;
+  const T *fin, *res;
+  T *fout;
+  BCDev<T> bc;
+};
+
+template <typename T> struct TileGradRules {
+  int8_t code[(LDS_BC_SLOTS + 1) * 9];
+  T cst[(LDS_BC_SLOTS + 1) * 9];
+};
+
+template <typename T, int Q>
+__device__ __forceinline__ void stage_rule_rows(const int8_t *gc, const T *gk, int nslots, int8_t *code, T *cst) {
+  const int n = nslots * Q;
+  for (int k = threadIdx.x; k < n + Q; k += blockDim.x) {
+    const bool id = k >= n || gc == nullptr;
+    code[k] = id ? int8_t(0) : gc[k];
+    cst[k] = id ? T(0) : gk[k];
+  }
+}
+
+template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
+__device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int tile, T *lds, T *ldiv,
+                                                 TileRules<T, 8> *sh, TileGradRules<T> *gsh) {
+  using C = ElastColl<T>;
+  using D = DarcyColl<T>;
+  using Aux = typename C::Aux;
+  constexpr int Q = 8;
+  constexpr int TT = TILE * TILE;
+  constexpr int R = HAS_NL ? 2 : 1;
+  const int nx = A.nx, ny = A.ny, steps = A.steps;
+  const int inner = TILE - 2 * R * steps;
+  const int bx = tile / A.tiles_y, by = tile % A.tiles_y;
+  const int x0 = bx * inner - R * steps, y0 = by * inner - R * steps;
+  const size_t N = (size_t)nx * ny;
+  const int nyi = A.iy1 - A.iy0;
+  const size_t Ni = (size_t)(A.ix1 - A.ix0) * nyi;
+  constexpr bool WALLS = BC != 0;
+  if constexpr (BC == 2) {
+    stage_rule_rows<T, Q>(A.bc.code_pc, A.bc.const_pc, A.bc.nslots, sh->code_pc, sh->const_pc);
+    stage_rule_rows<T, Q>(A.bc.code_ps, A.bc.const_ps, A.bc.nslots, sh->code_ps, sh->const_ps);
+  }
+  if constexpr (GWALLS) stage_rule_rows<T, 9>(A.nl.gbc.code_ps, A.nl.gbc.const_ps, A.nl.gbc.nslots, gsh->code, gsh->cst);
+  T f[TILE_CPT][Q];
+  Aux aux[TILE_CPT];
+  T kd[TILE_CPT];
+  int slot[TILE_CPT], gslot[TILE_CPT], ri[TILE_CPT];
+  bool mine = false, gmine = false;
+#pragma unroll
+  for (int k = 0; k < TILE_CPT; ++k) {
+    const int c = threadIdx.x + k * TILE_THREADS;
+    const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
+    const size_t cell = (size_t)gx * ny + gy;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) f[k][d] = A.fin[d * N + cell];
+    aux[k] = A.coll.load_aux(cell);
+    kd[k] = A.coll.kdiv(aux[k], A.nl.kdiv);
+    slot[k] = (WALLS && A.bc.slot != nullptr) ? A.bc.slot[cell] : -1;
+    gslot[k] = (GWALLS && A.nl.gbc.slot != nullptr) ? A.nl.gbc.slot[cell] : -1;
+    mine = mine || slot[k] >= 0;
+    gmine = gmine || gslot[k] >= 0;
+    ri[k] = -1;
+    if constexpr (HAS_RES) {
+      if (gx >= A.ix0 && gx < A.ix1 && gy >= A.iy0 && gy < A.iy1) ri[k] = (gx - A.ix0) * nyi + (gy - A.iy0);
+    }
+  }
+  const bool walls = BC == 2 && __syncthreads_or(mine) != 0;
+  const bool gwalls = GWALLS && __syncthreads_or(gmine) != 0;
+#pragma unroll
+  for (int k = 0; k < TILE_CPT; ++k) {
+    if constexpr (BC == 2) slot[k] = slot[k] >= 0 ? slot[k] : A.bc.nslots;
+    if constexpr (GWALLS) gslot[k] = gslot[k] >= 0 ? gslot[k] : A.nl.gbc.nslots;
+  }
+  for (int s = 0; s < steps; ++s) {
+    if constexpr (HAS_NL) {
+#pragma unroll
+      for (int k = 0; k < TILE_CPT; ++k) ldiv[threadIdx.x + k * TILE_THREADS] = elast_div_of<T>(f[k], kd[k]);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < TILE_CPT; ++k) {
+      const int c = threadIdx.x + k * TILE_THREADS;
+      const int rx = c / TILE, ry = c % TILE;
+      T pre[Q];
+#pragma unroll
+      for (int d = 0; d < Q; ++d) pre[d] = f[k][d];
+      if constexpr (HAS_RES) {
+        if (ri[k] >= 0) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) pre[d] = pre[d] - A.res[d * Ni + ri[k]];
+        }
+      }
+      Aux ax = aux[k];
+      if constexpr (HAS_NL) {
+        T qs[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+          const int sx = rx - D::cx(d), sy = ry - D::cy(d);
+          const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
+          qs[d] = ldiv[in ? sx * TILE + sy : c];
+        }
+        if (gwalls) {
+          // the gradient's rules read the cell's own divergence
+          const int8_t *code = gsh->code + gslot[k] * 9;
+          const T *cst = gsh->cst + gslot[k] * 9;
+          const T s0 = qs[0];
+          T o[9];
+#pragma unroll
+          for (int d = 0; d < 9; ++d) {
+            const int cc = code[d];
+            T b = cc > 0 ? s0 : -s0;
+            b = cc == XB_RULE_ZERO ? T(0) : b;
+            b = cc == 0 ? qs[d] : b;
+            o[d] = b + cst[d];
+          }
+#pragma unroll
+          for (int d = 0; d < 9; ++d) qs[d] = o[d];
+        }
+        T g0, g1;
+        stencil_grad<T>(qs, A.nl.m3s, g0, g1);
+        ax.b0 = A.nl.base != nullptr ? ax.b0 + g0 : g0;
+        ax.b1 = A.nl.base != nullptr ? ax.b1 + g1 : g1;
+      }
+      T pc[Q];
+      A.coll.collide(pre, ax, pc);
+      if constexpr (BC == 2) {
+        if (walls && A.bc.code_pc != nullptr) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) lds[d * TT + c] = pre[d];
+          rules_lds<T, Q, TT>(pc, lds + c, sh->code_pc + slot[k] * Q, sh->const_pc + slot[k] * Q);
+        }
+      } else if constexpr (BC == 1) {
+        if (slot[k] >= 0 && A.bc.code_pc != nullptr) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) lds[d * TT + c] = pre[d];
+          rules_lds<T, Q, TT>(pc, lds + c, A.bc.code_pc + (size_t)slot[k] * Q, A.bc.const_pc + (size_t)slot[k] * Q);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < Q; ++d) lds[d * TT + c] = pc[d];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TILE_CPT; ++k) {
+      const int c = threadIdx.x + k * TILE_THREADS;
+      const int rx = c / TILE, ry = c % TILE;
+      T v[Q];
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        const int sx = rx - C::cx(d), sy = ry - C::cy(d);
+        const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
+        v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
+      }
+      if constexpr (BC == 2) {
+        if (walls && A.bc.code_ps != nullptr)
+          rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + slot[k] * Q, sh->const_ps + slot[k] * Q);
+      } else if constexpr (BC == 1) {
+        if (slot[k] >= 0 && A.bc.code_ps != nullptr)
+          rules_lds<T, Q, TT>(v, lds + c, A.bc.code_ps + (size_t)slot[k] * Q, A.bc.const_ps + (size_t)slot[k] * Q);
+      }
+#pragma unroll
+      for (int d = 0; d < Q; ++d) f[k][d] = A.omg * f[k][d] + A.gamma * v[d];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < TILE_CPT; ++k) {
+    const int c = threadIdx.x + k * TILE_THREADS;
+    const int rx = c / TILE, ry = c % TILE;
+    const int h = R * steps;
+    if (rx < h || rx >= TILE - h || ry < h || ry >= TILE - h) continue;
+    const int gx = x0 + rx, gy = y0 + ry;
+    if (gx >= nx || gy >= ny) continue;
+    const size_t cell = (size_t)gx * ny + gy;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) A.fout[d * N + cell] = f[k][d];
+    if constexpr (HAS_NL) {
+      if (A.nl.div_out != nullptr) A.nl.div_out[cell] = elast_div_of<T>(f[k], kd[k]);
+    }
+  }
+}
+
+template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
+__device__ __forceinline__ void tile_smooth_run(const TileSmoothDev<T> &A, int tile) {
+  constexpr bool WALLS = BC == 2;
+  __shared__ T lds[8 * TILE * TILE];
+  if constexpr (HAS_NL) {
+    __shared__ T ldiv[TILE * TILE];
+    if constexpr (WALLS) {
+      __shared__ TileRules<T, 8> sh;
+      if constexpr (GWALLS) {
+        __shared__ TileGradRules<T> gsh;
+        tile_smooth_body<T, BC, HAS_RES, HAS_NL, GWALLS>(A, tile, lds, ldiv, &sh, &gsh);
+      } else {
+        tile_smooth_body<T, BC, HAS_RES, HAS_NL, GWALLS>(A, tile, lds, ldiv, &sh, nullptr);
+      }
+    } else {
+      if constexpr (GWALLS) {
+        __shared__ TileGradRules<T> gsh;
+        tile_smooth_body<T, BC, HAS_RES, HAS_NL, GWALLS>(A, tile, lds, ldiv, nullptr, &gsh);
+      } else {
+        tile_smooth_body<T, BC, HAS_RES, HAS_NL, GWALLS>(A, tile, lds, ldiv, nullptr, nullptr);
+      }
+    }
+  } else {
+    if constexpr (WALLS) {
+      __shared__ TileRules<T, 8> sh;
+      tile_smooth_body<T, BC, HAS_RES, false, false>(A, tile, lds, nullptr, &sh, nullptr);
+    } else {
+      tile_smooth_body<T, BC, HAS_RES, false, false>(A, tile, lds, nullptr, nullptr, nullptr);
+    }
+  }
+}
+
+template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
+__global__ void __launch_bounds__(TILE_THREADS) k_tile_smooth(const TileSmoothDev<T> A) {
+  tile_smooth_run<T, BC, HAS_RES, HAS_NL, GWALLS>(A, (int)blockIdx.x);
+}
+
+// Batched form: every block looks up its job (one level of one instance) and tile.
+template <typename T> struct TileSmoothJob {
+  TileSmoothDev<T> A;
+  int32_t inst, block0;
+};
+
+template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
+__global__ void __launch_bounds__(TILE_THREADS)
+k_tile_smooth_batch(const TileSmoothJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
+                    const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
+  const int b = xcd_block_id();
+  const int j = block_job[b];
+  if (j < 0) return;                                         // uniform per block
+  if (active != nullptr && active[block_inst[b]] == 0) return; // uniform per block
+  const TileSmoothJob<T> &J = jobs[j];
+  tile_smooth_run<T, BC, HAS_RES, HAS_NL, GWALLS>(J.A, b - J.block0);
+}
+
+template <typename T, int BC, bool HAS_RES>
+void launch_tile_smooth_nl(const TileSmoothDev<T> &A, dim3 grid, hipStream_t st, bool nl, bool gwalls) {
+  if (!nl) hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, false, false>), grid, dim3(TILE_THREADS), 0, st, A);
+  else if (gwalls) hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, true, true>), grid, dim3(TILE_THREADS), 0, st, A);
+  else hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, true, false>), grid, dim3(TILE_THREADS), 0, st, A);
+}
+
+// the tile-smoothing operands of one xb_elast_args; returns the number of tiles
+template <typename T> int make_tile_smooth(const xb_elast_args *a, int steps, TileSmoothDev<T> &A) {
+  A = TileSmoothDev<T>{};
+  NLDev<T> nl;
+  elast_coll<T>(a, A.coll, nl);
+  A.nl = nl;
+  A.nx = a->nx; A.ny = a->ny; A.steps = steps;
+  A.ix0 = a->ix0; A.ix1 = a->ix1; A.iy0 = a->iy0; A.iy1 = a->iy1;
+  A.gamma = T(a->gamma);
+  A.omg = T(1.0 - a->gamma);
+  A.fin = static_cast<const T *>(a->f_in);
+  A.res = static_cast<const T *>(a->resid);
+  A.fout = static_cast<T *>(a->f_out);
+  A.bc = bcdev<T>(a->bc);
+  const int R = a->div_in != nullptr ? 2 : 1;
+  const int inner = TILE - 2 * R * steps;
+  const int tx = cdiv(a->nx, inner), ty = cdiv(a->ny, inner);
+  A.tiles_y = ty;
+  return tx * ty;
+}
+
+template <typename T> int elast_smoothn_t(const xb_elast_args *a, int steps, hipStream_t st) {
+  TileSmoothDev<T> A;
+  const dim3 grid((unsigned)make_tile_smooth<T>(a, steps, A));
+  const NLDev<T> &nl = A.nl;
+  const bool has_nl = a->div_in != nullptr;
+  const int bcm = bc_mode(A.bc.slot, A.bc.nslots);
+  const bool gwalls = has_nl && nl.gbc.slot != nullptr;
+  const bool res = a->resid != nullptr;
+  if (bcm == 2 && res) launch_tile_smooth_nl<T, 2, true>(A, grid, st, has_nl, gwalls);
+  else if (bcm == 2) launch_tile_smooth_nl<T, 2, false>(A, grid, st, has_nl, gwalls);
+  else if (bcm == 1 && res) launch_tile_smooth_nl<T, 1, true>(A, grid, st, has_nl, gwalls);
+  else if (bcm == 1) launch_tile_smooth_nl<T, 1, false>(A, grid, st, has_nl, gwalls);
+  else if (res) launch_tile_smooth_nl<T, 0, true>(A, grid, st, has_nl, gwalls);
+  else launch_tile_smooth_nl<T, 0, false>(A, grid, st, has_nl, gwalls);
+  return check_launch("multi-step smoothing tile kernel");
+}
+
+
+int check_smoothn(const xb_elast_args *a, int steps, const char *who) {
+  if (int r = validate_elast(a, who)) return r;
+  if (a->mode != XB_MODE_SMOOTH) return fail(XB_EINVAL, "%s: needs XB_MODE_SMOOTH", who);
+  if (a->f_in == a->f_out) return fail(XB_EINVAL, "%s: f_in and f_out must differ", who);
+  const int R = a->div_in != nullptr ? 2 : 1;
+  if (steps < 1 || TILE - 2 * R * steps < 8)
+    return fail(XB_EINVAL, "%s: steps must be in [1, %d]", who, (TILE - 8) / (2 * R));
+  if (a->grad_bc != nullptr && a->grad_bc->cell_slot != nullptr && a->grad_bc->nslots > LDS_BC_SLOTS)
+    return fail(XB_EUNSUPPORTED, "%s: gradient rule tables of more than %d slots stay on the sweeps", who,
+                LDS_BC_SLOTS);
+  return XB_OK;
+}
+
+template <typename T> int build_smoothn_jobs(const xb_elast_args *args, const int32_t *inst, int32_t n,
+                                             std::vector<TileSmoothJob<T>> &jobs, BlockTable &bt, int &flags) {
+  for (int32_t k = 0; k < n; ++k) {
+    const xb_elast_args *a = &args[k];
+    if (int r = check_smoothn(a, a->rows, "xb_batch_prepare(smoothing tile job)")) return r;
+    if (a->dtype != args[0].dtype) return fail(XB_EINVAL, "xb_batch_prepare: job %d: dtype must agree", k);
+    const int f = (a->bc != nullptr && a->bc->cell_slot != nullptr ? 1 : 0) | (a->resid != nullptr ? 2 : 0) |
+                  (a->div_in != nullptr ? 4 : 0) |
+                  (a->div_in != nullptr && a->grad_bc != nullptr && a->grad_bc->cell_slot != nullptr ? 16 : 0);
+    if (k == 0) flags = f | 8;
+    else if (f != (flags & 23))
+      return fail(XB_EINVAL, "xb_batch_prepare: job %d: boundary tables / residual / coupling presence must agree "
+                  "across the batch", k);
+    if ((f & 1) && a->bc->nslots > LDS_BC_SLOTS) flags &= ~8;
+    TileSmoothJob<T> J{};
+    const int tiles = make_tile_smooth<T>(a, a->rows, J.A);
+    J.inst = inst != nullptr ? inst[k] : k;
+    J.block0 = (int32_t)bt.block_job.size();
+    bt.add(k, J.inst, tiles);
+    jobs.push_back(J);
+  }
+  bt.pad_xcd();
+  return XB_OK;
+}
+
+template <typename T, int BC, bool RES>
+void launch_smoothn_batch_nl(const TileSmoothJob<T> *jobs, const int32_t *bj, const int32_t *bi,
+                             const int32_t *active, dim3 grid, hipStream_t st, bool nl, bool gw) {
+  const dim3 block(TILE_THREADS);
+  if (!nl) hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, false, false>), grid, block, 0, st, jobs, bj, bi, active);
+  else if (gw) hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, true, true>), grid, block, 0, st, jobs, bj, bi, active);
+  else hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, true, false>), grid, block, 0, st, jobs, bj, bi, active);
+}
+
+template <typename T> int launch_smoothn_batch(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
+  const char *base = static_cast<const char *>(p->table);
+  const auto *jobs = reinterpret_cast<const TileSmoothJob<T> *>(base + p->jobs_offset);
+  const int32_t *bj = reinterpret_cast<const int32_t *>(base + p->blocks_offset);
+  const int32_t *bi = reinterpret_cast<const int32_t *>(base + p->insts_offset);
+  const dim3 grid(p->nblocks);
+  const int bcm = (p->flags & 1) ? ((p->flags & 8) ? 2 : 1) : 0;
+  const bool res = p->flags & 2, nl = p->flags & 4, gw = p->flags & 16;
+  if (bcm == 2 && res) launch_smoothn_batch_nl<T, 2, true>(jobs, bj, bi, active, grid, st, nl, gw);
+  else if (bcm == 2) launch_smoothn_batch_nl<T, 2, false>(jobs, bj, bi, active, grid, st, nl, gw);
+  else if (bcm == 1 && res) launch_smoothn_batch_nl<T, 1, true>(jobs, bj, bi, active, grid, st, nl, gw);
+  else if (bcm == 1) launch_smoothn_batch_nl<T, 1, false>(jobs, bj, bi, active, grid, st, nl, gw);
+  else if (res) launch_smoothn_batch_nl<T, 0, true>(jobs, bj, bi, active, grid, st, nl, gw);
+  else launch_smoothn_batch_nl<T, 0, false>(jobs, bj, bi, active, grid, st, nl, gw);
+  return check_launch("batched smoothing tile kernel");
+}
+
 int ew_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65535 * 8)); }
 
 template <typename T> int build_prolong_jobs(const xb_prolong_job *args, const int32_t *inst, int32_t n,
@@ -1422,6 +1788,13 @@ int batch_prepare_t(int32_t kind, const void *jobs, const int32_t *inst, int32_t
     if (int r = build_sweep_jobs<T>(a, inst, n, v, bt, flags)) return r;
     return finish_table(v, bt, kind, dt, a[0].mode, flags, table, bytes, plan, st, need);
   }
+  case XB_BATCH_ELAST_SMOOTHN: {
+    std::vector<TileSmoothJob<T>> v;
+    int flags = 0;
+    const xb_elast_args *a = static_cast<const xb_elast_args *>(jobs);
+    if (int r = build_smoothn_jobs<T>(a, inst, n, v, bt, flags)) return r;
+    return finish_table(v, bt, kind, dt, a[0].mode, flags, table, bytes, plan, st, need);
+  }
   case XB_BATCH_PROLONG_DIV: {
     std::vector<ProlongJob<T>> v;
     if (int r = build_prolong_jobs<T>(static_cast<const xb_prolong_job *>(jobs), inst, n, v, bt)) return r;
@@ -1476,6 +1849,8 @@ template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *
     if (p->mode == XB_MODE_STEP) return launch_elast_batch<T, XB_MODE_STEP>(p, active, st);
     if (p->mode == XB_MODE_SMOOTH) return launch_elast_batch<T, XB_MODE_SMOOTH>(p, active, st);
     return launch_elast_batch<T, XB_MODE_RESIDUAL>(p, active, st);
+  case XB_BATCH_ELAST_SMOOTHN:
+    return launch_smoothn_batch<T>(p, active, st);
   case XB_BATCH_PROLONG_DIV:
     hipLaunchKernelGGL(k_prolong_batch<T>, dim3(p->nblocks), dim3(256), 0, st,
                        reinterpret_cast<const ProlongJob<T> *>(base + p->jobs_offset), bj, bi, active);
@@ -2281,277 +2656,6 @@ template <typename T> TailDev<T> tail_params(const xb_tail_plan *p) {
   return P;
 }
 
-// ---------------------------------------------------------------------------
-// Damped smoothing sweeps of a multigrid level, several per launch on LDS tiles:
-// f <- (1 - gamma) f + gamma S(f - r; F(f)), r the level's residual on its interior
-// box, F the body force (finest level) + the external force + the fused coupling
-// force G_s(D f).  Per step every cell writes its divergence D f to LDS (coupled
-// solves); barrier; every cell forms the coupling force from its neighbours'
-// divergences (the gradient's wall rules included), collides f - r, applies its
-// post-collision rules and writes the post-collision populations to LDS; barrier;
-// every cell pulls, applies the post-streaming rules and damps.  The coupling force
-// reaches two cells per step (divergence, gradient) and the pull one more, so the
-// exact core shrinks by R = 2 cells per side and step with the coupling (1 without):
-// after m steps the inner (32 - 2 R m)^2 cells are exact and are written, with the
-// divergence of the result.  Per cell the arithmetic is that of m XB_MODE_SMOOTH
-// sweeps (bit-identical).  Rule tables of at most LDS_BC_SLOTS slots only (staged in
-// LDS with an identity row, as k_tile_steps); larger ones stay on the sweeps.
-template <typename T> struct TileSmoothDev {
-  int nx, ny, steps, tiles_y, ix0, ix1, iy0, iy1;
-  T gamma, omg;
-  ElastColl<T> coll;
-  NLDev<T> nl;
-  const T *fin, *res;
-  T *fout;
-  BCDev<T> bc;
-};
-
-template <typename T> struct TileGradRules {
-  int8_t code[(LDS_BC_SLOTS + 1) * 9];
-  T cst[(LDS_BC_SLOTS + 1) * 9];
-};
-
-template <typename T, int Q>
-__device__ __forceinline__ void stage_rule_rows(const int8_t *gc, const T *gk, int nslots, int8_t *code, T *cst) {
-  const int n = nslots * Q;
-  for (int k = threadIdx.x; k < n + Q; k += blockDim.x) {
-    const bool id = k >= n || gc == nullptr;
-    code[k] = id ? int8_t(0) : gc[k];
-    cst[k] = id ? T(0) : gk[k];
-  }
-}
-
-template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
-__device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, T *lds, T *ldiv, TileRules<T, 8> *sh,
-                                                 TileGradRules<T> *gsh) {
-  using C = ElastColl<T>;
-  using D = DarcyColl<T>;
-  using Aux = typename C::Aux;
-  constexpr int Q = 8;
-  constexpr int TT = TILE * TILE;
-  constexpr int R = HAS_NL ? 2 : 1;
-  const int nx = A.nx, ny = A.ny, steps = A.steps;
-  const int inner = TILE - 2 * R * steps;
-  const int bx = blockIdx.x / A.tiles_y, by = blockIdx.x % A.tiles_y;
-  const int x0 = bx * inner - R * steps, y0 = by * inner - R * steps;
-  const size_t N = (size_t)nx * ny;
-  const int nyi = A.iy1 - A.iy0;
-  const size_t Ni = (size_t)(A.ix1 - A.ix0) * nyi;
-  constexpr bool WALLS = BC != 0;
-  if constexpr (BC == 2) {
-    stage_rule_rows<T, Q>(A.bc.code_pc, A.bc.const_pc, A.bc.nslots, sh->code_pc, sh->const_pc);
-    stage_rule_rows<T, Q>(A.bc.code_ps, A.bc.const_ps, A.bc.nslots, sh->code_ps, sh->const_ps);
-  }
-  if constexpr (GWALLS) stage_rule_rows<T, 9>(A.nl.gbc.code_ps, A.nl.gbc.const_ps, A.nl.gbc.nslots, gsh->code, gsh->cst);
-  T f[TILE_CPT][Q];
-  Aux aux[TILE_CPT];
-  T kd[TILE_CPT];
-  int slot[TILE_CPT], gslot[TILE_CPT], ri[TILE_CPT];
-  bool mine = false, gmine = false;
-#pragma unroll
-  for (int k = 0; k < TILE_CPT; ++k) {
-    const int c = threadIdx.x + k * TILE_THREADS;
-    const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
-    const size_t cell = (size_t)gx * ny + gy;
-#pragma unroll
-    for (int d = 0; d < Q; ++d) f[k][d] = A.fin[d * N + cell];
-    aux[k] = A.coll.load_aux(cell);
-    kd[k] = A.coll.kdiv(aux[k], A.nl.kdiv);
-    slot[k] = WALLS ? A.bc.slot[cell] : -1;
-    gslot[k] = GWALLS ? A.nl.gbc.slot[cell] : -1;
-    mine = mine || slot[k] >= 0;
-    gmine = gmine || gslot[k] >= 0;
-    ri[k] = -1;
-    if constexpr (HAS_RES) {
-      if (gx >= A.ix0 && gx < A.ix1 && gy >= A.iy0 && gy < A.iy1) ri[k] = (gx - A.ix0) * nyi + (gy - A.iy0);
-    }
-  }
-  const bool walls = BC == 2 && __syncthreads_or(mine) != 0;
-  const bool gwalls = GWALLS && __syncthreads_or(gmine) != 0;
-#pragma unroll
-  for (int k = 0; k < TILE_CPT; ++k) {
-    if constexpr (BC == 2) slot[k] = slot[k] >= 0 ? slot[k] : A.bc.nslots;
-    if constexpr (GWALLS) gslot[k] = gslot[k] >= 0 ? gslot[k] : A.nl.gbc.nslots;
-  }
-  for (int s = 0; s < steps; ++s) {
-    if constexpr (HAS_NL) {
-#pragma unroll
-      for (int k = 0; k < TILE_CPT; ++k) ldiv[threadIdx.x + k * TILE_THREADS] = elast_div_of<T>(f[k], kd[k]);
-      __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) {
-      const int c = threadIdx.x + k * TILE_THREADS;
-      const int rx = c / TILE, ry = c % TILE;
-      T pre[Q];
-#pragma unroll
-      for (int d = 0; d < Q; ++d) pre[d] = f[k][d];
-      if constexpr (HAS_RES) {
-        if (ri[k] >= 0) {
-#pragma unroll
-          for (int d = 0; d < Q; ++d) pre[d] = pre[d] - A.res[d * Ni + ri[k]];
-        }
-      }
-      Aux ax = aux[k];
-      if constexpr (HAS_NL) {
-        T qs[9];
-#pragma unroll
-        for (int d = 0; d < 9; ++d) {
-          const int sx = rx - D::cx(d), sy = ry - D::cy(d);
-          const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
-          qs[d] = ldiv[in ? sx * TILE + sy : c];
-        }
-        if (gwalls) {
-          // the gradient's rules read the cell's own divergence
-          const int8_t *code = gsh->code + gslot[k] * 9;
-          const T *cst = gsh->cst + gslot[k] * 9;
-          const T s0 = qs[0];
-          T o[9];
-#pragma unroll
-          for (int d = 0; d < 9; ++d) {
-            const int cc = code[d];
-            T b = cc > 0 ? s0 : -s0;
-            b = cc == XB_RULE_ZERO ? T(0) : b;
-            b = cc == 0 ? qs[d] : b;
-            o[d] = b + cst[d];
-          }
-#pragma unroll
-          for (int d = 0; d < 9; ++d) qs[d] = o[d];
-        }
-        T g0, g1;
-        stencil_grad<T>(qs, A.nl.m3s, g0, g1);
-        ax.b0 = A.nl.base != nullptr ? ax.b0 + g0 : g0;
-        ax.b1 = A.nl.base != nullptr ? ax.b1 + g1 : g1;
-      }
-      T pc[Q];
-      A.coll.collide(pre, ax, pc);
-      if constexpr (BC == 2) {
-        if (walls && A.bc.code_pc != nullptr) {
-#pragma unroll
-          for (int d = 0; d < Q; ++d) lds[d * TT + c] = pre[d];
-          rules_lds<T, Q, TT>(pc, lds + c, sh->code_pc + slot[k] * Q, sh->const_pc + slot[k] * Q);
-        }
-      } else if constexpr (BC == 1) {
-        if (slot[k] >= 0 && A.bc.code_pc != nullptr) {
-#pragma unroll
-          for (int d = 0; d < Q; ++d) lds[d * TT + c] = pre[d];
-          rules_lds<T, Q, TT>(pc, lds + c, A.bc.code_pc + (size_t)slot[k] * Q, A.bc.const_pc + (size_t)slot[k] * Q);
-        }
-      }
-#pragma unroll
-      for (int d = 0; d < Q; ++d) lds[d * TT + c] = pc[d];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) {
-      const int c = threadIdx.x + k * TILE_THREADS;
-      const int rx = c / TILE, ry = c % TILE;
-      T v[Q];
-#pragma unroll
-      for (int d = 0; d < Q; ++d) {
-        const int sx = rx - C::cx(d), sy = ry - C::cy(d);
-        const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
-        v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
-      }
-      if constexpr (BC == 2) {
-        if (walls && A.bc.code_ps != nullptr)
-          rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + slot[k] * Q, sh->const_ps + slot[k] * Q);
-      } else if constexpr (BC == 1) {
-        if (slot[k] >= 0 && A.bc.code_ps != nullptr)
-          rules_lds<T, Q, TT>(v, lds + c, A.bc.code_ps + (size_t)slot[k] * Q, A.bc.const_ps + (size_t)slot[k] * Q);
-      }
-#pragma unroll
-      for (int d = 0; d < Q; ++d) f[k][d] = A.omg * f[k][d] + A.gamma * v[d];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k = 0; k < TILE_CPT; ++k) {
-    const int c = threadIdx.x + k * TILE_THREADS;
-    const int rx = c / TILE, ry = c % TILE;
-    const int h = R * steps;
-    if (rx < h || rx >= TILE - h || ry < h || ry >= TILE - h) continue;
-    const int gx = x0 + rx, gy = y0 + ry;
-    if (gx >= nx || gy >= ny) continue;
-    const size_t cell = (size_t)gx * ny + gy;
-#pragma unroll
-    for (int d = 0; d < Q; ++d) A.fout[d * N + cell] = f[k][d];
-    if constexpr (HAS_NL) {
-      if (A.nl.div_out != nullptr) A.nl.div_out[cell] = elast_div_of<T>(f[k], kd[k]);
-    }
-  }
-}
-
-template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
-__global__ void __launch_bounds__(TILE_THREADS) k_tile_smooth(const TileSmoothDev<T> A) {
-  constexpr bool WALLS = BC == 2;
-  __shared__ T lds[8 * TILE * TILE];
-  if constexpr (HAS_NL) {
-    __shared__ T ldiv[TILE * TILE];
-    if constexpr (WALLS) {
-      __shared__ TileRules<T, 8> sh;
-      if constexpr (GWALLS) {
-        __shared__ TileGradRules<T> gsh;
-        tile_smooth_body<T, BC, HAS_RES, HAS_NL, GWALLS>(A, lds, ldiv, &sh, &gsh);
-      } else {
-        tile_smooth_body<T, BC, HAS_RES, HAS_NL, GWALLS>(A, lds, ldiv, &sh, nullptr);
-      }
-    } else {
-      if constexpr (GWALLS) {
-        __shared__ TileGradRules<T> gsh;
-        tile_smooth_body<T, BC, HAS_RES, HAS_NL, GWALLS>(A, lds, ldiv, nullptr, &gsh);
-      } else {
-        tile_smooth_body<T, BC, HAS_RES, HAS_NL, GWALLS>(A, lds, ldiv, nullptr, nullptr);
-      }
-    }
-  } else {
-    if constexpr (WALLS) {
-      __shared__ TileRules<T, 8> sh;
-      tile_smooth_body<T, BC, HAS_RES, false, false>(A, lds, nullptr, &sh, nullptr);
-    } else {
-      tile_smooth_body<T, BC, HAS_RES, false, false>(A, lds, nullptr, nullptr, nullptr);
-    }
-  }
-}
-
-template <typename T, int BC, bool HAS_RES>
-void launch_tile_smooth_nl(const TileSmoothDev<T> &A, dim3 grid, hipStream_t st, bool nl, bool gwalls) {
-  if (!nl) hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, false, false>), grid, dim3(TILE_THREADS), 0, st, A);
-  else if (gwalls) hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, true, true>), grid, dim3(TILE_THREADS), 0, st, A);
-  else hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, true, false>), grid, dim3(TILE_THREADS), 0, st, A);
-}
-
-template <typename T> int elast_smoothn_t(const xb_elast_args *a, int steps, hipStream_t st) {
-  TileSmoothDev<T> A{};
-  NLDev<T> nl;
-  elast_coll<T>(a, A.coll, nl);
-  A.nl = nl;
-  A.nx = a->nx; A.ny = a->ny; A.steps = steps;
-  A.ix0 = a->ix0; A.ix1 = a->ix1; A.iy0 = a->iy0; A.iy1 = a->iy1;
-  A.gamma = T(a->gamma);
-  A.omg = T(1.0 - a->gamma);
-  A.fin = static_cast<const T *>(a->f_in);
-  A.res = static_cast<const T *>(a->resid);
-  A.fout = static_cast<T *>(a->f_out);
-  A.bc = bcdev<T>(a->bc);
-  const bool has_nl = a->div_in != nullptr;
-  const int R = has_nl ? 2 : 1;
-  const int inner = TILE - 2 * R * steps;
-  const int tx = cdiv(a->nx, inner), ty = cdiv(a->ny, inner);
-  A.tiles_y = ty;
-  const dim3 grid((unsigned)(tx * ty));
-  const int bcm = bc_mode(A.bc.slot, A.bc.nslots);
-  const bool gwalls = has_nl && nl.gbc.slot != nullptr;
-  const bool res = a->resid != nullptr;
-  if (bcm == 2 && res) launch_tile_smooth_nl<T, 2, true>(A, grid, st, has_nl, gwalls);
-  else if (bcm == 2) launch_tile_smooth_nl<T, 2, false>(A, grid, st, has_nl, gwalls);
-  else if (bcm == 1 && res) launch_tile_smooth_nl<T, 1, true>(A, grid, st, has_nl, gwalls);
-  else if (bcm == 1) launch_tile_smooth_nl<T, 1, false>(A, grid, st, has_nl, gwalls);
-  else if (res) launch_tile_smooth_nl<T, 0, true>(A, grid, st, has_nl, gwalls);
-  else launch_tile_smooth_nl<T, 0, false>(A, grid, st, has_nl, gwalls);
-  return check_launch("multi-step smoothing tile kernel");
-}
-
 } // namespace
 
 // ============================================================================
@@ -2854,7 +2958,7 @@ int xb_norm_finalize(const double *partials, int32_t n, double *out, void *strea
 
 int64_t xb_batch_table_bytes(int32_t kind, int32_t dtype, const void *jobs, int32_t njobs) {
   if (check_dtype(dtype) != XB_OK || jobs == nullptr || njobs <= 0) return -1;
-  if (kind < XB_BATCH_ELAST_STEP || kind > XB_BATCH_NORM) return -1;
+  if (kind < XB_BATCH_ELAST_STEP || kind > XB_BATCH_ELAST_SMOOTHN) return -1;
   int64_t need = -1;
   const int r = dtype == XB_F64 ? batch_prepare_t<double>(kind, jobs, nullptr, njobs, nullptr, 0, nullptr, nullptr, &need)
                                 : batch_prepare_t<float>(kind, jobs, nullptr, njobs, nullptr, 0, nullptr, nullptr, &need);
@@ -2865,8 +2969,10 @@ int xb_batch_prepare(int32_t kind, int32_t dtype, const void *jobs, const int32_
                      int64_t table_bytes, xb_batch_plan *plan, void *stream) {
   if (int r = check_dtype(dtype)) return r;
   if (jobs == nullptr || njobs <= 0 || plan == nullptr) return fail(XB_EINVAL, "xb_batch_prepare: bad arguments");
-  if (kind < XB_BATCH_ELAST_STEP || kind > XB_BATCH_NORM) return fail(XB_EINVAL, "xb_batch_prepare: bad kind %d", kind);
-  if (kind == XB_BATCH_ELAST_STEP && static_cast<const xb_elast_args *>(jobs)[0].dtype != dtype)
+  if (kind < XB_BATCH_ELAST_STEP || kind > XB_BATCH_ELAST_SMOOTHN)
+    return fail(XB_EINVAL, "xb_batch_prepare: bad kind %d", kind);
+  if ((kind == XB_BATCH_ELAST_STEP || kind == XB_BATCH_ELAST_SMOOTHN) &&
+      static_cast<const xb_elast_args *>(jobs)[0].dtype != dtype)
     return fail(XB_EINVAL, "xb_batch_prepare: job dtype differs from the batch dtype");
   hipStream_t st = static_cast<hipStream_t>(stream);
   return dtype == XB_F64 ? batch_prepare_t<double>(kind, jobs, inst, njobs, table, table_bytes, plan, st, nullptr)
@@ -2875,7 +2981,7 @@ int xb_batch_prepare(int32_t kind, int32_t dtype, const void *jobs, const int32_
 
 int xb_batch_launch(const xb_batch_plan *plan, const int32_t *active, void *stream) {
   if (plan == nullptr || plan->table == nullptr) return fail(XB_EINVAL, "xb_batch_launch: no plan");
-  if (plan->kind < XB_BATCH_ELAST_STEP || plan->kind > XB_BATCH_FILL)
+  if (plan->kind < XB_BATCH_ELAST_STEP || plan->kind > XB_BATCH_ELAST_SMOOTHN || plan->kind == XB_BATCH_NORM)
     return fail(XB_EINVAL, "xb_batch_launch: plan kind %d is not launchable", plan->kind);
   if (plan->nblocks == 0) return XB_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -3254,15 +3360,7 @@ int xb_elast_stepn(const xb_elast_args *a, int32_t steps, void *stream) {
 }
 
 int xb_elast_smoothn(const xb_elast_args *a, int32_t steps, void *stream) {
-  if (int r = validate_elast(a, "xb_elast_smoothn")) return r;
-  if (a->mode != XB_MODE_SMOOTH) return fail(XB_EINVAL, "xb_elast_smoothn: needs XB_MODE_SMOOTH");
-  if (a->f_in == a->f_out) return fail(XB_EINVAL, "xb_elast_smoothn: f_in and f_out must differ");
-  const int R = a->div_in != nullptr ? 2 : 1;
-  if (steps < 1 || TILE - 2 * R * steps < 8)
-    return fail(XB_EINVAL, "xb_elast_smoothn: steps must be in [1, %d]", (TILE - 8) / (2 * R));
-  if (a->grad_bc != nullptr && a->grad_bc->cell_slot != nullptr && a->grad_bc->nslots > LDS_BC_SLOTS)
-    return fail(XB_EUNSUPPORTED, "xb_elast_smoothn: gradient rule tables of more than %d slots stay on the sweeps",
-                LDS_BC_SLOTS);
+  if (int r = check_smoothn(a, steps, "xb_elast_smoothn")) return r;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (a->dtype == XB_F64) return elast_smoothn_t<double>(a, steps, st);
   return elast_smoothn_t<float>(a, steps, st);

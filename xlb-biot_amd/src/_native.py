@@ -86,7 +86,7 @@ class ElastMGAdjointArgs(ctypes.Structure):
                 ("grad_mu", c_void_p), ("bc", ctypes.POINTER(BCTable))]
 
 
-XB_BATCH_ELAST_STEP, XB_BATCH_PROLONG_DIV, XB_BATCH_FILL, XB_BATCH_NORM = 1, 2, 3, 4
+XB_BATCH_ELAST_STEP, XB_BATCH_PROLONG_DIV, XB_BATCH_FILL, XB_BATCH_NORM, XB_BATCH_ELAST_SMOOTHN = 1, 2, 3, 4, 5
 
 
 class ProlongJob(ctypes.Structure):

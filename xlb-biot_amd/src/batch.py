@@ -37,7 +37,8 @@ class BatchPlan:
     """One prepared batched launch: the job table lives in a device workspace."""
 
     _JOB_TYPES = {nat.XB_BATCH_ELAST_STEP: nat.ElastArgs, nat.XB_BATCH_PROLONG_DIV: nat.ProlongJob,
-                  nat.XB_BATCH_FILL: nat.FillJob, nat.XB_BATCH_NORM: nat.NormJob}
+                  nat.XB_BATCH_FILL: nat.FillJob, nat.XB_BATCH_NORM: nat.NormJob,
+                  nat.XB_BATCH_ELAST_SMOOTHN: nat.ElastArgs}
 
     def __init__(self, kind, dtype, jobs, inst, device):
         lib = nat.lib()
@@ -200,6 +201,16 @@ class ElasticityBatch:
             a.rows = int(rows)
         return a
 
+    def _tile_steps(self, level):
+        """Smoothing steps per batched launch at ``level``: the single-instance choice
+        (``LBMElasticity._smooth_tile_steps``) when every instance at that level agrees,
+        else one sweep per step."""
+        ms = {I.el._smooth_tile_steps(level, I.nlf if self.fused else None)
+              for I in self.inst if I.levels > level}
+        if not self.fused:
+            return 1
+        return ms.pop() if len(ms) == 1 else 1
+
     def _signature(self):
         import hashlib
         sig = repr(sorted((I.el.nx, I.el.ny, I.levels, I.el._tables.get(0) is not None) for I in self.inst))
@@ -255,23 +266,35 @@ class ElasticityBatch:
             k = 1 if f is I.C[level][0] else 0
             return I.C[level][k], I.D[level][k]
 
-        def smooth(level, last_step):
+        def smooth(level, last_step, m=1):
+            # m steps in one launch: on LDS tiles (m > 1) or one sweep
             jobs, ids = [], []
             for i, I in enumerate(self.inst):
                 if I.levels <= level:
                     continue
                 f, d = cur[i, level]
                 of, od = next_buf(i, level, last_step and level == 0)
-                jobs.append(self._sweep_args(I, level, nat.XB_MODE_SMOOTH, f, of, d, od))
+                a = self._sweep_args(I, level, nat.XB_MODE_SMOOTH, f, of, d, od)
+                if m > 1:
+                    a.rows = m
+                jobs.append(a)
                 ids.append(i)
                 cur[i, level] = (of, od)
             if jobs:
-                plans.append(BatchPlan(nat.XB_BATCH_ELAST_STEP, dt, jobs, ids, self.device))
+                kind = nat.XB_BATCH_ELAST_SMOOTHN if m > 1 else nat.XB_BATCH_ELAST_STEP
+                plans.append(BatchPlan(kind, dt, jobs, ids, self.device))
+
+        def smooth_phase(level, post):
+            m = self._tile_steps(level)
+            done = 0
+            while done < ks:
+                step = min(m, ks - done)
+                done += step
+                smooth(level, post and done == ks, step)
 
         norm_jobs = []
         for level in range(L):
-            for _ in range(ks):
-                smooth(level, False)
+            smooth_phase(level, False)
             deeper = [i for i, I in enumerate(self.inst) if I.levels - 1 > level]
             if not deeper:
                 continue
@@ -307,8 +330,7 @@ class ElasticityBatch:
                     jobs.append(nat.ProlongJob(nx, ny, cnx, cny, x0, x1, y0, y1, el.lamda, el.mu, nat.ptr(f),
                                                nat.ptr(fc), nat.ptr(d) if self.fused else None))
                 plans.append(BatchPlan(nat.XB_BATCH_PROLONG_DIV, dt, jobs, deeper, self.device))
-            for k in range(ks):
-                smooth(level, k == ks - 1)
+            smooth_phase(level, True)
         if len(norm_jobs) != self.n:
             raise RuntimeError("every instance needs a finest-level residual")
         norm_jobs.sort(key=lambda t: t[1])
