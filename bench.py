@@ -236,28 +236,38 @@ def bench_biot_vcycle(torch, n=4096, levels=6, reps=10):
     vg.load(fe)
     vg.run(2)
     t = event_pair_time(torch, vg.replay, reps)
-    byt = vcycle_bytes(n, levels, el.k_smooth)
-    gbs = byt / t / 1e9
-    fine = 2 * el.k_smooth + 1  # smoothing + residual sweeps over the finest grid
-    return {"grid": n, "levels": levels, "k_smooth": el.k_smooth, "vcycle_ms": round(t * 1e3, 3),
-            "algorithmic_gb": round(byt / 1e9, 3), "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-            "fine_cell_vcycles_per_s_millions": round(n * n / t / 1e6, 1),
-            "finest_level_sweeps_per_cycle": fine}
+    m = el._smooth_tile_steps(0, ctx["nlf"])
+    byt = vcycle_bytes(n, levels, el.k_smooth, m=m)
+    byt1 = vcycle_bytes(n, levels, el.k_smooth)
+    gbs, gbs1 = byt / t / 1e9, byt1 / t / 1e9
+    return {"grid": n, "levels": levels, "k_smooth": el.k_smooth, "smoothing_steps_per_launch": m,
+            "vcycle_ms": round(t * 1e3, 3),
+            "compulsory_gb": round(byt / 1e9, 3), "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "single_sweep_formulation_gb": round(byt1 / 1e9, 3),
+            "single_sweep_equivalent_gbs": round(gbs1, 1),
+            "single_sweep_equivalent_hbm_frac": round(gbs1 / HBM_PEAK_GBS, 4),
+            "fine_cell_vcycles_per_s_millions": round(n * n / t / 1e6, 1)}
 
 
-def vcycle_bytes(n, levels, k, S=8):
-    """Algorithmic HBM bytes of one fused-coupling V-cycle of a periodic n x n grid:
-    level 0 smoothing reads f, the force and the divergence and writes f and the
-    divergence (20 S per cell), the residual sweep reads f, force and divergence and
-    writes the restriction (11 S + 8 S / 4); coarse levels also read the restricted
-    residual (smoothing 26 S, residual 17 S + 8 S / 4), are zero-filled (9 S) and
-    prolonged (8 S / 4 read coarse, 16 S + 1 S per fine cell)."""
+def vcycle_bytes(n, levels, k, S=8, m=1):
+    """Algorithmic HBM bytes of one fused-coupling V-cycle of a periodic n x n grid
+    whose smoothing runs ``m`` steps per launch (``m = 1``: one sweep per step).  A
+    smoothing sweep of level 0 reads f, the force and the divergence and writes f and
+    the divergence (20 S per cell); a tile launch of m steps forms the divergence
+    itself (19 S per launch); coarse levels also read the restricted residual (26 S,
+    25 S).  The residual sweep reads f, force and divergence and writes the
+    restriction (11 S + 8 S / 4; coarse levels 17 S + 8 S / 4); coarse states are
+    zero-filled (9 S) and prolonged (8 S / 4 read coarse, 16 S + 1 S per fine cell)."""
     total = 0.0
+    launches = 2 * (-(-k // m))
     for lvl in range(levels):
         c = (n >> lvl) ** 2
         coarsest = lvl == levels - 1
-        smooth = 20 if lvl == 0 else 26
-        total += 2 * k * smooth * c
+        if m == 1:
+            smooth = 20 if lvl == 0 else 26
+        else:
+            smooth = 19 if lvl == 0 else 25
+        total += launches * smooth * c
         if not coarsest:
             total += ((11 if lvl == 0 else 17) + 2) * c   # residual + restriction out
             total += 9 * c / 4                             # zero fill of the coarse state
@@ -341,7 +351,8 @@ def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smo
     torch.cuda.synchronize()
     all_active = int(eb.nactive.item())
     batch_cycle_s = e0.elapsed_time(e1) / 1e3 / 6
-    batch_bytes = sum(vcycle_bytes(s_["n"], L, k_smooth) for s_, L in zip(specs, levels))
+    m_b = eb._tile_steps(0)
+    batch_bytes = sum(vcycle_bytes(s_["n"], L, k_smooth, m=m_b) for s_, L in zip(specs, levels))
     big = max(range(n_inst), key=lambda i: (specs[i]["n"], -levels[i]))
     vg = seq[big]._inner
     vg.replay()
@@ -351,7 +362,8 @@ def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smo
     e1.record()
     torch.cuda.synchronize()
     one_cycle_s = e0.elapsed_time(e1) / 1e3 / 6
-    one_bytes = vcycle_bytes(specs[big]["n"], levels[big], k_smooth)
+    one_bytes = vcycle_bytes(specs[big]["n"], levels[big], k_smooth,
+                             m=seq[big].elasticity._smooth_tile_steps(0, seq[big].coupling_force()))
     per_step = [[p.inner_counts[t] for p in seq] for t in range(steps)]
     counts = [c for p in seq for c in p.inner_counts]
     rels = [r for p in seq for r in p.inner_residuals]
@@ -370,6 +382,7 @@ def bench_c5(torch, n_inst=64, steps=2, tol=1e-8, seed=3, max_cycles=3000, k_smo
             "bitwise_equal_to_sequential": bool(same), "depth_tuning_s": round(tune_s, 2),
             "batched_host_split_s": {k: round(v, 4) for k, v in bat.timing.items()},
             "batched_cycle_all_active": {"instances_active": all_active, "ms": round(batch_cycle_s * 1e3, 3),
+                                         "smoothing_steps_per_launch": m_b,
                                          "algorithmic_gb": round(batch_bytes / 1e9, 3),
                                          "gbs": round(batch_bytes / batch_cycle_s / 1e9, 1),
                                          "hbm_frac": round(batch_bytes / batch_cycle_s / 1e9 / HBM_PEAK_GBS, 4)},
