@@ -887,6 +887,50 @@ template <typename T, int Q> struct TileRules {
   T const_pc[(LDS_BC_SLOTS + 1) * Q], const_ps[(LDS_BC_SLOTS + 1) * Q];
 };
 
+// Pull plan of one tile cell: for every direction the LDS slot its streamed
+// population comes from (the neighbour's slot, or the cell's own slot of the source
+// population where a post-streaming rule replaces it) and a sign (0: the rule gives
+// zero, 1: +, 2: -), 16 bits per direction.  Rules are fixed per cell, so the plan is
+// built once per launch and the rules cost nothing in the steps but the constants of
+// rule cells.
+template <int Q> struct PullPlan {
+  uint32_t w[(Q + 1) / 2];
+};
+
+template <class Lat, int Q, int TT>
+__device__ __forceinline__ PullPlan<Q> make_pull_plan(int c, const int8_t *code) {
+  PullPlan<Q> p;
+  const int rx = c / TILE, ry = c % TILE;
+#pragma unroll
+  for (int i = 0; i < (Q + 1) / 2; ++i) p.w[i] = 0u;
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    const int sx = rx - Lat::cx(d), sy = ry - Lat::cy(d);
+    const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
+    uint32_t off = (uint32_t)(d * TT + (in ? sx * TILE + sy : c)), sg = 1u;
+    const int cc = code != nullptr ? code[d] : 0;
+    if (cc == XB_RULE_ZERO) {
+      sg = 0u;
+    } else if (cc != 0) {
+      off = (uint32_t)(((cc > 0 ? cc : -cc) - 1) * TT + c);
+      sg = cc > 0 ? 1u : 2u;
+    }
+    p.w[d >> 1] |= (off | (sg << 14)) << ((d & 1) * 16);
+  }
+  return p;
+}
+
+template <typename T, int Q>
+__device__ __forceinline__ void pull_with_plan(const T *lds, const PullPlan<Q> &p, T (&v)[Q]) {
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    const uint32_t e = (p.w[d >> 1] >> ((d & 1) * 16)) & 0xFFFFu;
+    const T x = lds[e & 0x3FFFu];
+    const uint32_t sg = e >> 14;
+    v[d] = sg == 1u ? x : (sg == 2u ? -x : T(0));
+  }
+}
+
 template <typename T, class Coll, int BC>
 __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int tiles_y, const Coll &coll,
                                                 const T *__restrict__ fin, T *__restrict__ fout, const BCDev<T> &bc,
@@ -925,13 +969,21 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
     slot[k] = HAS_BC ? bc.slot[cell] : -1;
   }
   bool walls = false; // BC == 2: this tile holds a cell with rules (uniform over the block)
+  PullPlan<Q> plan[TILE_CPT];
+  int cslot[TILE_CPT];
   if constexpr (BC == 2) {
     bool mine = false;
 #pragma unroll
     for (int k = 0; k < TILE_CPT; ++k) mine = mine || slot[k] >= 0;
     walls = __syncthreads_or(mine) != 0;
 #pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) slot[k] = slot[k] >= 0 ? slot[k] : bc.nslots;
+    for (int k = 0; k < TILE_CPT; ++k) {
+      const int c = threadIdx.x + k * TILE_THREADS;
+      const int8_t *code = (slot[k] >= 0 && bc.code_ps != nullptr) ? sh->code_ps + slot[k] * Q : nullptr;
+      plan[k] = make_pull_plan<Coll, Q, TT>(c, code);
+      cslot[k] = code != nullptr ? slot[k] : -1;
+      slot[k] = slot[k] >= 0 ? slot[k] : bc.nslots;
+    }
   }
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
@@ -965,17 +1017,23 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
       const int c = threadIdx.x + k * TILE_THREADS;
       const int rx = c / TILE, ry = c % TILE;
       T v[Q];
+      if constexpr (BC == 2) {
+        // the post-streaming rules are folded into the pull; rule cells add their constants
+        pull_with_plan<T, Q>(lds, plan[k], v);
+        if (cslot[k] >= 0) {
+          const T *cst = sh->const_ps + cslot[k] * Q;
 #pragma unroll
-      for (int d = 0; d < Q; ++d) {
-        const int sx = rx - Coll::cx(d), sy = ry - Coll::cy(d);
-        const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
-        v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
+          for (int d = 0; d < Q; ++d) v[d] = v[d] + cst[d];
+        }
+      } else {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+          const int sx = rx - Coll::cx(d), sy = ry - Coll::cy(d);
+          const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
+          v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
+        }
       }
       if constexpr (BC == 2) {
-        if (walls && bc.code_ps != nullptr) {
-          const int sl = slot[k];
-          rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + sl * Q, sh->const_ps + sl * Q);
-        }
       } else if constexpr (HAS_BC) {
         const int sl = slot[k];
         if (sl >= 0 && bc.code_ps != nullptr)
