@@ -69,13 +69,13 @@ def timed_steps(torch, run, f, steps):
     return f, time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3
 
 
-def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True, tile_steps=0):
+def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True, tile_steps=0, tile_threads=0):
     d = darcy_workload(n, precision)
     if tune:
         d.autotune()
     if tile_steps:
         from src import autotune
-        d.step_launch = autotune.tile_choice(tile_steps)
+        d.step_launch = autotune.tile_choice(tile_steps, tile_threads)
     f = d.assign_fields_sharded()
     f = d.run_steps(f, warmup, chunk)
     f, wall, gpu = timed_steps(torch, lambda f, k: d.run_steps(f, k, chunk), f, steps)
@@ -88,8 +88,14 @@ def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True, tile_st
     from src import autotune
     choice, spl, _ = d._multi()
     kind = "tile" if choice > autotune.TILE_BASE else ("sweep2" if choice == autotune.SWEEP2 else "sweep")
+    threads, blocks = None, None
+    if kind == "tile":
+        inner = 32 - 2 * spl
+        blocks = -(-d.nx // inner) * -(-d.ny // inner)
+        # the library's own choice (threads 0): 512 for at most four tiles per CU
+        threads = autotune.tile_threads(choice) or (512 if blocks <= 4 * 256 else 256)
     return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 19 * S,
-                steps_per_launch=spl, launch_kind=kind, solver=d)
+                steps_per_launch=spl, launch_kind=kind, tile_threads=threads, tile_blocks=blocks, solver=d)
 
 
 def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=32):
@@ -595,9 +601,11 @@ def only(torch, args):
         log(f"copy: {args.steps} launches of {a.numel() * 8} B")
     elif args.only == "c2":
         from src import autotune
-        r = bench_darcy(torch, args.n, "f64/f64", args.steps, 2, chunk=16, tile_steps=args.tile_steps)
+        r = bench_darcy(torch, args.n, "f64/f64", args.steps, 2, chunk=16, tile_steps=args.tile_steps,
+                        tile_threads=args.tile_threads)
         d = r["solver"]
-        log(f"c2 {d.nx}x{d.ny}: {r['launch_kind']} launches of {r['steps_per_launch']} steps, "
+        log(f"c2 {d.nx}x{d.ny}: {r['launch_kind']} launches of {r['steps_per_launch']} steps "
+            f"({r['tile_threads']} threads per tile), "
             f"{r['kernel_s'] * 1e6:.2f} us/launch, {r['kernel_s'] * 1e6 / r['steps_per_launch']:.2f} us/step")
     elif args.only == "biot_vcycle":
         r = bench_biot_vcycle(torch, reps=args.steps)
@@ -610,10 +618,15 @@ def only(torch, args):
         print(json.dumps(bench_biot_adjoint(torch, n=args.n, steps=args.steps)), flush=True)
     elif args.only == "large_elast":
         r = bench_large_elasticity(torch, n, args.steps, 2)
-        log(f"elasticity {n}^2: {r['kernel_s'] * 1e6:.1f} us/launch, {r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
+        log(f"elasticity {n}^2: {r['steps_per_launch']} steps per launch, {r['kernel_s'] * 1e6:.1f} us/launch, "
+            f"{r['kernel_s'] * 1e6 / r['steps_per_launch']:.1f} us/step, {r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
     else:
-        r = bench_darcy(torch, n - 2, "f64/f64", args.steps, 2, chunk=16)
-        log(f"darcy {n}^2: {r['kernel_s'] * 1e6:.1f} us/launch, {r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
+        r = bench_darcy(torch, n - 2, "f64/f64", args.steps, 2, chunk=16, tile_steps=args.tile_steps,
+                        tile_threads=args.tile_threads)
+        log(f"darcy {n}^2: {r['launch_kind']} launches of {r['steps_per_launch']} steps "
+            f"({r['tile_threads']} threads per tile), "
+            f"{r['kernel_s'] * 1e6:.1f} us/launch, {r['kernel_s'] * 1e6 / r['steps_per_launch']:.1f} us/step, "
+            f"{r['bytes'] / r['kernel_s'] / 1e9:.0f} GB/s")
 
 
 def main():
@@ -627,7 +640,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched Biot extra")
     ap.add_argument("--tile-steps", type=int, default=0,
-                    help="with --only c2: steps per launch of the LDS-tile kernel instead of the tuned launch")
+                    help="with --only c2 or large_darcy: steps per launch of the LDS-tile kernel instead of the tuned launch")
+    ap.add_argument("--tile-threads", type=int, default=0, choices=[0, 256, 512],
+                    help="with --tile-steps: threads per tile block (0: the library's choice)")
     ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint", "biot_vcycle",
                              "biot_adjoint"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
@@ -654,9 +669,8 @@ def main():
     kind, m = r["launch_kind"], r["steps_per_launch"]
     if kind == "tile":
         rows = None
-        inner = 32 - 2 * m
-        blocks = -(-d.nx // inner) * -(-d.ny // inner)
-        kname = "k_tile_steps<double, xb::DarcyColl<double>, 2>"
+        blocks = r["tile_blocks"]
+        kname = f"k_tile_steps<double, xb::DarcyColl<double>, 2, {r['tile_threads']}>"
     else:
         two = kind == "sweep2"
         rows = autotune.lookup(d._tune_key(two))
@@ -679,6 +693,7 @@ def main():
                                         "calibrated on a copy (profiles/darcy_c2_514_fp64_hbm_traffic.md)")
                      if traffic else "not profiled for this launch grid",
                      "kernel": kname, "launch_kind": kind, "launch_blocks": blocks, "rows_per_wavefront": rows,
+                     "threads_per_block": r["tile_threads"],
                      "kernel_us": round(r["kernel_s"] * 1e6, 3), "steps_per_launch": r["steps_per_launch"],
                      "bytes_per_cell_per_launch": 152, "cells": r["cells"],
                      "per_step_equivalent_gbs": round(achieved * r["steps_per_launch"], 1)},

@@ -140,7 +140,9 @@ int xb_elast_step2(const xb_elast_args *args, void *stream);
 
 /* `steps` (1..8) consecutive plain steps in one launch on 32 x 32 LDS tiles
  * (temporal blocking; bit-identical to `steps` XB_MODE_STEP calls).  The same
- * restrictions as xb_elast_step2. */
+ * restrictions as xb_elast_step2.  Here args->rows is the tile block size: 256 or
+ * 512 threads per tile; any other value lets the library choose (512 for grids of
+ * at most four tiles per CU, else 256).  Results do not depend on it. */
 int xb_elast_stepn(const xb_elast_args *args, int32_t steps, void *stream);
 
 /* `steps` consecutive XB_MODE_SMOOTH sweeps of one multigrid level in one launch on
@@ -358,7 +360,8 @@ int xb_darcy_step(const xb_darcy_args *args, void *stream);
 int xb_darcy_step2(const xb_darcy_args *args, void *stream);
 
 /* `steps` (1..8) consecutive Darcy steps in one launch on 32 x 32 LDS tiles
- * (temporal blocking; bit-identical to `steps` xb_darcy_step calls; no fpc_out). */
+ * (temporal blocking; bit-identical to `steps` xb_darcy_step calls; no fpc_out).
+ * args->rows is the tile block size, as in xb_elast_stepn. */
 int xb_darcy_stepn(const xb_darcy_args *args, int32_t steps, void *stream);
 
 /* p = sum_d f_d + (s_a + s_b)/2 ; grad = sum_d f_d (-3 omega) c_d.

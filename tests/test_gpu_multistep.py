@@ -27,21 +27,23 @@ def _single(step, f, n):
 @pytest.mark.parametrize("n,precision,hetero", [(514, "f64/f64", True), (70, "f32/f32", True), (16, "f64/f64", False),
                                                (5, "f64/f64", False)])
 @pytest.mark.parametrize("m", [1, 3, 4, 8])
-def test_darcy_stepn_matches_single_steps(n, precision, hetero, m):
-    say("darcy", n, precision, "hetero", hetero, "steps per launch", m)
+@pytest.mark.parametrize("threads", [256, 512])
+def test_darcy_stepn_matches_single_steps(n, precision, hetero, m, threads):
+    say("darcy", n, precision, "hetero", hetero, "steps per launch", m, "threads per tile", threads)
     d = P.darcy_c2("gpu", n=n - 2, precision=precision, heterogeneous=hetero, seed=0, variance=1.0, corr=4.0)
     g = torch.Generator().manual_seed(1)
     f = (1.0 + 0.1 * torch.randn((9, d.nx, d.ny), generator=g, dtype=torch.float64)).to(d._tdt).cuda()
     src = (1e-3 * torch.randn((1, d.nx, d.ny), generator=g, dtype=torch.float64)).to(d._tdt).cuda()
     ref = _single(lambda a, b: d._step_soa(a, src, False, out=b), f, m)
-    got = d._stepn_into(f, torch.empty_like(f), m, src)
+    got = d._stepn_into(f, torch.empty_like(f), m, src, threads=threads)
     assert torch.equal(got, ref), (got - ref).abs().max().item()
 
 
 @pytest.mark.parametrize("case", ["periodic", "walls", "fields"])
 @pytest.mark.parametrize("m", [2, 5, 8])
-def test_elast_stepn_matches_single_steps(case, m):
-    say("elasticity", case, "steps per launch", m)
+@pytest.mark.parametrize("threads", [256, 512])
+def test_elast_stepn_matches_single_steps(case, m, threads):
+    say("elasticity", case, "steps per launch", m, "threads per tile", threads)
     g = torch.Generator().manual_seed(2)
     if case == "periodic":
         el = P.periodic_elasticity("gpu", nx=64)
@@ -54,7 +56,7 @@ def test_elast_stepn_matches_single_steps(case, m):
     f = (1e-3 * torch.randn((8, el.nx, el.ny), generator=g, dtype=torch.float64)).to(el._tdt).cuda()
     ext = (1e-4 * torch.randn((2, el.nx, el.ny), generator=g, dtype=torch.float64)).to(el._tdt).cuda()
     ref = _single(lambda a, b: el._step_into(a, b, ext), f, m)
-    got = el._stepn_into(f, torch.empty_like(f), m, ext)
+    got = el._stepn_into(f, torch.empty_like(f), m, ext, threads)
     assert torch.equal(got, ref), (got - ref).abs().max().item()
 
 

@@ -24,13 +24,15 @@ template <typename T> __device__ __forceinline__ T from_prev_lane(T v) { return 
 template <typename T> __device__ __forceinline__ T from_next_lane(T v) { return __shfl_down(v, 1u, WAVE); }
 
 // Blocks are dealt round-robin to the 8 XCDs; renumber them so that consecutive
-// logical blocks (neighbouring strips and row blocks, which share halo rows)
-// land on the same XCD and therefore the same L2.
+// logical blocks (neighbouring strips, row blocks and tiles, which share halo cells)
+// land on the same XCD and therefore the same L2.  XCD x (blocks b = x mod 8) takes
+// the run of logical blocks [x q + min(x, r), ...) of length q + (x < r), where
+// nb = 8 q + r: a bijection of [0, nb) for every grid size.
 __device__ __forceinline__ int xcd_block_id() {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
-  if ((nb & 7) != 0) return b;
-  return (b & 7) * (nb >> 3) + (b >> 3);
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + (b >> 3);
 }
 
 template <typename T> struct BCDev {

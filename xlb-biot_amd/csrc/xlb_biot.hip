@@ -850,6 +850,12 @@ template <typename T> BCDev<T> bcdev(const xb_bc_table *t) {
 constexpr int TILE = 32;
 constexpr int TILE_THREADS = 256;
 constexpr int TILE_CPT = TILE * TILE / TILE_THREADS;
+// Multi-step tiles also run as blocks of 512 threads (2 cells each): fewer registers
+// per thread and twice the waves per tile, for grids of a few tiles per CU, where a
+// tile's latency is the launch time.  Blocks of 256 threads (4 cells each) keep two
+// blocks per CU (their register budget is capped at 256), whose loads and barriers
+// overlap on grids of many tiles per CU.
+constexpr int TILE_THREADS_WIDE = 512;
 constexpr int TILE_MAX_STEPS = 8;
 
 __device__ __forceinline__ int wrapm(int i, int n) {
@@ -931,16 +937,117 @@ __device__ __forceinline__ void pull_with_plan(const T *lds, const PullPlan<Q> &
   }
 }
 
-template <typename T, class Coll, int BC>
+// Streaming inside a tile: population d of cell c comes from the upstream cell's
+// slot, or from the cell's own slot at the tile border (a halo cell, whose value is
+// not written out).
+template <typename T, class Lat>
+__device__ __forceinline__ void tile_pull(const T *lds, int c, T (&v)[Lat::Q]) {
+  constexpr int TT = TILE * TILE;
+  const int rx = c / TILE, ry = c % TILE;
+#pragma unroll
+  for (int d = 0; d < Lat::Q; ++d) {
+    const int sx = rx - Lat::cx(d), sy = ry - Lat::cy(d);
+    const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
+    v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
+  }
+}
+
+// The steps of one tile.  Blocks of 512 threads (PLAN): tiles that hold a rule cell of
+// a table in LDS run their own copy of the loop (WALLS), where the post-collision
+// rules read their sources through the cells' own LDS slots and the post-streaming
+// rules are folded into a per-cell pull plan; the other tiles run the plain loop, in
+// which no plan registers are live.  Blocks of 256 threads (four cells per thread,
+// whose register budget has no room for plans): one loop for all tiles, in which the
+// tiles that hold a rule cell (``walls``, uniform per block) apply both kinds of
+// rules through the cells' own LDS slots.  The plain pull's LDS addresses are
+// immediate offsets from the cell's own.  BC == 1 (a table in global memory): per-cell
+// rules through the own slots.
+template <typename T, class Coll, int BC, bool WALLS, int NT, int CPT = TILE * TILE / NT>
+__device__ __forceinline__ void tile_step_loop(int steps, const Coll &coll, T (&f)[CPT][Coll::Q],
+                                               const typename Coll::Aux (&aux)[CPT], const int (&slot)[CPT],
+                                               bool walls, const BCDev<T> &bc, T *lds, TileRules<T, Coll::Q> *sh) {
+  constexpr int Q = Coll::Q;
+  constexpr int TT = TILE * TILE;
+  PullPlan<Q> plan[WALLS ? CPT : 1];
+  int cslot[WALLS ? CPT : 1];
+  int row[CPT]; // BC == 2: the cell's rule row (cells without rules: the identity row)
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    row[k] = slot[k] >= 0 ? slot[k] : bc.nslots;
+    if constexpr (WALLS) {
+      const int c = threadIdx.x + k * NT;
+      const int8_t *code = (slot[k] >= 0 && bc.code_ps != nullptr) ? sh->code_ps + slot[k] * Q : nullptr;
+      plan[k] = make_pull_plan<Coll, Q, TT>(c, code);
+      cslot[k] = code != nullptr ? slot[k] : -1;
+    }
+  }
+  for (int s = 0; s < steps; ++s) {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int c = threadIdx.x + k * NT;
+      T pc[Q];
+      coll.collide(f[k], aux[k], pc);
+      // post-collision rules: their sources, the cell's pre-collision populations, go
+      // through its own LDS slots (only this thread touches them before the barrier)
+      if constexpr (BC == 2) {
+        if ((WALLS || walls) && bc.code_pc != nullptr) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) lds[d * TT + c] = f[k][d];
+          rules_lds<T, Q, TT>(pc, lds + c, sh->code_pc + row[k] * Q, sh->const_pc + row[k] * Q);
+        }
+      } else if constexpr (BC == 1) {
+        const int sl = slot[k];
+        if (sl >= 0 && bc.code_pc != nullptr) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) lds[d * TT + c] = f[k][d];
+          rules_lds<T, Q, TT>(pc, lds + c, bc.code_pc + (size_t)sl * Q, bc.const_pc + (size_t)sl * Q);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < Q; ++d) lds[d * TT + c] = pc[d];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int c = threadIdx.x + k * NT;
+      if constexpr (WALLS) {
+        // the post-streaming rules are folded into the pull; rule cells add their constants
+        pull_with_plan<T, Q>(lds, plan[k], f[k]);
+        if (cslot[k] >= 0) {
+          const T *cst = sh->const_ps + cslot[k] * Q;
+#pragma unroll
+          for (int d = 0; d < Q; ++d) f[k][d] = f[k][d] + cst[d];
+        }
+      } else {
+        T v[Q];
+        tile_pull<T, Coll>(lds, c, v);
+        if constexpr (BC == 2) {
+          if (walls && bc.code_ps != nullptr)
+            rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + row[k] * Q, sh->const_ps + row[k] * Q);
+        } else if constexpr (BC == 1) {
+          const int sl = slot[k];
+          if (sl >= 0 && bc.code_ps != nullptr)
+            rules_lds<T, Q, TT>(v, lds + c, bc.code_ps + (size_t)sl * Q, bc.const_ps + (size_t)sl * Q);
+        }
+#pragma unroll
+        for (int d = 0; d < Q; ++d) f[k][d] = v[d];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, class Coll, int BC, int NT>
 __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int tiles_y, const Coll &coll,
                                                 const T *__restrict__ fin, T *__restrict__ fout, const BCDev<T> &bc,
                                                 T *lds, TileRules<T, Coll::Q> *sh) {
   constexpr int Q = Coll::Q;
+  constexpr int CPT = TILE * TILE / NT;
   constexpr bool HAS_BC = BC != 0;
-  constexpr int TT = TILE * TILE;
   using Aux = typename Coll::Aux;
   const int inner = TILE - 2 * steps;
-  const int bx = blockIdx.x / tiles_y, by = blockIdx.x % tiles_y;
+  const int tile = xcd_block_id();
+  const int bx = tile / tiles_y, by = tile % tiles_y;
   const int x0 = bx * inner - steps, y0 = by * inner - steps;
   const size_t N = (size_t)nx * ny;
   if constexpr (BC == 2) {
@@ -955,12 +1062,12 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
       sh->const_ps[k] = (!id && bc.const_ps != nullptr) ? bc.const_ps[k] : T(0);
     }
   }
-  T f[TILE_CPT][Q];
-  Aux aux[TILE_CPT];
-  int slot[TILE_CPT];
+  T f[CPT][Q];
+  Aux aux[CPT];
+  int slot[CPT];
 #pragma unroll
-  for (int k = 0; k < TILE_CPT; ++k) {
-    const int c = threadIdx.x + k * TILE_THREADS;
+  for (int k = 0; k < CPT; ++k) {
+    const int c = threadIdx.x + k * NT;
     const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
     const size_t cell = (size_t)gx * ny + gy;
 #pragma unroll
@@ -968,85 +1075,27 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
     aux[k] = coll.load_aux(cell);
     slot[k] = HAS_BC ? bc.slot[cell] : -1;
   }
-  bool walls = false; // BC == 2: this tile holds a cell with rules (uniform over the block)
-  PullPlan<Q> plan[TILE_CPT];
-  int cslot[TILE_CPT];
   if constexpr (BC == 2) {
+    // whether this tile holds a cell with rules (uniform over the block; the barrier
+    // also orders the table staging above before its reads)
     bool mine = false;
 #pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) mine = mine || slot[k] >= 0;
-    walls = __syncthreads_or(mine) != 0;
-#pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) {
-      const int c = threadIdx.x + k * TILE_THREADS;
-      const int8_t *code = (slot[k] >= 0 && bc.code_ps != nullptr) ? sh->code_ps + slot[k] * Q : nullptr;
-      plan[k] = make_pull_plan<Coll, Q, TT>(c, code);
-      cslot[k] = code != nullptr ? slot[k] : -1;
-      slot[k] = slot[k] >= 0 ? slot[k] : bc.nslots;
+    for (int k = 0; k < CPT; ++k) mine = mine || slot[k] >= 0;
+    const bool walls = __syncthreads_or(mine) != 0;
+    if constexpr (NT == TILE_THREADS_WIDE) {
+      if (walls)
+        tile_step_loop<T, Coll, 2, true, NT>(steps, coll, f, aux, slot, true, bc, lds, sh);
+      else
+        tile_step_loop<T, Coll, 2, false, NT>(steps, coll, f, aux, slot, false, bc, lds, sh);
+    } else {
+      tile_step_loop<T, Coll, 2, false, NT>(steps, coll, f, aux, slot, walls, bc, lds, sh);
     }
-  }
-  for (int s = 0; s < steps; ++s) {
-#pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) {
-      const int c = threadIdx.x + k * TILE_THREADS;
-      T pc[Q];
-      coll.collide(f[k], aux[k], pc);
-      // post-collision rules: their sources, the cell's pre-collision populations, go
-      // through its own LDS slots (only this thread touches them before the barrier)
-      if constexpr (BC == 2) {
-        if (walls && bc.code_pc != nullptr) {
-          const int sl = slot[k];
-#pragma unroll
-          for (int d = 0; d < Q; ++d) lds[d * TT + c] = f[k][d];
-          rules_lds<T, Q, TT>(pc, lds + c, sh->code_pc + sl * Q, sh->const_pc + sl * Q);
-        }
-      } else if constexpr (HAS_BC) {
-        const int sl = slot[k];
-        if (sl >= 0 && bc.code_pc != nullptr) {
-#pragma unroll
-          for (int d = 0; d < Q; ++d) lds[d * TT + c] = f[k][d];
-          rules_lds<T, Q, TT>(pc, lds + c, bc.code_pc + (size_t)sl * Q, bc.const_pc + (size_t)sl * Q);
-        }
-      }
-#pragma unroll
-      for (int d = 0; d < Q; ++d) lds[d * TT + c] = pc[d];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) {
-      const int c = threadIdx.x + k * TILE_THREADS;
-      const int rx = c / TILE, ry = c % TILE;
-      T v[Q];
-      if constexpr (BC == 2) {
-        // the post-streaming rules are folded into the pull; rule cells add their constants
-        pull_with_plan<T, Q>(lds, plan[k], v);
-        if (cslot[k] >= 0) {
-          const T *cst = sh->const_ps + cslot[k] * Q;
-#pragma unroll
-          for (int d = 0; d < Q; ++d) v[d] = v[d] + cst[d];
-        }
-      } else {
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-          const int sx = rx - Coll::cx(d), sy = ry - Coll::cy(d);
-          const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
-          v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
-        }
-      }
-      if constexpr (BC == 2) {
-      } else if constexpr (HAS_BC) {
-        const int sl = slot[k];
-        if (sl >= 0 && bc.code_ps != nullptr)
-          rules_lds<T, Q, TT>(v, lds + c, bc.code_ps + (size_t)sl * Q, bc.const_ps + (size_t)sl * Q);
-      }
-#pragma unroll
-      for (int d = 0; d < Q; ++d) f[k][d] = v[d];
-    }
-    __syncthreads();
+  } else {
+    tile_step_loop<T, Coll, BC, false, NT>(steps, coll, f, aux, slot, false, bc, lds, sh);
   }
 #pragma unroll
-  for (int k = 0; k < TILE_CPT; ++k) {
-    const int c = threadIdx.x + k * TILE_THREADS;
+  for (int k = 0; k < CPT; ++k) {
+    const int c = threadIdx.x + k * NT;
     const int rx = c / TILE, ry = c % TILE;
     if (rx < steps || rx >= TILE - steps || ry < steps || ry >= TILE - steps) continue;
     const int gx = x0 + rx, gy = y0 + ry;
@@ -1057,33 +1106,50 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
   }
 }
 
-template <typename T, class Coll, int BC>
-__global__ void __launch_bounds__(TILE_THREADS)
+template <typename T, class Coll, int BC, int NT>
+__global__ void __launch_bounds__(NT, NT == TILE_THREADS ? 2 : 1)
 k_tile_steps(int nx, int ny, int steps, int tiles_y, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
              const BCDev<T> bc) {
   __shared__ T lds[Coll::Q * TILE * TILE];
   if constexpr (BC == 2) {
     __shared__ TileRules<T, Coll::Q> sh;
-    tile_steps_body<T, Coll, BC>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, &sh);
+    tile_steps_body<T, Coll, BC, NT>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, &sh);
   } else {
-    tile_steps_body<T, Coll, BC>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, nullptr);
+    tile_steps_body<T, Coll, BC, NT>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, nullptr);
+  }
+}
+
+// Threads per tile block: 256 or 512 as asked; otherwise 512 for grids of at most
+// four tiles per CU (256 CUs), else 256.
+inline int tile_block_threads(int asked, int tiles) {
+  if (asked == TILE_THREADS || asked == TILE_THREADS_WIDE) return asked;
+  return tiles <= 4 * 256 ? TILE_THREADS_WIDE : TILE_THREADS;
+}
+
+template <typename T, class Coll, int NT>
+void launch_tile_steps_nt(dim3 grid, int nx, int ny, int steps, int ty, const Coll &coll, const T *fi, T *fo,
+                          const BCDev<T> &bc, hipStream_t st) {
+  const dim3 block(NT);
+  switch (bc_mode(bc.slot, bc.nslots)) {
+  case 2: hipLaunchKernelGGL((k_tile_steps<T, Coll, 2, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
+  case 1: hipLaunchKernelGGL((k_tile_steps<T, Coll, 1, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
+  default: hipLaunchKernelGGL((k_tile_steps<T, Coll, 0, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
   }
 }
 
 template <typename T, class Coll>
 int launch_tile_steps(int nx, int ny, int steps, const Coll &coll, const void *fin, void *fout,
-                      const xb_bc_table *bct, hipStream_t st) {
+                      const xb_bc_table *bct, int threads, hipStream_t st) {
   const BCDev<T> bc = bcdev<T>(bct);
   const int inner = TILE - 2 * steps;
   const int tx = cdiv(nx, inner), ty = cdiv(ny, inner);
-  const dim3 grid((unsigned)(tx * ty)), block(TILE_THREADS);
+  const dim3 grid((unsigned)(tx * ty));
   const T *fi = static_cast<const T *>(fin);
   T *fo = static_cast<T *>(fout);
-  switch (bc_mode(bc.slot, bc.nslots)) {
-  case 2: hipLaunchKernelGGL((k_tile_steps<T, Coll, 2>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
-  case 1: hipLaunchKernelGGL((k_tile_steps<T, Coll, 1>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
-  default: hipLaunchKernelGGL((k_tile_steps<T, Coll, 0>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
-  }
+  if (tile_block_threads(threads, tx * ty) == TILE_THREADS_WIDE)
+    launch_tile_steps_nt<T, Coll, TILE_THREADS_WIDE>(grid, nx, ny, steps, ty, coll, fi, fo, bc, st);
+  else
+    launch_tile_steps_nt<T, Coll, TILE_THREADS>(grid, nx, ny, steps, ty, coll, fi, fo, bc, st);
   return check_launch("multi-step tile kernel");
 }
 
@@ -1607,7 +1673,7 @@ __device__ __forceinline__ void tile_smooth_run(const TileSmoothDev<T> &A, int t
 
 template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
 __global__ void __launch_bounds__(TILE_THREADS) k_tile_smooth(const TileSmoothDev<T> A) {
-  tile_smooth_run<T, BC, HAS_RES, HAS_NL, GWALLS>(A, (int)blockIdx.x);
+  tile_smooth_run<T, BC, HAS_RES, HAS_NL, GWALLS>(A, xcd_block_id());
 }
 
 // Batched form: every block looks up its job (one level of one instance) and tile.
@@ -3386,7 +3452,7 @@ int xb_darcy_stepn(const xb_darcy_args *a, int32_t steps, void *stream) {
     c.om_field = static_cast<const double *>(a->omega_field);
     c.sa = static_cast<const double *>(a->source_a);
     c.sb = static_cast<const double *>(a->source_b);
-    return launch_tile_steps<double>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, st);
+    return launch_tile_steps<double>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, a->rows, st);
   }
   DarcyColl<float> c;
   c.omega = (float)a->omega;
@@ -3394,11 +3460,17 @@ int xb_darcy_stepn(const xb_darcy_args *a, int32_t steps, void *stream) {
   c.om_field = static_cast<const float *>(a->omega_field);
   c.sa = static_cast<const float *>(a->source_a);
   c.sb = static_cast<const float *>(a->source_b);
-  return launch_tile_steps<float>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, st);
+  return launch_tile_steps<float>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, a->rows, st);
 }
 
 int xb_elast_stepn(const xb_elast_args *a, int32_t steps, void *stream) {
-  if (int r = validate_elast(a, "xb_elast_stepn")) return r;
+  if (a == nullptr) return fail(XB_EINVAL, "xb_elast_stepn: null args");
+  {
+    // rows is the tile block size here (any value), not the sweep launch shape
+    xb_elast_args b = *a;
+    b.rows = 0;
+    if (int r = validate_elast(&b, "xb_elast_stepn")) return r;
+  }
   if (a->mode != XB_MODE_STEP || a->fpc_out != nullptr || a->resid != nullptr || a->div_in != nullptr)
     return fail(XB_EUNSUPPORTED, "xb_elast_stepn: plain steps only (no post-collision output, residual or "
                 "coupling force)");
@@ -3409,12 +3481,12 @@ int xb_elast_stepn(const xb_elast_args *a, int32_t steps, void *stream) {
     ElastColl<double> c;
     NLDev<double> nl;
     elast_coll<double>(a, c, nl);
-    return launch_tile_steps<double>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, st);
+    return launch_tile_steps<double>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, a->rows, st);
   }
   ElastColl<float> c;
   NLDev<float> nl;
   elast_coll<float>(a, c, nl);
-  return launch_tile_steps<float>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, st);
+  return launch_tile_steps<float>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, a->rows, st);
 }
 
 int xb_elast_smoothn(const xb_elast_args *a, int32_t steps, void *stream) {

@@ -83,17 +83,26 @@ def tune(key, launch_with_rows, candidates=CANDIDATES, reps=20):
 # Plain-step launch kinds (the value cached under a solver's "/multi" key): one step
 # per sweep, two per sweep (register temporal blocking), or m steps on LDS tiles.
 SWEEP1, SWEEP2 = 1, 2
+# tile launches: TILE_BASE * k + m steps per launch, k = 1 the library's block size,
+# k = 2 blocks of 256 threads, k = 3 blocks of 512 threads
 TILE_BASE = 100
 TILE_STEPS = (2, 3, 4, 6, 8)
+TILE_THREADS = (256, 512)
 
 
-def tile_choice(m):
-    return TILE_BASE + int(m)
+def tile_choice(m, threads=0):
+    k = {0: 1, 256: 2, 512: 3}[int(threads)]
+    return TILE_BASE * k + int(m)
 
 
 def steps_per_launch(choice):
     """Steps one launch of ``choice`` advances."""
-    return choice - TILE_BASE if choice > TILE_BASE else choice
+    return choice % TILE_BASE if choice > TILE_BASE else choice
+
+
+def tile_threads(choice):
+    """Threads per tile block of a tile ``choice`` (0: the library's choice)."""
+    return (0, 0, 256, 512)[choice // TILE_BASE]
 
 
 def tune_multi(key, launches, reps=20):

@@ -175,15 +175,18 @@ class AdvectionDiffusionReactionBGK(LBMBase):
             out[key2] = autotune.tune(key2, run2)
             launches[autotune.SWEEP2] = lambda: self._step2_into(f, o)
         for m in autotune.TILE_STEPS:
-            launches[autotune.tile_choice(m)] = (lambda m=m: self._stepn_into(f, o, m))
+            for nt in autotune.TILE_THREADS:
+                launches[autotune.tile_choice(m, nt)] = (lambda m=m, nt=nt: self._stepn_into(f, o, m, threads=nt))
         km = self._tune_key() + "/multi"
         out[km] = autotune.tune_multi(km, launches)
         return out
 
-    def _stepn_into(self, f, out, m, ext_source=None):
-        """``m`` steps in one launch on LDS tiles (``xb_darcy_stepn``)."""
-        nat.check(nat.lib().xb_darcy_stepn(self._args(f, out, ext_source, None), int(m), nat.stream()),
-                  "xb_darcy_stepn")
+    def _stepn_into(self, f, out, m, ext_source=None, threads=0):
+        """``m`` steps in one launch on LDS tiles (``xb_darcy_stepn``) of ``threads``
+        threads per block (0: the library's choice)."""
+        a = self._args(f, out, ext_source, None)
+        a.rows = int(threads)
+        nat.check(nat.lib().xb_darcy_stepn(a, int(m), nat.stream()), "xb_darcy_stepn")
         return out
 
     def _multi(self):
@@ -195,7 +198,8 @@ class AdvectionDiffusionReactionBGK(LBMBase):
             choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
         m = autotune.steps_per_launch(choice)
         if choice > autotune.TILE_BASE:
-            return choice, m, (lambda s, d: self._stepn_into(s, d, m))
+            nt = autotune.tile_threads(choice)
+            return choice, m, (lambda s, d: self._stepn_into(s, d, m, threads=nt))
         if choice == autotune.SWEEP2:
             return choice, 2, (lambda s, d: self._step2_into(s, d))
         return choice, 1, None

@@ -422,9 +422,11 @@ class LBMElasticity(object):
         nat.check(nat.lib().xb_elast_step2(a, nat.stream()), "xb_elast_step2")
         return out
 
-    def _stepn_into(self, f, out, m, ext=None):
-        """``m`` single-grid steps in one launch on LDS tiles (``xb_elast_stepn``)."""
+    def _stepn_into(self, f, out, m, ext=None, threads=0):
+        """``m`` single-grid steps in one launch on LDS tiles (``xb_elast_stepn``) of
+        ``threads`` threads per block (0: the library's choice)."""
         a = self._launch(f, out, 0, nat.XB_MODE_STEP, ext=ext)
+        a.rows = int(threads)
         nat.check(nat.lib().xb_elast_stepn(a, int(m), nat.stream()), "xb_elast_stepn")
         return out
 
@@ -441,7 +443,8 @@ class LBMElasticity(object):
         if min(self.nx, self.ny) >= 2:
             launches[autotune.SWEEP2] = lambda: self._step2_into(f, o, ext)
         for m in autotune.TILE_STEPS:
-            launches[autotune.tile_choice(m)] = (lambda m=m: self._stepn_into(f, o, m, ext))
+            for nt in autotune.TILE_THREADS:
+                launches[autotune.tile_choice(m, nt)] = (lambda m=m, nt=nt: self._stepn_into(f, o, m, ext, nt))
         key = self._multi_key(external_force)
         return {key: autotune.tune_multi(key, launches)}
 
@@ -451,7 +454,8 @@ class LBMElasticity(object):
             choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
         m = autotune.steps_per_launch(choice)
         if choice > autotune.TILE_BASE:
-            return choice, m, (lambda s, d: self._stepn_into(s, d, m, ext))
+            nt = autotune.tile_threads(choice)
+            return choice, m, (lambda s, d: self._stepn_into(s, d, m, ext, nt))
         if choice == autotune.SWEEP2:
             return choice, 2, (lambda s, d: self._step2_into(s, d, ext))
         return choice, 1, None
