@@ -108,9 +108,15 @@ def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=3
     f, wall, gpu = timed_steps(torch, lambda f, k: el.run_steps(f, k, chunk), f, steps)
     S = 8 if precision == "f64/f64" else 4
     cells = n * n
-    _, spl, _ = el._multi(None)
+    choice, spl, _ = el._multi(None)
+    from src import autotune
+    threads = None
+    if choice > autotune.TILE_BASE:
+        t = -(-n // (32 - 2 * spl))
+        blocks = t * t
+        threads = autotune.tile_threads(choice) or (512 if blocks <= 4 * 256 else 256)
     return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 18 * S,
-                steps_per_launch=spl)
+                steps_per_launch=spl, tile_threads=threads)
 
 
 def residual_solve(torch, d, tol, max_steps, check_every=1000, chunk=64):
@@ -690,7 +696,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and launch grid, "
-                                        "calibrated on a copy (profiles/darcy_c2_514_fp64_hbm_traffic.md)")
+                                        "calibrated on a copy (profiles/darcy_tile_fp64_hbm_traffic.md)")
                      if traffic else "not profiled for this launch grid",
                      "kernel": kname, "launch_kind": kind, "launch_blocks": blocks, "rows_per_wavefront": rows,
                      "threads_per_block": r["tile_threads"],
@@ -738,6 +744,7 @@ def main():
             extras[f"elasticity_{n}_fp64"] = {"mlups": round(rl["cells"] * 200 / rl["elapsed"] / 1e6, 1),
                                               "kernel_us": round(rl["kernel_s"] * 1e6, 2), "gbs": round(gbs, 1),
                                               "steps_per_launch": rl["steps_per_launch"],
+                                              "threads_per_block": rl["tile_threads"],
                                               "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": 144}
             log(f"darcy {n}^2 fp64 (HBM-resident)")
             rd = bench_darcy(torch, n - 2, "f64/f64", 200, 20, chunk=32)
@@ -745,7 +752,11 @@ def main():
             extras[f"darcy_{n}_fp64"] = {"mlups": round(rd["cells"] * 200 / rd["elapsed"] / 1e6, 1),
                                          "kernel_us": round(rd["kernel_s"] * 1e6, 2), "gbs": round(gbs, 1),
                                          "steps_per_launch": rd["steps_per_launch"],
+                                         "launch_kind": rd["launch_kind"], "threads_per_block": rd["tile_threads"],
                                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": 152}
+            if rd["launch_kind"] == "tile":
+                kn = f"k_tile_steps<double, xb::DarcyColl<double>, 2, {rd['tile_threads']}>"
+                extras[f"darcy_{n}_fp64"]["traffic"] = profiled_traffic(kn, rd["tile_blocks"], rd["cells"])
     line["extra"] = extras
     if not args.no_cpu:
         log(f"cpu baseline ({args.cpu_seconds:.0f} s)")
