@@ -305,6 +305,16 @@ typedef struct xb_elast_mg_adjoint_args {
 
 int xb_elast_mg_adjoint(const xb_elast_mg_adjoint_args *args, void *stream);
 
+/* `sweeps` consecutive XB_ADJ_SMOOTH transposed sweeps of a grid of at most 65536
+ * cells, each followed by xb_elast_coupling_adjoint of its force_bar (scale, grad_bc,
+ * the args' lam / mu and fields), in one workgroup: bit-identical to the sequence of
+ * those calls, in 1 launch instead of 2 * sweeps.  abar_out receives the result; tmp
+ * is (8, nx, ny) scratch; abar_in, abar_out and tmp differ; force_bar is written.  No
+ * abar_add and no parameter derivatives (the coarse levels of the transposed
+ * V-cycle).  No reference counterpart (the reference's gradients are jax.grad). */
+int xb_elast_smooth_adjoint_small(const xb_elast_mg_adjoint_args *args, int32_t sweeps, double scale,
+                                  const xb_bc_table *grad_bc, void *tmp, void *stream);
+
 /* Transposed Biot coupling force F = G_scale(D f) of one sweep (the fused force of
  * xb_elast_args.div_in): abar[d] += k w2_d (G^T force_bar) with w2 = 1,1,1,1,2,2,2,2
  * and k the divergence coefficient; with f_in and the fields, the Lame-parameter

@@ -113,6 +113,9 @@ def test_biot_gradient_matches_autograd_through_oracle(alt):
     targets = _misfit_targets(nx, ny, 3)
     p = P.biot_consolidation("gpu", nx=nx, levels=levels, steps=n_inner, k_smooth=k)
     p.alternative_gradient = alt
+    # the recompute takes each V-cycle's coarse correction from the forward (alt False)
+    # or solves the coarse levels again (alt True: no budget for keeping them)
+    p.adjoint_coarse_budget_bytes = 0 if alt else 8e9
     ts = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for v in (om, lam, mu)]
     # K = 2 < 3 V-cycles per step: V-cycle checkpoints inside each step; K = 6:
     # checkpoints every second step, segments of two steps recomputed
@@ -167,3 +170,35 @@ def test_biot_gradient_matches_finite_differences_16_directions():
             say(i, fd, an)
     say("worst relative difference", worst)
     assert worst < 1e-6
+
+
+@pytest.mark.parametrize("case,precision", [("consolidation", "f64/f64"), ("consolidation", "f32/f32"),
+                                            ("periodic", "f64/f64")])
+@pytest.mark.parametrize("level", [1, 2])
+def test_small_grid_transposed_smoothing_matches_sweeps(case, precision, level):
+    """xb_elast_smooth_adjoint_small (a run of transposed smoothing sweeps with their
+    transposed coupling force in one workgroup) is bitwise the sequence of
+    xb_elast_mg_adjoint and xb_elast_coupling_adjoint calls it replaces."""
+    from src import _native as nat
+    from src.biot_adjoint import BiotAdjointSession
+    say("small-grid transposed smoothing", case, precision, "level", level)
+    if case == "consolidation":
+        p = P.biot_consolidation("gpu", nx=32, levels=3, steps=2, k_smooth=4, precision=precision)
+    else:
+        p = P.biot_periodic("gpu", nx=32, levels=3, steps=2, precision=precision)
+    s = BiotAdjointSession(p, 1, 4)
+    el = p.elasticity
+    nx, ny = el.grid_size(level)
+    x0, x1, y0, y1 = el.interior_box(level)
+    g = torch.Generator().manual_seed(7)
+    ab = torch.randn((8, nx, ny), generator=g, dtype=torch.float64).to(el._tdt).cuda()
+    rbar0 = torch.randn((8, x1 - x0, y1 - y0), generator=g, dtype=torch.float64).to(el._tdt).cuda()
+    no_acc = {"lam": None, "mu": None, "base": None}
+    for sweeps in (1, 3, el.k_smooth):
+        rb1, a = rbar0.clone(), ab
+        for _ in range(sweeps):
+            a = s._sweep_adjoint(level, nat.XB_ADJ_SMOOTH, a, None, None, rb1, no_acc)
+        rb2 = rbar0.clone()
+        b = s._smooth_adjoint_small(level, ab, sweeps, rb2)
+        assert torch.equal(a, b), (sweeps, (a - b).abs().max().item())
+        assert torch.equal(rb1, rb2), sweeps

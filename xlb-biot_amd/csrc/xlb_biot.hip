@@ -2183,8 +2183,9 @@ template <typename T> struct ElastMGAdjDev {
   BCDev<T> bc;
 };
 
+// One cell of a transposed smoothing / residual sweep.
 template <typename T, int MODE, bool HAS_BC>
-__global__ void __launch_bounds__(256) k_elast_mg_adjoint(const ElastMGAdjDev<T> A) {
+__device__ __forceinline__ void elast_mg_adjoint_cell(const ElastMGAdjDev<T> &A, size_t i) {
   using C = ElastColl<T>;
   const int nx = A.nx, ny = A.ny;
   const size_t N = (size_t)nx * ny;
@@ -2205,7 +2206,7 @@ __global__ void __launch_bounds__(256) k_elast_mg_adjoint(const ElastMGAdjDev<T>
       return -res_bar(d, (int)(j / ny), (int)(j % ny));
     }
   };
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+  {
     const int x = (int)(i / ny), y = (int)(i % ny);
     T pb[8], ob[8], fpre[8];
     adj_post_collision<T, 8, C, HAS_BC>(vbar, A.bc, nx, ny, x, y, i, pb);
@@ -2257,6 +2258,13 @@ __global__ void __launch_bounds__(256) k_elast_mg_adjoint(const ElastMGAdjDev<T>
   }
 }
 
+template <typename T, int MODE, bool HAS_BC>
+__global__ void __launch_bounds__(256) k_elast_mg_adjoint(const ElastMGAdjDev<T> A) {
+  const size_t N = (size_t)A.nx * A.ny;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x)
+    elast_mg_adjoint_cell<T, MODE, HAS_BC>(A, i);
+}
+
 // Transposed lattice gradient G_s (xb_darcy_gradient with walls): qbar(x) =
 // sum over the stencil slots d of the cells that read q(x) of (-3 s w_d) c_d . gbar.
 // COUPLE: the transposed Biot coupling force F = G_s(D f) of a sweep instead adds
@@ -2275,8 +2283,9 @@ template <typename T> struct GradAdjDev {
   BCDev<T> bc;
 };
 
+// One cell of a transposed gradient / coupling force.
 template <typename T, bool HAS_BC, bool COUPLE>
-__global__ void __launch_bounds__(256) k_grad_adjoint(const GradAdjDev<T> A) {
+__device__ __forceinline__ void grad_adjoint_cell(const GradAdjDev<T> &A, size_t i) {
   using D = DarcyColl<T>;
   const int nx = A.nx, ny = A.ny;
   const size_t N = (size_t)nx * ny;
@@ -2287,7 +2296,7 @@ __global__ void __launch_bounds__(256) k_grad_adjoint(const GradAdjDev<T> A) {
     if (D::cy(d) != 0) v = v + (D::cy(d) > 0 ? cw : -cw) * A.gbar[N + j];
     return v;
   };
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+  {
     const int x = (int)(i / ny), y = (int)(i % ny);
     T pb[9];
     adj_post_collision<T, 9, D, HAS_BC>(slot_bar, A.bc, nx, ny, x, y, i, pb);
@@ -2320,6 +2329,41 @@ __global__ void __launch_bounds__(256) k_grad_adjoint(const GradAdjDev<T> A) {
         A.base_bar[N + i] += (double)A.gbar[N + i];
       }
     }
+  }
+}
+
+template <typename T, bool HAS_BC, bool COUPLE>
+__global__ void __launch_bounds__(256) k_grad_adjoint(const GradAdjDev<T> A) {
+  const size_t N = (size_t)A.nx * A.ny;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x)
+    grad_adjoint_cell<T, HAS_BC, COUPLE>(A, i);
+}
+
+// `sweeps` transposed smoothing sweeps of a small grid, each followed by the
+// transposed coupling force of its force adjoint, in one workgroup (global memory,
+// barriers between the phases): per cell the arithmetic of the sequence of
+// k_elast_mg_adjoint<SMOOTH> and k_grad_adjoint<COUPLE> launches (bit-identical).
+// The sweeps alternate between tmp and A.ab_out so that the last one writes ab_out.
+constexpr int ADJ_SMALL_THREADS = 1024;
+constexpr int ADJ_SMALL_MAX_CELLS = 65536;
+
+template <typename T, bool HAS_BC, bool HAS_GBC>
+__global__ void __launch_bounds__(ADJ_SMALL_THREADS)
+k_smooth_adjoint_small(const ElastMGAdjDev<T> A, const GradAdjDev<T> G, int sweeps, T *tmp) {
+  const size_t N = (size_t)A.nx * A.ny;
+  const T *in = A.ab_in;
+  for (int s = 0; s < sweeps; ++s) {
+    T *out = ((sweeps - 1 - s) & 1) == 0 ? A.ab_out : tmp;
+    ElastMGAdjDev<T> a = A;
+    a.ab_in = in;
+    a.ab_out = out;
+    for (size_t i = threadIdx.x; i < N; i += ADJ_SMALL_THREADS) elast_mg_adjoint_cell<T, XB_ADJ_SMOOTH, HAS_BC>(a, i);
+    __syncthreads();
+    GradAdjDev<T> g = G;
+    g.fbar = out;
+    for (size_t i = threadIdx.x; i < N; i += ADJ_SMALL_THREADS) grad_adjoint_cell<T, HAS_GBC, true>(g, i);
+    __syncthreads();
+    in = out;
   }
 }
 
@@ -2501,7 +2545,7 @@ __global__ void __launch_bounds__(256) k_darcy_adjoint(const DarcyAdjDev<T> A) {
   }
 }
 
-template <typename T> int elast_mg_adjoint_t(const xb_elast_mg_adjoint_args *a, hipStream_t st) {
+template <typename T> ElastMGAdjDev<T> mg_adjoint_dev(const xb_elast_mg_adjoint_args *a) {
   ElastMGAdjDev<T> A{};
   A.nx = a->nx; A.ny = a->ny;
   A.ix0 = a->ix0; A.ix1 = a->ix1; A.iy0 = a->iy0; A.iy1 = a->iy1;
@@ -2518,6 +2562,11 @@ template <typename T> int elast_mg_adjoint_t(const xb_elast_mg_adjoint_args *a, 
   A.g_lam = a->grad_lam;
   A.g_mu = a->grad_mu;
   A.bc = bcdev<T>(a->bc);
+  return A;
+}
+
+template <typename T> int elast_mg_adjoint_t(const xb_elast_mg_adjoint_args *a, hipStream_t st) {
+  const ElastMGAdjDev<T> A = mg_adjoint_dev<T>(a);
   const bool has_bc = A.bc.slot != nullptr;
   const dim3 gr = ew_grid((int64_t)a->nx * a->ny);
   if (a->mode == XB_ADJ_SMOOTH) {
@@ -2559,6 +2608,29 @@ int grad_adjoint_t(int nx, int ny, double scale, double b, double kdiv, const vo
     else hipLaunchKernelGGL((k_grad_adjoint<T, false, false>), gr, dim3(256), 0, st, A);
   }
   return check_launch(couple ? "xb_elast_coupling_adjoint" : "xb_darcy_gradient_adjoint");
+}
+
+template <typename T>
+int smooth_adjoint_small_t(const xb_elast_mg_adjoint_args *a, int sweeps, double scale, const xb_bc_table *gbc,
+                           void *tmp, hipStream_t st) {
+  const ElastMGAdjDev<T> A = mg_adjoint_dev<T>(a);
+  GradAdjDev<T> G{};
+  G.nx = a->nx; G.ny = a->ny;
+  G.m3s = T(-3 * scale);
+  G.b = T(1);
+  G.kdiv = T(-1.5 / (3 * (a->lam + a->mu) + 1));
+  G.gbar = static_cast<const T *>(a->force_bar);
+  G.lamf = static_cast<const T *>(a->lam_field);
+  G.muf = static_cast<const T *>(a->mu_field);
+  G.bc = bcdev<T>(gbc);
+  T *t = static_cast<T *>(tmp);
+  const bool hb = A.bc.slot != nullptr, hg = G.bc.slot != nullptr;
+  const dim3 gr(1), bl(ADJ_SMALL_THREADS);
+  if (hb && hg) hipLaunchKernelGGL((k_smooth_adjoint_small<T, true, true>), gr, bl, 0, st, A, G, sweeps, t);
+  else if (hb) hipLaunchKernelGGL((k_smooth_adjoint_small<T, true, false>), gr, bl, 0, st, A, G, sweeps, t);
+  else if (hg) hipLaunchKernelGGL((k_smooth_adjoint_small<T, false, true>), gr, bl, 0, st, A, G, sweeps, t);
+  else hipLaunchKernelGGL((k_smooth_adjoint_small<T, false, false>), gr, bl, 0, st, A, G, sweeps, t);
+  return check_launch("xb_elast_smooth_adjoint_small");
 }
 
 // ---------------------------------------------------------------------------
@@ -3219,6 +3291,32 @@ int xb_elast_mg_adjoint(const xb_elast_mg_adjoint_args *a, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (a->dtype == XB_F64) return elast_mg_adjoint_t<double>(a, st);
   return elast_mg_adjoint_t<float>(a, st);
+}
+
+int xb_elast_smooth_adjoint_small(const xb_elast_mg_adjoint_args *a, int32_t sweeps, double scale,
+                                  const xb_bc_table *grad_bc, void *tmp, void *stream) {
+  if (a == nullptr) return fail(XB_EINVAL, "xb_elast_smooth_adjoint_small: null args");
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->mode != XB_ADJ_SMOOTH) return fail(XB_EINVAL, "xb_elast_smooth_adjoint_small: needs XB_ADJ_SMOOTH");
+  if (a->nx <= 0 || a->ny <= 0 || (int64_t)a->nx * a->ny > ADJ_SMALL_MAX_CELLS)
+    return fail(XB_EINVAL, "xb_elast_smooth_adjoint_small: grid %d x %d (at most %d cells)", a->nx, a->ny,
+                ADJ_SMALL_MAX_CELLS);
+  if (sweeps < 1) return fail(XB_EINVAL, "xb_elast_smooth_adjoint_small: sweeps must be >= 1");
+  if (a->abar_in == nullptr || a->abar_out == nullptr || tmp == nullptr || a->force_bar == nullptr)
+    return fail(XB_EINVAL, "xb_elast_smooth_adjoint_small: null abar_in, abar_out, tmp or force_bar");
+  if (a->abar_in == a->abar_out || tmp == a->abar_in || tmp == a->abar_out)
+    return fail(XB_EINVAL, "xb_elast_smooth_adjoint_small: abar_in, abar_out and tmp must differ");
+  if (a->abar_add != nullptr || a->grad_lam != nullptr || a->grad_mu != nullptr)
+    return fail(XB_EUNSUPPORTED, "xb_elast_smooth_adjoint_small: no abar_add or parameter derivatives");
+  if (a->ix0 < 0 || a->ix1 > a->nx || a->iy0 < 0 || a->iy1 > a->ny || a->ix0 >= a->ix1 || a->iy0 >= a->iy1)
+    return fail(XB_EINVAL, "xb_elast_smooth_adjoint_small: bad interior box");
+  if ((a->lam_field == nullptr) != (a->mu_field == nullptr))
+    return fail(XB_EINVAL, "xb_elast_smooth_adjoint_small: lam_field and mu_field go together");
+  if (int r = check_bc(a->bc, false)) return r;
+  if (int r = check_bc(grad_bc, true)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (a->dtype == XB_F64) return smooth_adjoint_small_t<double>(a, sweeps, scale, grad_bc, tmp, st);
+  return smooth_adjoint_small_t<float>(a, sweeps, scale, grad_bc, tmp, st);
 }
 
 int xb_elast_coupling_adjoint(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *lam_field,

@@ -62,6 +62,7 @@ class DarcyAdjointArgs(ctypes.Structure):
 XB_ADJ_SMOOTH, XB_ADJ_RESIDUAL = 1, 2
 # rule tables of at most this many slots are staged in LDS (xb_device.h LDS_BC_SLOTS)
 LDS_BC_SLOTS = 32
+ADJ_SMALL_MAX_CELLS = 65536  # xb_elast_smooth_adjoint_small
 
 
 class TailLevel(ctypes.Structure):
@@ -144,6 +145,8 @@ SIGNATURES = {
                                          ctypes.POINTER(TailLevel), c_int32, c_void_p, c_int64,
                                          ctypes.POINTER(TailPlan)]),
     "xb_vcycle_tail": (c_int32, [ctypes.POINTER(TailPlan), c_void_p]),
+    "xb_elast_smooth_adjoint_small": (c_int32, [ctypes.POINTER(ElastMGAdjointArgs), c_int32, c_double,
+                                                ctypes.POINTER(BCTable), c_void_p, c_void_p]),
     "xb_elast_coupling_adjoint": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
                                             c_void_p, c_double, ctypes.POINTER(BCTable), c_void_p, c_void_p,
                                             c_void_p, c_void_p, c_void_p, c_void_p]),

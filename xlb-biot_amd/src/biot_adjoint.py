@@ -69,10 +69,52 @@ class BiotAdjointSession:
         if da._omega_field is None:
             da.set_omega_field(torch.full((nx, ny), da.omega, dtype=torch.float64))
         self.nlf = p.coupling_force()
-        self.vg = el.vcycle_graph(_zeros((2, nx, ny), self.dtype, self.device), self.nlf)
+        self.vg = el.vcycle_graph(_zeros((2, nx, ny), self.dtype, self.device), self.nlf, keep_coarse=True)
+        # the forward keeps every V-cycle's coarse correction (a quarter of a state) when
+        # they fit this many bytes: the recompute then redoes only the finest level
+        nxc, nyc = el.grid_size(1)
+        fc_bytes = self.steps * self.n * 8 * nxc * nyc * torch.tensor([], dtype=self.dtype).element_size()
+        self.keep_coarse = fc_bytes <= float(getattr(p, "adjoint_coarse_budget_bytes", 8e9))
+        self.fc = []
         self.generation = 0
         self.ckpt_step = []
         self.ckpt_cycle = []
+        # coarse grids of at most this many cells run their transposed smoothing in one
+        # workgroup (one launch per run of sweeps instead of two per sweep)
+        self.adjoint_small_cells = 600
+        self._coarse_adjoint_graph()
+
+    def _coarse_adjoint_graph(self):
+        """The transposed coarse V-cycle (levels >= 1) as one captured HIP graph.  It
+        is the same linear map in every V-cycle (no recorded states, no parameter
+        derivatives): the restricted adjoint in :attr:`ab_c` to the adjoint of the
+        level-0 residual's restriction in :attr:`rbar_c`."""
+        el = self.el
+        nxc, nyc = el.grid_size(1)
+        x0, x1, y0, y1 = el.interior_box(0)
+        self.ab_c = _zeros((8, nxc, nyc), self.dtype, self.device)
+        self.rbar_c = _zeros((8, (x1 - x0) // 2, (y1 - y0) // 2), self.dtype, self.device)
+        no_acc = {"lam": None, "mu": None, "base": None}
+
+        def body():
+            nat.fill(self.rbar_c, 0.0)
+            self._level_adjoint(1, self.ab_c, None, self.rbar_c, no_acc)
+        body()  # eager once: the gradient wall tables of the coarse grids exist before capture
+        torch.cuda.synchronize()
+        self.coarse_graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.coarse_graph):
+            body()
+        # the coarse solve of a recomputed V-cycle (residual of the pre-smoothed state,
+        # its restriction, the coarse V-cycle from zero) from static inputs
+        nx, ny = el.grid_size(0)
+        self.cs_f = _zeros((8, nx, ny), self.dtype, self.device)
+        self.cs_div = _zeros((1, nx, ny), self.dtype, self.device)
+        self.cs_base = _zeros((2, nx, ny), self.dtype, self.device)
+        el._coarse_solve_nl(self.cs_f, self.cs_div, 0, None, self.cs_base, self.nlf)
+        torch.cuda.synchronize()
+        self.solve_graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.solve_graph):
+            self.cs_out, _ = el._coarse_solve_nl(self.cs_f, self.cs_div, 0, None, self.cs_base, self.nlf)
 
     # ---- parameters
     def bind(self, omega, lam, mu):
@@ -85,7 +127,7 @@ class BiotAdjointSession:
         self.generation += 1
         p, vg = self.p, self.vg
         fe, fd, dde = p.initial_state()
-        self.ckpt_step, self.ckpt_cycle = [], []
+        self.ckpt_step, self.ckpt_cycle, self.fc = [], [], []
         out = None
         for t in range(self.steps):
             if t % self.P == 0:
@@ -93,25 +135,36 @@ class BiotAdjointSession:
             ctx = p._pre_inner(fe, fd, dde, t)
             vg.set_force(ctx["force"])
             vg.load(fe)
-            cps = []
+            cps, fcs = [], []
             for j in range(self.n):
                 if j > 0 and j % self.K == 0:
                     cps.append(vg.state.clone())
                 vg.replay()
+                if self.keep_coarse:
+                    fcs.append(vg.coarse.clone())
             self.ckpt_cycle.append(cps)
+            self.fc.append(fcs)
             out = p._post_inner(ctx, vg.state.clone())
             fe, fd, dde = out[0], out[1], out[4]
         return out[2], out[5]
 
     # ---- recompute of V-cycles with the finest-level sweep inputs kept
-    def _vcycle_record(self, f, div, base):
+    def _vcycle_record(self, f, div, base, f_c=None):
+        """One V-cycle from ``f`` (divergence ``div``) with its finest-level sweep
+        inputs kept; ``f_c``: its coarse correction when the forward kept it (else the
+        coarse levels are solved again)."""
         el, nlf, k = self.el, self.nlf, self.el.k_smooth
         states = []
         for _ in range(k):
             states.append(f)
             f, div = el._smooth_nl(f, div, 0, None, base, nlf)
         states.append(f)
-        f_c, _ = el._coarse_solve_nl(f, div, 0, None, base, nlf)
+        if f_c is None:
+            self.cs_f.copy_(f)
+            self.cs_div.copy_(div)
+            self.cs_base.copy_(base)
+            self.solve_graph.replay()
+            f_c = self.cs_out
         nxc, nyc = el.grid_size(1)
         f, div = f.clone(), div.clone()
         nx, ny = el.grid_size(0)
@@ -124,12 +177,12 @@ class BiotAdjointSession:
             f, div = el._smooth_nl(f, div, 0, None, base, nlf)
         return f, div, states
 
-    def _recompute(self, start, m, base):
+    def _recompute(self, start, m, base, fcs=None):
         f = start
         div = self.el._divergence_soa(f)
         recs = []
-        for _ in range(m):
-            f, div, states = self._vcycle_record(f, div, base)
+        for i in range(m):
+            f, div, states = self._vcycle_record(f, div, base, fcs[i] if fcs else None)
             recs.append(states)
         return recs, f
 
@@ -165,24 +218,58 @@ class BiotAdjointSession:
         adjoint of the level's input populations."""
         el, k = self.el, self.el.k_smooth
         S, R = nat.XB_ADJ_SMOOTH, nat.XB_ADJ_RESIDUAL
+        nx, ny = el.grid_size(level)
+        small = states is None and level > 0 and nx * ny <= self.adjoint_small_cells
         post = states[k + 1:] if states is not None else [None] * k
-        for j in range(k - 1, -1, -1):
-            ab = self._sweep_adjoint(level, S, ab, None, post[j], rbar, acc)
+        if small:
+            ab = self._smooth_adjoint_small(level, ab, k, rbar)
+        else:
+            for j in range(k - 1, -1, -1):
+                ab = self._sweep_adjoint(level, S, ab, None, post[j], rbar, acc)
         if level < el.levels - 1:
             nx, ny = el.grid_size(level)
             nxc, nyc = el.grid_size(level + 1)
             x0, x1, y0, y1 = el.interior_box(level)
-            ab_c = torch.empty((8, nxc, nyc), dtype=self.dtype, device=self.device)
+            graph = level == 0 and getattr(self, "coarse_graph", None) is not None
+            ab_c = self.ab_c if graph else torch.empty((8, nxc, nyc), dtype=self.dtype, device=self.device)
             nat.check(nat.lib().xb_elast_restrict_sum(nx, ny, nat.dtype_code(self.dtype), nat.ptr(ab), nat.ptr(ab_c),
                                                       nxc, nyc, x0, x1, y0, y1, nat.stream()), "xb_elast_restrict_sum")
-            rbar_c = nat.fill(torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=self.dtype,
-                                          device=self.device), 0.0)
-            self._level_adjoint(level + 1, ab_c, None, rbar_c, acc)
+            if graph:
+                # rbar_c is read by the residual sweep below before the next replay
+                self.coarse_graph.replay()
+                rbar_c = self.rbar_c
+            else:
+                rbar_c = nat.fill(torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=self.dtype,
+                                              device=self.device), 0.0)
+                self._level_adjoint(level + 1, ab_c, None, rbar_c, acc)
             ab = self._sweep_adjoint(level, R, rbar_c, ab, states[k] if states is not None else None, rbar, acc)
         pre = states[:k] if states is not None else [None] * k
+        if small:
+            return self._smooth_adjoint_small(level, ab, k, rbar)
         for j in range(k - 1, -1, -1):
             ab = self._sweep_adjoint(level, S, ab, None, pre[j], rbar, acc)
         return ab
+
+    def _smooth_adjoint_small(self, level, ab_in, sweeps, rbar):
+        """``sweeps`` transposed smoothing sweeps of a coarse level (no recorded
+        states) with their transposed coupling force in one launch
+        (``xb_elast_smooth_adjoint_small``)."""
+        el = self.el
+        nx, ny = el.grid_size(level)
+        x0, x1, y0, y1 = el.interior_box(level)
+        out = torch.empty((8, nx, ny), dtype=self.dtype, device=self.device)
+        tmp = torch.empty_like(out)
+        fbar = torch.empty((2, nx, ny), dtype=self.dtype, device=self.device)
+        table = el._tables.get(level)
+        a = nat.ElastMGAdjointArgs(nx, ny, nat.dtype_code(self.dtype), nat.XB_ADJ_SMOOTH, el.lamda, el.mu, el.gamma,
+                                   None, None, x0, x1, y0, y1, nat.ptr(ab_in), None, nat.ptr(out), nat.ptr(rbar),
+                                   nat.ptr(fbar), None, None, None, None,
+                                   table.ref() if table is not None else None)
+        gt = self.da._grad_table(nx, ny)
+        nat.check(nat.lib().xb_elast_smooth_adjoint_small(a, int(sweeps), float(self.nlf.scale),
+                                                          gt.ref() if gt is not None else None, nat.ptr(tmp),
+                                                          nat.stream()), "xb_elast_smooth_adjoint_small")
+        return out
 
     def _inner_adjoint(self, t, fe_prev, base, ab, acc, last_recs):
         """Adjoint of the V-cycles of physical step ``t`` (``ab``: adjoint of their
@@ -193,7 +280,7 @@ class BiotAdjointSession:
             if s == len(starts) - 1:
                 recs = last_recs
             else:
-                recs, _ = self._recompute(starts[s], self.K, base)
+                recs, _ = self._recompute(starts[s], self.K, base, self.fc[t][s * self.K:(s + 1) * self.K])
             for states in reversed(recs):
                 ab = self._level_adjoint(0, ab, states, None, acc)
         return ab
@@ -209,7 +296,7 @@ class BiotAdjointSession:
             ctx = p._pre_inner(fe, fd, dde, t)
             starts = [fe] + self.ckpt_cycle[t]
             m_last = self.n - (len(starts) - 1) * self.K
-            recs, fe_new = self._recompute(starts[-1], m_last, ctx["force"])
+            recs, fe_new = self._recompute(starts[-1], m_last, ctx["force"], self.fc[t][self.n - m_last:])
             out.append((fe, fd, dde, ctx, recs, fe_new))
             if t + 1 < t1:
                 o = p._post_inner(ctx, fe_new)

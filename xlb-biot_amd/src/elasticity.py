@@ -699,13 +699,16 @@ class LBMElasticity(object):
             done += step
         return f, div
 
-    def _vcycle_nl(self, f, div, level, residual, base, nlf, out=None, out_div=None):
+    def _vcycle_nl(self, f, div, level, residual, base, nlf, out=None, out_div=None, coarse=None):
         """``_vcycle`` with the Biot coupling force fused into the sweeps; ``div`` is
-        the divergence of ``f`` and is carried along.  Returns ``(f, div[, norm])``."""
+        the divergence of ``f`` and is carried along.  Returns ``(f, div[, norm])``;
+        ``coarse`` (a list) receives the coarse correction prolonged into ``f``."""
         f, div = self._smooth_seq_nl(f, div, level, residual, base, nlf, self.k_smooth)
         norm = None
         if level < self.levels - 1:
             f_c, norm = self._coarse_solve_nl(f, div, level, residual, base, nlf)
+            if coarse is not None:
+                coarse.append(f_c)
             nxc, nyc = self.grid_size(level + 1)
             nx, ny = self.grid_size(level)
             x0, x1, y0, y1 = self.interior_box(level)
@@ -789,9 +792,9 @@ class LBMElasticity(object):
             _, ws, n = self._residual(f0, 0, None, ext)
         return ws.finalize(n)[0]
 
-    def vcycle_graph(self, external_force=None, nonlinear_force=None):
+    def vcycle_graph(self, external_force=None, nonlinear_force=None, keep_coarse=False):
         """A V-cycle captured as HIP graphs: see :class:`VCycleGraph`."""
-        return VCycleGraph(self, external_force, nonlinear_force)
+        return VCycleGraph(self, external_force, nonlinear_force, keep_coarse)
 
     # ---- driver (elasticity.py:647-699, 1346-1456)
     def assign_fields_sharded(self):
@@ -915,7 +918,7 @@ class VCycleGraph:
     Each replay advances the state by one V-cycle and leaves the level-0 residual
     norm of that cycle in :attr:`norm` (a device scalar; reading it synchronises)."""
 
-    def __init__(self, el, external_force=None, nonlinear_force=None):
+    def __init__(self, el, external_force=None, nonlinear_force=None, keep_coarse=False):
         if el.levels < 2:
             raise ValueError("V-cycle graphs need MG_levels >= 2")
         self.el = el
@@ -926,25 +929,33 @@ class VCycleGraph:
         self.fused = el._fusable(nlf)
         self.div = [torch.zeros((1, el.nx, el.ny), dtype=el._tdt, device=el.device) for _ in range(2)]
 
-        def cycle(src):
+        if keep_coarse and not self.fused:
+            raise ValueError("keep_coarse needs the fused coupling force")
+
+        def cycle(src, coarse=None):
             if self.fused:
                 _, _, norm = el._vcycle_nl(self.buf[src], self.div[src], 0, None, self.ext, nlf,
-                                           out=self.buf[1 - src], out_div=self.div[1 - src])
+                                           out=self.buf[1 - src], out_div=self.div[1 - src], coarse=coarse)
             else:
                 _, norm = el._vcycle(self.buf[src], 0, None, self.ext, nlf, out=self.buf[1 - src])
             return norm
         # eager warm-up: rule tables, norm workspaces and gradient tables exist before capture
         cycle(0)
         torch.cuda.synchronize()
-        self.graphs, self.norms = [], []
+        self.graphs, self.norms, self.coarse_bufs = [], [], []
         for src in (0, 1):
             g = torch.cuda.CUDAGraph()
+            kept = [] if keep_coarse else None
             with torch.cuda.graph(g):
-                norm = cycle(src)
+                norm = cycle(src, kept)
             self.graphs.append(g)
             self.norms.append(norm)
+            # the graph's coarse correction buffer (held here, so the graph's memory pool
+            # never hands it to another allocation of the capture)
+            self.coarse_bufs.append(kept[0] if keep_coarse else None)
         self.cur = 0
         self.norm = None
+        self.coarse = None
 
     @property
     def state(self):
@@ -967,6 +978,7 @@ class VCycleGraph:
     def replay(self):
         g = self.graphs[self.cur]
         self.norm = self.norms[self.cur]
+        self.coarse = self.coarse_bufs[self.cur]  # with keep_coarse: this cycle's coarse correction
         g.replay()
         self.cur = 1 - self.cur
         return self.norm
