@@ -234,20 +234,29 @@ def bench_copy(torch, reps=20):
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
-def bench_biot_vcycle(torch, n=4096, levels=6, reps=10):
+def bench_biot_vcycle(torch, n=4096, levels=6, reps=10, eager=False):
     """One fused-coupling V-cycle of the periodic Biot step (manufactured forcing,
     lambda = mu = 1, K = 1, k = 2 smoothing steps, FP64) on an n x n grid, replayed
-    from its captured graph; bytes from :func:`vcycle_bytes`."""
+    from its captured graph (``eager``: launched one kernel at a time); bytes from
+    :func:`vcycle_bytes`."""
     import problems as P
     b = P.biot_periodic("gpu", nx=n, levels=levels, steps=1, device_hooks=True)
     el = b.elasticity
     el.autotune(external_force=True)
     fe, fd, dde = b.initial_state()
     ctx = b._pre_inner(fe, fd, dde, 1)
-    vg = el.vcycle_graph(ctx["force"], ctx["nlf"])
-    vg.load(fe)
-    vg.run(2)
-    t = event_pair_time(torch, vg.replay, reps)
+    if eager:
+        div = el._divergence_soa(fe)
+
+        def cycle():
+            el._vcycle_nl(fe, div, 0, None, ctx["force"], ctx["nlf"])
+        cycle()
+        t = event_pair_time(torch, cycle, reps)
+    else:
+        vg = el.vcycle_graph(ctx["force"], ctx["nlf"])
+        vg.load(fe)
+        vg.run(2)
+        t = event_pair_time(torch, vg.replay, reps)
     m = el._smooth_tile_steps(0, ctx["nlf"])
     byt = vcycle_bytes(n, levels, el.k_smooth, m=m)
     byt1 = vcycle_bytes(n, levels, el.k_smooth)
@@ -616,6 +625,10 @@ def only(torch, args):
     elif args.only == "biot_vcycle":
         r = bench_biot_vcycle(torch, reps=args.steps)
         log(f"biot vcycle: {r}")
+    elif args.only == "biot_vcycle_eager":
+        # the same V-cycle launched eagerly (no graph replay), for kernel traces
+        r = bench_biot_vcycle(torch, reps=args.steps, eager=True)
+        log(f"biot vcycle (eager launches): {r}")
     elif args.only == "c5":
         print(json.dumps(bench_c5(torch, n_inst=args.instances)), flush=True)
     elif args.only == "adjoint":
@@ -650,7 +663,7 @@ def main():
     ap.add_argument("--tile-threads", type=int, default=0, choices=[0, 256, 512],
                     help="with --tile-steps: threads per tile block (0: the library's choice)")
     ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint", "biot_vcycle",
-                             "biot_adjoint"],
+                                       "biot_vcycle_eager", "biot_adjoint"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:

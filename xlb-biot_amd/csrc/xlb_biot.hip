@@ -1453,6 +1453,10 @@ template <typename T> int build_sweep_jobs(const xb_elast_args *args, const int3
 // divergence of the result.  Per cell the arithmetic is that of m XB_MODE_SMOOTH
 // sweeps (bit-identical).  Rule tables of at most LDS_BC_SLOTS slots only (staged in
 // LDS with an identity row, as k_tile_steps); larger ones stay on the sweeps.
+// Blocks of the smoothing tiles (cells per thread: SMOOTH_CPT).
+constexpr int SMOOTH_THREADS = TILE_THREADS;
+constexpr int SMOOTH_CPT = TILE * TILE / SMOOTH_THREADS;
+
 template <typename T> struct TileSmoothDev {
   int nx, ny, steps, tiles_y, ix0, ix1, iy0, iy1;
   T gamma, omg;
@@ -1500,14 +1504,14 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
     stage_rule_rows<T, Q>(A.bc.code_ps, A.bc.const_ps, A.bc.nslots, sh->code_ps, sh->const_ps);
   }
   if constexpr (GWALLS) stage_rule_rows<T, 9>(A.nl.gbc.code_ps, A.nl.gbc.const_ps, A.nl.gbc.nslots, gsh->code, gsh->cst);
-  T f[TILE_CPT][Q];
-  Aux aux[TILE_CPT];
-  T kd[TILE_CPT];
-  int slot[TILE_CPT], gslot[TILE_CPT], ri[TILE_CPT];
+  T f[SMOOTH_CPT][Q];
+  Aux aux[SMOOTH_CPT];
+  T kd[SMOOTH_CPT];
+  int slot[SMOOTH_CPT], gslot[SMOOTH_CPT], ri[SMOOTH_CPT];
   bool mine = false, gmine = false;
 #pragma unroll
-  for (int k = 0; k < TILE_CPT; ++k) {
-    const int c = threadIdx.x + k * TILE_THREADS;
+  for (int k = 0; k < SMOOTH_CPT; ++k) {
+    const int c = threadIdx.x + k * SMOOTH_THREADS;
     const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
     const size_t cell = (size_t)gx * ny + gy;
 #pragma unroll
@@ -1526,19 +1530,19 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
   const bool walls = BC == 2 && __syncthreads_or(mine) != 0;
   const bool gwalls = GWALLS && __syncthreads_or(gmine) != 0;
 #pragma unroll
-  for (int k = 0; k < TILE_CPT; ++k) {
+  for (int k = 0; k < SMOOTH_CPT; ++k) {
     if constexpr (BC == 2) slot[k] = slot[k] >= 0 ? slot[k] : A.bc.nslots;
     if constexpr (GWALLS) gslot[k] = gslot[k] >= 0 ? gslot[k] : A.nl.gbc.nslots;
   }
   for (int s = 0; s < steps; ++s) {
     if constexpr (HAS_NL) {
 #pragma unroll
-      for (int k = 0; k < TILE_CPT; ++k) ldiv[threadIdx.x + k * TILE_THREADS] = elast_div_of<T>(f[k], kd[k]);
+      for (int k = 0; k < SMOOTH_CPT; ++k) ldiv[threadIdx.x + k * SMOOTH_THREADS] = elast_div_of<T>(f[k], kd[k]);
       __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) {
-      const int c = threadIdx.x + k * TILE_THREADS;
+    for (int k = 0; k < SMOOTH_CPT; ++k) {
+      const int c = threadIdx.x + k * SMOOTH_THREADS;
       const int rx = c / TILE, ry = c % TILE;
       T pre[Q];
 #pragma unroll
@@ -1600,8 +1604,8 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < TILE_CPT; ++k) {
-      const int c = threadIdx.x + k * TILE_THREADS;
+    for (int k = 0; k < SMOOTH_CPT; ++k) {
+      const int c = threadIdx.x + k * SMOOTH_THREADS;
       const int rx = c / TILE, ry = c % TILE;
       T v[Q];
 #pragma unroll
@@ -1623,8 +1627,8 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
     __syncthreads();
   }
 #pragma unroll
-  for (int k = 0; k < TILE_CPT; ++k) {
-    const int c = threadIdx.x + k * TILE_THREADS;
+  for (int k = 0; k < SMOOTH_CPT; ++k) {
+    const int c = threadIdx.x + k * SMOOTH_THREADS;
     const int rx = c / TILE, ry = c % TILE;
     const int h = R * steps;
     if (rx < h || rx >= TILE - h || ry < h || ry >= TILE - h) continue;
@@ -1672,7 +1676,7 @@ __device__ __forceinline__ void tile_smooth_run(const TileSmoothDev<T> &A, int t
 }
 
 template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
-__global__ void __launch_bounds__(TILE_THREADS) k_tile_smooth(const TileSmoothDev<T> A) {
+__global__ void __launch_bounds__(SMOOTH_THREADS) k_tile_smooth(const TileSmoothDev<T> A) {
   tile_smooth_run<T, BC, HAS_RES, HAS_NL, GWALLS>(A, xcd_block_id());
 }
 
@@ -1683,7 +1687,7 @@ template <typename T> struct TileSmoothJob {
 };
 
 template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
-__global__ void __launch_bounds__(TILE_THREADS)
+__global__ void __launch_bounds__(SMOOTH_THREADS)
 k_tile_smooth_batch(const TileSmoothJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
                     const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
   const int b = xcd_block_id();
@@ -1696,9 +1700,9 @@ k_tile_smooth_batch(const TileSmoothJob<T> *__restrict__ jobs, const int32_t *__
 
 template <typename T, int BC, bool HAS_RES>
 void launch_tile_smooth_nl(const TileSmoothDev<T> &A, dim3 grid, hipStream_t st, bool nl, bool gwalls) {
-  if (!nl) hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, false, false>), grid, dim3(TILE_THREADS), 0, st, A);
-  else if (gwalls) hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, true, true>), grid, dim3(TILE_THREADS), 0, st, A);
-  else hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, true, false>), grid, dim3(TILE_THREADS), 0, st, A);
+  if (!nl) hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, false, false>), grid, dim3(SMOOTH_THREADS), 0, st, A);
+  else if (gwalls) hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, true, true>), grid, dim3(SMOOTH_THREADS), 0, st, A);
+  else hipLaunchKernelGGL((k_tile_smooth<T, BC, HAS_RES, true, false>), grid, dim3(SMOOTH_THREADS), 0, st, A);
 }
 
 // the tile-smoothing operands of one xb_elast_args; returns the number of tiles
@@ -1781,7 +1785,7 @@ template <typename T> int build_smoothn_jobs(const xb_elast_args *args, const in
 template <typename T, int BC, bool RES>
 void launch_smoothn_batch_nl(const TileSmoothJob<T> *jobs, const int32_t *bj, const int32_t *bi,
                              const int32_t *active, dim3 grid, hipStream_t st, bool nl, bool gw) {
-  const dim3 block(TILE_THREADS);
+  const dim3 block(SMOOTH_THREADS);
   if (!nl) hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, false, false>), grid, block, 0, st, jobs, bj, bi, active);
   else if (gw) hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, true, true>), grid, block, 0, st, jobs, bj, bi, active);
   else hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, true, false>), grid, block, 0, st, jobs, bj, bi, active);
