@@ -1624,7 +1624,10 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
 #pragma unroll
       for (int d = 0; d < Q; ++d) f[k][d] = A.omg * f[k][d] + A.gamma * v[d];
     }
-    __syncthreads();
+    // the pull's LDS reads end before the next step's post-collision writes: with the
+    // coupling force the next step's divergence barrier (ldiv is another array) already
+    // orders them, and after the last step only registers are read
+    if constexpr (!HAS_NL) __syncthreads();
   }
 #pragma unroll
   for (int k = 0; k < SMOOTH_CPT; ++k) {
