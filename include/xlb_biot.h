@@ -418,7 +418,7 @@ enum {
   XB_BATCH_ELAST_STEP = 1,  /* jobs: xb_elast_args[] (one xb_elast_step each)             */
   XB_BATCH_PROLONG_DIV = 2, /* jobs: xb_prolong_job[] (xb_elast_prolong_add[_div] each)   */
   XB_BATCH_FILL = 3,        /* jobs: xb_fill_job[] (xb_fill each)                         */
-  XB_BATCH_NORM = 4,        /* jobs: xb_norm_job[] (for xb_batch_converge)                */
+  XB_BATCH_NORM = 4,        /* jobs: xb_norm_job[] (for xb_batch_converge), one per instance */
   XB_BATCH_ELAST_SMOOTHN = 5 /* jobs: xb_elast_args[] in XB_MODE_SMOOTH, rows = steps per
                                 launch (one xb_elast_smoothn each)                        */
 };

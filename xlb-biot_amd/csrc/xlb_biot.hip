@@ -1323,63 +1323,65 @@ struct NormJob {
   int32_t inst;
 };
 
-// Per-instance convergence test after a batched V-cycle: the block sums each active
-// job's residual partials in turn (block_sum, as k_norm_finalize does), then thread
-// 0 updates that instance's state and clears its activity flag once converged
-// (r <= tol * s, s the instance's residual scale if given, else r0), out of cycles,
-// or non-finite, and finally counts the active instances.  One block.
+// Per-instance convergence test after a batched V-cycle, one block per job (one job
+// per instance): the block sums the job's residual partials (block_sum, as
+// k_norm_finalize does) if the instance is active, then thread 0 updates that
+// instance's state and clears its activity flag once converged (r <= tol * s, s the
+// instance's residual scale if given, else r0), out of cycles, or non-finite.
+// k_count_active then counts the active instances.
 __global__ void __launch_bounds__(NORM_THREADS)
-k_batch_converge(const NormJob *__restrict__ jobs, int njobs, int ninst, double tol, int max_cycles,
-                 double *__restrict__ r0, double *__restrict__ rlast, int32_t *__restrict__ cycles,
-                 int32_t *__restrict__ active, int32_t *__restrict__ status, int32_t *__restrict__ nactive,
-                 const double *__restrict__ rref) {
+k_batch_converge(const NormJob *__restrict__ jobs, double tol, int max_cycles, double *__restrict__ r0,
+                 double *__restrict__ rlast, int32_t *__restrict__ cycles, int32_t *__restrict__ active,
+                 int32_t *__restrict__ status, const double *__restrict__ rref) {
   __shared__ double red[NORM_THREADS];
-  for (int j = 0; j < njobs; ++j) {
-    const NormJob J = jobs[j];
-    // every thread reads the flag before thread 0 may clear it below
-    const bool on = active[J.inst] != 0;
-    __syncthreads();
-    if (!on) continue; // uniform over the block
-    const double s = block_sum(J.partials, J.count, red);
-    if (threadIdx.x == 0) {
-      const int i = J.inst;
-      const double r = sqrt(s);
-      const int c = cycles[i] + 1;
-      cycles[i] = c;
-      rlast[i] = r;
-      if (!isfinite(s)) {
-        status[i] = 3;
+  const NormJob J = jobs[blockIdx.x];
+  // every thread reads the flag before block_sum's barriers; thread 0 writes it after
+  if (active[J.inst] == 0) return; // uniform over the block
+  const double s = block_sum(J.partials, J.count, red);
+  if (threadIdx.x == 0) {
+    const int i = J.inst;
+    const double r = sqrt(s);
+    const int c = cycles[i] + 1;
+    cycles[i] = c;
+    rlast[i] = r;
+    if (!isfinite(s)) {
+      status[i] = 3;
+      active[i] = 0;
+    } else {
+      if (r0[i] < 0.0) r0[i] = r;
+      const double scale = (rref != nullptr && rref[i] > 0.0) ? rref[i] : r0[i];
+      if (r <= tol * scale) {
+        status[i] = 1;
         active[i] = 0;
-      } else {
-        if (r0[i] < 0.0) r0[i] = r;
-        const double scale = (rref != nullptr && rref[i] > 0.0) ? rref[i] : r0[i];
-        if (r <= tol * scale) {
-          status[i] = 1;
-          active[i] = 0;
-        } else if (c >= max_cycles) {
-          status[i] = 2;
-          active[i] = 0;
-        }
+      } else if (c >= max_cycles) {
+        status[i] = 2;
+        active[i] = 0;
       }
     }
-    __syncthreads(); // thread 0's writes are seen by the block before the next job
-  }
-  if (threadIdx.x == 0) {
-    int n = 0;
-    for (int i = 0; i < ninst; ++i) n += active[i] != 0;
-    *nactive = n;
   }
 }
 
-// out[inst] = sqrt(sum of the job's partials), the jobs in turn by one block.
 __global__ void __launch_bounds__(NORM_THREADS)
-k_batch_norms(const NormJob *__restrict__ jobs, int njobs, double *__restrict__ out) {
-  __shared__ double red[NORM_THREADS];
-  for (int j = 0; j < njobs; ++j) {
-    const NormJob J = jobs[j];
-    const double s = block_sum(J.partials, J.count, red);
-    if (threadIdx.x == 0) out[J.inst] = sqrt(s);
+k_count_active(const int32_t *__restrict__ active, int ninst, int32_t *__restrict__ nactive) {
+  __shared__ int red[NORM_THREADS];
+  int n = 0;
+  for (int i = threadIdx.x; i < ninst; i += NORM_THREADS) n += active[i] != 0;
+  red[threadIdx.x] = n;
+  __syncthreads();
+  for (int w = NORM_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) *nactive = red[0];
+}
+
+// out[inst] = sqrt(sum of the job's partials), one block per job.
+__global__ void __launch_bounds__(NORM_THREADS)
+k_batch_norms(const NormJob *__restrict__ jobs, double *__restrict__ out) {
+  __shared__ double red[NORM_THREADS];
+  const NormJob J = jobs[blockIdx.x];
+  const double s = block_sum(J.partials, J.count, red);
+  if (threadIdx.x == 0) out[J.inst] = sqrt(s);
 }
 
 // Host side of the batches: job tables are built here and copied into the
@@ -1859,11 +1861,20 @@ template <typename T> int build_fill_jobs(const xb_fill_job *args, const int32_t
 }
 
 int build_norm_jobs(const xb_norm_job *args, const int32_t *inst, int32_t n, std::vector<NormJob> &jobs) {
+  std::vector<int32_t> seen;
   for (int32_t k = 0; k < n; ++k) {
     if (args[k].partials == nullptr || args[k].count <= 0)
       return fail(XB_EINVAL, "xb_batch_prepare: norm job %d: bad arguments", k);
-    jobs.push_back(NormJob{args[k].partials, args[k].count, inst != nullptr ? inst[k] : k});
+    const int32_t i = inst != nullptr ? inst[k] : k;
+    if (i < 0) return fail(XB_EINVAL, "xb_batch_prepare: norm job %d: negative instance", k);
+    seen.push_back(i);
+    jobs.push_back(NormJob{args[k].partials, args[k].count, i});
   }
+  // the convergence and norm kernels give each job a block of its own and write its
+  // instance's state: one job per instance
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+    return fail(XB_EINVAL, "xb_batch_prepare: a norm plan takes one job per instance");
   return XB_OK;
 }
 
@@ -3200,9 +3211,11 @@ int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converg
     return fail(XB_EINVAL, "xb_batch_converge: incomplete state");
   if (ninst <= 0 || max_cycles <= 0 || !(tol >= 0.0)) return fail(XB_EINVAL, "xb_batch_converge: bad arguments");
   const NormJob *jobs = reinterpret_cast<const NormJob *>(static_cast<const char *>(plan->table) + plan->jobs_offset);
-  hipLaunchKernelGGL(k_batch_converge, dim3(1), dim3(NORM_THREADS), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs,
-                     ninst, tol, max_cycles, state->r0, state->rlast, state->cycles, state->active, state->status,
-                     state->nactive, state->rref);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (plan->njobs > 0)
+    hipLaunchKernelGGL(k_batch_converge, dim3(plan->njobs), dim3(NORM_THREADS), 0, st, jobs, tol, max_cycles,
+                       state->r0, state->rlast, state->cycles, state->active, state->status, state->rref);
+  hipLaunchKernelGGL(k_count_active, dim3(1), dim3(NORM_THREADS), 0, st, state->active, ninst, state->nactive);
   return check_launch("xb_batch_converge");
 }
 
@@ -3271,7 +3284,9 @@ int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream) {
   if (plan == nullptr || plan->kind != XB_BATCH_NORM || plan->table == nullptr || out == nullptr)
     return fail(XB_EINVAL, "xb_batch_norms: needs a norm plan and an output");
   const NormJob *jobs = reinterpret_cast<const NormJob *>(static_cast<const char *>(plan->table) + plan->jobs_offset);
-  hipLaunchKernelGGL(k_batch_norms, dim3(1), dim3(NORM_THREADS), 0, static_cast<hipStream_t>(stream), jobs, plan->njobs, out);
+  if (plan->njobs > 0)
+    hipLaunchKernelGGL(k_batch_norms, dim3(plan->njobs), dim3(NORM_THREADS), 0, static_cast<hipStream_t>(stream), jobs,
+                       out);
   return check_launch("xb_batch_norms");
 }
 
