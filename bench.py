@@ -90,10 +90,8 @@ def bench_darcy(torch, n, precision, steps, warmup, chunk=64, tune=True, tile_st
     kind = "tile" if choice > autotune.TILE_BASE else ("sweep2" if choice == autotune.SWEEP2 else "sweep")
     threads, blocks = None, None
     if kind == "tile":
-        inner = 32 - 2 * spl
-        blocks = -(-d.nx // inner) * -(-d.ny // inner)
-        # the library's own choice (threads 0): 512 for at most four tiles per CU
-        threads = autotune.tile_threads(choice) or (512 if blocks <= 4 * 256 else 256)
+        threads = autotune.tile_threads(choice) or autotune.library_tile_threads(d.nx, d.ny, spl)
+        blocks = autotune.tile_blocks(d.nx, d.ny, spl, threads)
     return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 19 * S,
                 steps_per_launch=spl, launch_kind=kind, tile_threads=threads, tile_blocks=blocks, solver=d)
 
@@ -112,9 +110,7 @@ def bench_large_elasticity(torch, n, steps, warmup, precision="f64/f64", chunk=3
     from src import autotune
     threads = None
     if choice > autotune.TILE_BASE:
-        t = -(-n // (32 - 2 * spl))
-        blocks = t * t
-        threads = autotune.tile_threads(choice) or (512 if blocks <= 4 * 256 else 256)
+        threads = autotune.tile_threads(choice) or autotune.library_tile_threads(n, n, spl)
     return dict(elapsed=wall, gpu=gpu, cells=cells, kernel_s=gpu / steps * spl, bytes=cells * 18 * S,
                 steps_per_launch=spl, tile_threads=threads)
 
@@ -660,8 +656,9 @@ def main():
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched Biot extra")
     ap.add_argument("--tile-steps", type=int, default=0,
                     help="with --only c2 or large_darcy: steps per launch of the LDS-tile kernel instead of the tuned launch")
-    ap.add_argument("--tile-threads", type=int, default=0, choices=[0, 256, 512],
-                    help="with --tile-steps: threads per tile block (0: the library's choice)")
+    ap.add_argument("--tile-threads", type=int, default=0, choices=[0, 256, 512, 704],
+                    help="with --tile-steps: threads per tile block, which fixes the tile shape (256, 512: "
+                         "32 x 32; 704: 44 x 48; 0: the library's choice)")
     ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint", "biot_vcycle",
                                        "biot_vcycle_eager", "biot_adjoint"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
@@ -689,7 +686,8 @@ def main():
     if kind == "tile":
         rows = None
         blocks = r["tile_blocks"]
-        kname = f"k_tile_steps<double, xb::DarcyColl<double>, 2, {r['tile_threads']}>"
+        tx_, ty_ = autotune.TILE_SHAPES[r["tile_threads"]]
+        kname = f"k_tile_steps<double, xb::DarcyColl<double>, 2, {tx_}, {ty_}, {r['tile_threads']}>"
     else:
         two = kind == "sweep2"
         rows = autotune.lookup(d._tune_key(two))
@@ -768,7 +766,8 @@ def main():
                                          "launch_kind": rd["launch_kind"], "threads_per_block": rd["tile_threads"],
                                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": 152}
             if rd["launch_kind"] == "tile":
-                kn = f"k_tile_steps<double, xb::DarcyColl<double>, 2, {rd['tile_threads']}>"
+                tx_, ty_ = autotune.TILE_SHAPES[rd["tile_threads"]]
+                kn = f"k_tile_steps<double, xb::DarcyColl<double>, 2, {tx_}, {ty_}, {rd['tile_threads']}>"
                 extras[f"darcy_{n}_fp64"]["traffic"] = profiled_traffic(kn, rd["tile_blocks"], rd["cells"])
     line["extra"] = extras
     if not args.no_cpu:

@@ -140,9 +140,10 @@ int xb_elast_step2(const xb_elast_args *args, void *stream);
 
 /* `steps` (1..8) consecutive plain steps in one launch on 32 x 32 LDS tiles
  * (temporal blocking; bit-identical to `steps` XB_MODE_STEP calls).  The same
- * restrictions as xb_elast_step2.  Here args->rows is the tile block size: 256 or
- * 512 threads per tile; any other value lets the library choose (512 for grids of
- * at most four tiles per CU, else 256).  Results do not depend on it. */
+ * restrictions as xb_elast_step2.  Here args->rows selects the tile shape: 256 or
+ * 512 threads per 32 x 32 tile, 704 threads per 44 x 48 tile; any other value lets the
+ * library choose (32 x 32 tiles, 512 threads for grids of at most four tiles per
+ * CU, else 256).  Results do not depend on it. */
 int xb_elast_stepn(const xb_elast_args *args, int32_t steps, void *stream);
 
 /* `steps` consecutive XB_MODE_SMOOTH sweeps of one multigrid level in one launch on

@@ -27,7 +27,7 @@ def _single(step, f, n):
 @pytest.mark.parametrize("n,precision,hetero", [(514, "f64/f64", True), (70, "f32/f32", True), (16, "f64/f64", False),
                                                (5, "f64/f64", False)])
 @pytest.mark.parametrize("m", [1, 3, 4, 8])
-@pytest.mark.parametrize("threads", [256, 512])
+@pytest.mark.parametrize("threads", [256, 512, 704])
 def test_darcy_stepn_matches_single_steps(n, precision, hetero, m, threads):
     say("darcy", n, precision, "hetero", hetero, "steps per launch", m, "threads per tile", threads)
     d = P.darcy_c2("gpu", n=n - 2, precision=precision, heterogeneous=hetero, seed=0, variance=1.0, corr=4.0)
@@ -41,7 +41,7 @@ def test_darcy_stepn_matches_single_steps(n, precision, hetero, m, threads):
 
 @pytest.mark.parametrize("case", ["periodic", "walls", "fields"])
 @pytest.mark.parametrize("m", [2, 5, 8])
-@pytest.mark.parametrize("threads", [256, 512])
+@pytest.mark.parametrize("threads", [256, 512, 704])
 def test_elast_stepn_matches_single_steps(case, m, threads):
     say("elasticity", case, "steps per launch", m, "threads per tile", threads)
     g = torch.Generator().manual_seed(2)

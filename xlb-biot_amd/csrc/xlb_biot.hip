@@ -856,6 +856,12 @@ constexpr int TILE_CPT = TILE * TILE / TILE_THREADS;
 // blocks per CU (their register budget is capped at 256), whose loads and barriers
 // overlap on grids of many tiles per CU.
 constexpr int TILE_THREADS_WIDE = 512;
+// One-wave tiles: 44 x 48 cells in blocks of 704 threads (3 cells each), the largest
+// region whose D2Q9 FP64 populations and rule table fit the 160 KB of LDS of a CU.  A
+// grid of up to about 16 x 15 such tiles (the 514^2 C2 grid at 6 steps per launch:
+// 255 tiles) runs as a single wave of blocks on the 256 CUs: every launch costs one
+// tile's load, steps and store instead of two or more waves of 32 x 32 tiles.
+constexpr int TILE_ONE_WAVE_X = 44, TILE_ONE_WAVE_Y = 48, TILE_THREADS_ONE_WAVE = 704;
 constexpr int TILE_MAX_STEPS = 8;
 
 __device__ __forceinline__ int wrapm(int i, int n) {
@@ -903,17 +909,19 @@ template <int Q> struct PullPlan {
   uint32_t w[(Q + 1) / 2];
 };
 
-template <class Lat, int Q, int TT>
+template <class Lat, int Q, int TX, int TY>
 __device__ __forceinline__ PullPlan<Q> make_pull_plan(int c, const int8_t *code) {
+  constexpr int TT = TX * TY;
+  static_assert(Q * TT <= (1 << 14), "pull plan offsets are 14 bits");
   PullPlan<Q> p;
-  const int rx = c / TILE, ry = c % TILE;
+  const int rx = c / TY, ry = c % TY;
 #pragma unroll
   for (int i = 0; i < (Q + 1) / 2; ++i) p.w[i] = 0u;
 #pragma unroll
   for (int d = 0; d < Q; ++d) {
     const int sx = rx - Lat::cx(d), sy = ry - Lat::cy(d);
-    const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
-    uint32_t off = (uint32_t)(d * TT + (in ? sx * TILE + sy : c)), sg = 1u;
+    const bool in = sx >= 0 && sx < TX && sy >= 0 && sy < TY;
+    uint32_t off = (uint32_t)(d * TT + (in ? sx * TY + sy : c)), sg = 1u;
     const int cc = code != nullptr ? code[d] : 0;
     if (cc == XB_RULE_ZERO) {
       sg = 0u;
@@ -940,15 +948,15 @@ __device__ __forceinline__ void pull_with_plan(const T *lds, const PullPlan<Q> &
 // Streaming inside a tile: population d of cell c comes from the upstream cell's
 // slot, or from the cell's own slot at the tile border (a halo cell, whose value is
 // not written out).
-template <typename T, class Lat>
+template <typename T, class Lat, int TX, int TY>
 __device__ __forceinline__ void tile_pull(const T *lds, int c, T (&v)[Lat::Q]) {
-  constexpr int TT = TILE * TILE;
-  const int rx = c / TILE, ry = c % TILE;
+  constexpr int TT = TX * TY;
+  const int rx = c / TY, ry = c % TY;
 #pragma unroll
   for (int d = 0; d < Lat::Q; ++d) {
     const int sx = rx - Lat::cx(d), sy = ry - Lat::cy(d);
-    const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
-    v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
+    const bool in = sx >= 0 && sx < TX && sy >= 0 && sy < TY;
+    v[d] = lds[d * TT + (in ? sx * TY + sy : c)];
   }
 }
 
@@ -962,12 +970,12 @@ __device__ __forceinline__ void tile_pull(const T *lds, int c, T (&v)[Lat::Q]) {
 // rules through the cells' own LDS slots.  The plain pull's LDS addresses are
 // immediate offsets from the cell's own.  BC == 1 (a table in global memory): per-cell
 // rules through the own slots.
-template <typename T, class Coll, int BC, bool WALLS, int NT, int CPT = TILE * TILE / NT>
+template <typename T, class Coll, int BC, bool WALLS, int TX, int TY, int NT, int CPT = TX * TY / NT>
 __device__ __forceinline__ void tile_step_loop(int steps, const Coll &coll, T (&f)[CPT][Coll::Q],
                                                const typename Coll::Aux (&aux)[CPT], const int (&slot)[CPT],
                                                bool walls, const BCDev<T> &bc, T *lds, TileRules<T, Coll::Q> *sh) {
   constexpr int Q = Coll::Q;
-  constexpr int TT = TILE * TILE;
+  constexpr int TT = TX * TY;
   PullPlan<Q> plan[WALLS ? CPT : 1];
   int cslot[WALLS ? CPT : 1];
   int row[CPT]; // BC == 2: the cell's rule row (cells without rules: the identity row)
@@ -977,7 +985,7 @@ __device__ __forceinline__ void tile_step_loop(int steps, const Coll &coll, T (&
     if constexpr (WALLS) {
       const int c = threadIdx.x + k * NT;
       const int8_t *code = (slot[k] >= 0 && bc.code_ps != nullptr) ? sh->code_ps + slot[k] * Q : nullptr;
-      plan[k] = make_pull_plan<Coll, Q, TT>(c, code);
+      plan[k] = make_pull_plan<Coll, Q, TX, TY>(c, code);
       cslot[k] = code != nullptr ? slot[k] : -1;
     }
   }
@@ -1020,7 +1028,7 @@ __device__ __forceinline__ void tile_step_loop(int steps, const Coll &coll, T (&
         }
       } else {
         T v[Q];
-        tile_pull<T, Coll>(lds, c, v);
+        tile_pull<T, Coll, TX, TY>(lds, c, v);
         if constexpr (BC == 2) {
           if (walls && bc.code_ps != nullptr)
             rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + row[k] * Q, sh->const_ps + row[k] * Q);
@@ -1037,18 +1045,18 @@ __device__ __forceinline__ void tile_step_loop(int steps, const Coll &coll, T (&
   }
 }
 
-template <typename T, class Coll, int BC, int NT>
+template <typename T, class Coll, int BC, int TX, int TY, int NT>
 __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int tiles_y, const Coll &coll,
                                                 const T *__restrict__ fin, T *__restrict__ fout, const BCDev<T> &bc,
                                                 T *lds, TileRules<T, Coll::Q> *sh) {
   constexpr int Q = Coll::Q;
-  constexpr int CPT = TILE * TILE / NT;
+  static_assert(TX * TY % NT == 0, "every thread owns the same number of cells");
+  constexpr int CPT = TX * TY / NT;
   constexpr bool HAS_BC = BC != 0;
   using Aux = typename Coll::Aux;
-  const int inner = TILE - 2 * steps;
   const int tile = xcd_block_id();
   const int bx = tile / tiles_y, by = tile % tiles_y;
-  const int x0 = bx * inner - steps, y0 = by * inner - steps;
+  const int x0 = bx * (TX - 2 * steps) - steps, y0 = by * (TY - 2 * steps) - steps;
   const size_t N = (size_t)nx * ny;
   if constexpr (BC == 2) {
     // the (small, deduplicated) rule table into LDS by the whole block, then the
@@ -1068,7 +1076,7 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
     const int c = threadIdx.x + k * NT;
-    const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
+    const int gx = wrapm(x0 + c / TY, nx), gy = wrapm(y0 + c % TY, ny);
     const size_t cell = (size_t)gx * ny + gy;
 #pragma unroll
     for (int d = 0; d < Q; ++d) f[k][d] = fin[d * N + cell];
@@ -1084,20 +1092,20 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
     const bool walls = __syncthreads_or(mine) != 0;
     if constexpr (NT == TILE_THREADS_WIDE) {
       if (walls)
-        tile_step_loop<T, Coll, 2, true, NT>(steps, coll, f, aux, slot, true, bc, lds, sh);
+        tile_step_loop<T, Coll, 2, true, TX, TY, NT>(steps, coll, f, aux, slot, true, bc, lds, sh);
       else
-        tile_step_loop<T, Coll, 2, false, NT>(steps, coll, f, aux, slot, false, bc, lds, sh);
+        tile_step_loop<T, Coll, 2, false, TX, TY, NT>(steps, coll, f, aux, slot, false, bc, lds, sh);
     } else {
-      tile_step_loop<T, Coll, 2, false, NT>(steps, coll, f, aux, slot, walls, bc, lds, sh);
+      tile_step_loop<T, Coll, 2, false, TX, TY, NT>(steps, coll, f, aux, slot, walls, bc, lds, sh);
     }
   } else {
-    tile_step_loop<T, Coll, BC, false, NT>(steps, coll, f, aux, slot, false, bc, lds, sh);
+    tile_step_loop<T, Coll, BC, false, TX, TY, NT>(steps, coll, f, aux, slot, false, bc, lds, sh);
   }
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
     const int c = threadIdx.x + k * NT;
-    const int rx = c / TILE, ry = c % TILE;
-    if (rx < steps || rx >= TILE - steps || ry < steps || ry >= TILE - steps) continue;
+    const int rx = c / TY, ry = c % TY;
+    if (rx < steps || rx >= TX - steps || ry < steps || ry >= TY - steps) continue;
     const int gx = x0 + rx, gy = y0 + ry;
     if (gx >= nx || gy >= ny) continue;
     const size_t cell = (size_t)gx * ny + gy;
@@ -1106,34 +1114,43 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
   }
 }
 
-template <typename T, class Coll, int BC, int NT>
+template <typename T, class Coll, int BC, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT, NT == TILE_THREADS ? 2 : 1)
 k_tile_steps(int nx, int ny, int steps, int tiles_y, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
              const BCDev<T> bc) {
-  __shared__ T lds[Coll::Q * TILE * TILE];
+  __shared__ T lds[Coll::Q * TX * TY];
   if constexpr (BC == 2) {
     __shared__ TileRules<T, Coll::Q> sh;
-    tile_steps_body<T, Coll, BC, NT>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, &sh);
+    tile_steps_body<T, Coll, BC, TX, TY, NT>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, &sh);
   } else {
-    tile_steps_body<T, Coll, BC, NT>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, nullptr);
+    tile_steps_body<T, Coll, BC, TX, TY, NT>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, nullptr);
   }
 }
 
-// Threads per tile block: 256 or 512 as asked; otherwise 512 for grids of at most
-// four tiles per CU (256 CUs), else 256.
-inline int tile_block_threads(int asked, int tiles) {
-  if (asked == TILE_THREADS || asked == TILE_THREADS_WIDE) return asked;
+// Tile shape of a multi-step launch from the asked threads per block: 256, 512 (32 x 32
+// tiles) or 704 (44 x 48); otherwise 512 for grids of at most four 32 x 32 tiles per
+// CU (256 CUs), else 256.
+inline int tile_block_threads(int asked, int nx, int ny, int steps) {
+  if (asked == TILE_THREADS || asked == TILE_THREADS_WIDE || asked == TILE_THREADS_ONE_WAVE) return asked;
+  const int tiles = cdiv(nx, TILE - 2 * steps) * cdiv(ny, TILE - 2 * steps);
   return tiles <= 4 * 256 ? TILE_THREADS_WIDE : TILE_THREADS;
 }
 
-template <typename T, class Coll, int NT>
-void launch_tile_steps_nt(dim3 grid, int nx, int ny, int steps, int ty, const Coll &coll, const T *fi, T *fo,
-                          const BCDev<T> &bc, hipStream_t st) {
-  const dim3 block(NT);
+template <typename T, class Coll, int TX, int TY, int NT>
+void launch_tile_steps_shape(int nx, int ny, int steps, const Coll &coll, const T *fi, T *fo, const BCDev<T> &bc,
+                             hipStream_t st) {
+  const int tx = cdiv(nx, TX - 2 * steps), ty = cdiv(ny, TY - 2 * steps);
+  const dim3 grid((unsigned)(tx * ty)), block(NT);
   switch (bc_mode(bc.slot, bc.nslots)) {
-  case 2: hipLaunchKernelGGL((k_tile_steps<T, Coll, 2, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
-  case 1: hipLaunchKernelGGL((k_tile_steps<T, Coll, 1, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
-  default: hipLaunchKernelGGL((k_tile_steps<T, Coll, 0, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc); break;
+  case 2:
+    hipLaunchKernelGGL((k_tile_steps<T, Coll, 2, TX, TY, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc);
+    break;
+  case 1:
+    hipLaunchKernelGGL((k_tile_steps<T, Coll, 1, TX, TY, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc);
+    break;
+  default:
+    hipLaunchKernelGGL((k_tile_steps<T, Coll, 0, TX, TY, NT>), grid, block, 0, st, nx, ny, steps, ty, coll, fi, fo, bc);
+    break;
   }
 }
 
@@ -1141,15 +1158,20 @@ template <typename T, class Coll>
 int launch_tile_steps(int nx, int ny, int steps, const Coll &coll, const void *fin, void *fout,
                       const xb_bc_table *bct, int threads, hipStream_t st) {
   const BCDev<T> bc = bcdev<T>(bct);
-  const int inner = TILE - 2 * steps;
-  const int tx = cdiv(nx, inner), ty = cdiv(ny, inner);
-  const dim3 grid((unsigned)(tx * ty));
   const T *fi = static_cast<const T *>(fin);
   T *fo = static_cast<T *>(fout);
-  if (tile_block_threads(threads, tx * ty) == TILE_THREADS_WIDE)
-    launch_tile_steps_nt<T, Coll, TILE_THREADS_WIDE>(grid, nx, ny, steps, ty, coll, fi, fo, bc, st);
-  else
-    launch_tile_steps_nt<T, Coll, TILE_THREADS>(grid, nx, ny, steps, ty, coll, fi, fo, bc, st);
+  switch (tile_block_threads(threads, nx, ny, steps)) {
+  case TILE_THREADS_ONE_WAVE:
+    launch_tile_steps_shape<T, Coll, TILE_ONE_WAVE_X, TILE_ONE_WAVE_Y, TILE_THREADS_ONE_WAVE>(nx, ny, steps, coll, fi,
+                                                                                              fo, bc, st);
+    break;
+  case TILE_THREADS_WIDE:
+    launch_tile_steps_shape<T, Coll, TILE, TILE, TILE_THREADS_WIDE>(nx, ny, steps, coll, fi, fo, bc, st);
+    break;
+  default:
+    launch_tile_steps_shape<T, Coll, TILE, TILE, TILE_THREADS>(nx, ny, steps, coll, fi, fo, bc, st);
+    break;
+  }
   return check_launch("multi-step tile kernel");
 }
 

@@ -83,16 +83,30 @@ def tune(key, launch_with_rows, candidates=CANDIDATES, reps=20):
 # Plain-step launch kinds (the value cached under a solver's "/multi" key): one step
 # per sweep, two per sweep (register temporal blocking), or m steps on LDS tiles.
 SWEEP1, SWEEP2 = 1, 2
-# tile launches: TILE_BASE * k + m steps per launch, k = 1 the library's block size,
-# k = 2 blocks of 256 threads, k = 3 blocks of 512 threads
+# tile launches: TILE_BASE * k + m steps per launch, k = 1 the library's tile shape,
+# k = 2: 32 x 32 tiles in blocks of 256 threads, k = 3: 32 x 32 in blocks of 512,
+# k = 4: 44 x 48 tiles in blocks of 704 threads
 TILE_BASE = 100
 TILE_STEPS = (2, 3, 4, 6, 8)
-TILE_THREADS = (256, 512)
+TILE_THREADS = (256, 512, 704)
+TILE_SHAPES = {256: (32, 32), 512: (32, 32), 704: (44, 48)}
 
 
 def tile_choice(m, threads=0):
-    k = {0: 1, 256: 2, 512: 3}[int(threads)]
+    k = {0: 1, 256: 2, 512: 3, 704: 4}[int(threads)]
     return TILE_BASE * k + int(m)
+
+
+def tile_blocks(nx, ny, m, threads):
+    """Blocks of a tile launch of ``m`` steps with ``threads`` threads per block."""
+    tx, ty = TILE_SHAPES[threads]
+    return -(-nx // (tx - 2 * m)) * -(-ny // (ty - 2 * m))
+
+
+def library_tile_threads(nx, ny, m):
+    """The library's own tile shape for ``threads`` 0: 512 threads for grids of at
+    most four 32 x 32 tiles per CU (256 CUs), else 256."""
+    return 512 if tile_blocks(nx, ny, m, 512) <= 4 * 256 else 256
 
 
 def steps_per_launch(choice):
@@ -102,7 +116,7 @@ def steps_per_launch(choice):
 
 def tile_threads(choice):
     """Threads per tile block of a tile ``choice`` (0: the library's choice)."""
-    return (0, 0, 256, 512)[choice // TILE_BASE]
+    return (0, 0, 256, 512, 704)[choice // TILE_BASE]
 
 
 def tune_multi(key, launches, reps=20):
