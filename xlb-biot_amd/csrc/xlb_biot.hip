@@ -863,6 +863,9 @@ constexpr int TILE_THREADS_WIDE = 512;
 // tile's load, steps and store instead of two or more waves of 32 x 32 tiles.
 constexpr int TILE_ONE_WAVE_X = 44, TILE_ONE_WAVE_Y = 48, TILE_THREADS_ONE_WAVE = 704;
 constexpr int TILE_MAX_STEPS = 8;
+// LDS slots on either side of a tile's population planes: the largest offset of an
+// upstream cell in LDS order (one row plus one cell).
+#define TILE_LDS_PAD(TY) ((TY) + 1)
 
 __device__ __forceinline__ int wrapm(int i, int n) {
   const int r = i % n;
@@ -948,16 +951,17 @@ __device__ __forceinline__ void pull_with_plan(const T *lds, const PullPlan<Q> &
 // Streaming inside a tile: population d of cell c comes from the upstream cell's
 // slot, or from the cell's own slot at the tile border (a halo cell, whose value is
 // not written out).
+// The upstream cell of a border cell lies outside the tile; the pull then reads the
+// neighbouring slot in LDS order (another row, or the adjacent direction's plane, or
+// the padding of TILE_LDS_PAD(TY) slots on either side of the array).  Any value will
+// do there: a border cell is in the tile's halo, whose error moves inward one cell per
+// step and never reaches the written inner cells, so the pull needs no bounds test
+// and every direction is one LDS read at a constant offset from the cell's own slot.
 template <typename T, class Lat, int TX, int TY>
 __device__ __forceinline__ void tile_pull(const T *lds, int c, T (&v)[Lat::Q]) {
   constexpr int TT = TX * TY;
-  const int rx = c / TY, ry = c % TY;
 #pragma unroll
-  for (int d = 0; d < Lat::Q; ++d) {
-    const int sx = rx - Lat::cx(d), sy = ry - Lat::cy(d);
-    const bool in = sx >= 0 && sx < TX && sy >= 0 && sy < TY;
-    v[d] = lds[d * TT + (in ? sx * TY + sy : c)];
-  }
+  for (int d = 0; d < Lat::Q; ++d) v[d] = lds[d * TT + c - (Lat::cx(d) * TY + Lat::cy(d))];
 }
 
 // The steps of one tile.  Blocks of 512 threads (PLAN): tiles that hold a rule cell of
@@ -1118,7 +1122,8 @@ template <typename T, class Coll, int BC, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT, NT == TILE_THREADS ? 2 : 1)
 k_tile_steps(int nx, int ny, int steps, int tiles_y, const Coll coll, const T *__restrict__ fin, T *__restrict__ fout,
              const BCDev<T> bc) {
-  __shared__ T lds[Coll::Q * TX * TY];
+  __shared__ T lds_raw[Coll::Q * TX * TY + 2 * TILE_LDS_PAD(TY)];
+  T *lds = lds_raw + TILE_LDS_PAD(TY);
   if constexpr (BC == 2) {
     __shared__ TileRules<T, Coll::Q> sh;
     tile_steps_body<T, Coll, BC, TX, TY, NT>(nx, ny, steps, tiles_y, coll, fin, fout, bc, lds, &sh);
@@ -1579,13 +1584,10 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
       }
       Aux ax = aux[k];
       if constexpr (HAS_NL) {
+        // as tile_pull: border cells read a neighbouring slot (they are in the halo)
         T qs[9];
 #pragma unroll
-        for (int d = 0; d < 9; ++d) {
-          const int sx = rx - D::cx(d), sy = ry - D::cy(d);
-          const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
-          qs[d] = ldiv[in ? sx * TILE + sy : c];
-        }
+        for (int d = 0; d < 9; ++d) qs[d] = ldiv[c - (D::cx(d) * TILE + D::cy(d))];
         if (gwalls) {
           // the gradient's rules read the cell's own divergence
           const int8_t *code = gsh->code + gslot[k] * 9;
@@ -1632,12 +1634,7 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
       const int c = threadIdx.x + k * SMOOTH_THREADS;
       const int rx = c / TILE, ry = c % TILE;
       T v[Q];
-#pragma unroll
-      for (int d = 0; d < Q; ++d) {
-        const int sx = rx - C::cx(d), sy = ry - C::cy(d);
-        const bool in = sx >= 0 && sx < TILE && sy >= 0 && sy < TILE;
-        v[d] = lds[d * TT + (in ? sx * TILE + sy : c)];
-      }
+      tile_pull<T, C, TILE, TILE>(lds, c, v);
       if constexpr (BC == 2) {
         if (walls && A.bc.code_ps != nullptr)
           rules_lds<T, Q, TT>(v, lds + c, sh->code_ps + slot[k] * Q, sh->const_ps + slot[k] * Q);
@@ -1673,9 +1670,11 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
 template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
 __device__ __forceinline__ void tile_smooth_run(const TileSmoothDev<T> &A, int tile) {
   constexpr bool WALLS = BC == 2;
-  __shared__ T lds[8 * TILE * TILE];
+  __shared__ T lds_raw[8 * TILE * TILE + 2 * TILE_LDS_PAD(TILE)];
+  T *lds = lds_raw + TILE_LDS_PAD(TILE);
   if constexpr (HAS_NL) {
-    __shared__ T ldiv[TILE * TILE];
+    __shared__ T ldiv_raw[TILE * TILE + 2 * TILE_LDS_PAD(TILE)];
+    T *ldiv = ldiv_raw + TILE_LDS_PAD(TILE);
     if constexpr (WALLS) {
       __shared__ TileRules<T, 8> sh;
       if constexpr (GWALLS) {
