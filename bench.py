@@ -621,6 +621,8 @@ def only(torch, args):
     elif args.only == "biot_vcycle":
         r = bench_biot_vcycle(torch, reps=args.steps)
         log(f"biot vcycle: {r}")
+    elif args.only == "c1":
+        print(json.dumps(bench_c1(torch)), flush=True)
     elif args.only == "biot_vcycle_eager":
         # the same V-cycle launched eagerly (no graph replay), for kernel traces
         r = bench_biot_vcycle(torch, reps=args.steps, eager=True)
@@ -660,7 +662,7 @@ def main():
                     help="with --tile-steps: threads per tile block, which fixes the tile shape (256, 512: "
                          "32 x 32; 704: 44 x 48; 0: the library's choice)")
     ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "adjoint", "biot_vcycle",
-                                       "biot_vcycle_eager", "biot_adjoint"],
+                                       "biot_vcycle_eager", "biot_adjoint", "c1"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
