@@ -1702,7 +1702,7 @@ __device__ __forceinline__ void tile_smooth_run(const TileSmoothDev<T> &A, int t
 }
 
 template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
-__global__ void __launch_bounds__(SMOOTH_THREADS) k_tile_smooth(const TileSmoothDev<T> A) {
+__global__ void __launch_bounds__(SMOOTH_THREADS, HAS_RES ? 2 : 1) k_tile_smooth(const TileSmoothDev<T> A) {
   tile_smooth_run<T, BC, HAS_RES, HAS_NL, GWALLS>(A, xcd_block_id());
 }
 
@@ -1713,7 +1713,7 @@ template <typename T> struct TileSmoothJob {
 };
 
 template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
-__global__ void __launch_bounds__(SMOOTH_THREADS)
+__global__ void __launch_bounds__(SMOOTH_THREADS, HAS_RES ? 2 : 1)
 k_tile_smooth_batch(const TileSmoothJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
                     const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
   const int b = xcd_block_id();
