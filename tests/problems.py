@@ -233,6 +233,33 @@ def darcy_c2(kind, n=32, precision="f64/f64", heterogeneous=True, seed=0, varian
                  "lattice": m["D2Q9"](precision), "print_info_rate": 0})
 
 
+def darcy_obstacle(kind, n=14, precision="f64/f64"):
+    """The C2 walls around an interior solid obstacle: an L-shaped voxel set whose
+    surface holds p = 0.3 (anti-bounce-back on solid cells, so the fluid cells around
+    it take their missing directions from the grid mask, diagonals included)."""
+    m = modules(kind)
+    ADR_ABB, ADR_BB = m["ADR_ABB"], m["ADR_BB"]
+    nx = ny = n + 2
+
+    class Obstacle(m["Darcy"]):
+        def set_boundary_conditions(self):
+            bb = self.boundingBoxIndices
+            self.BCs.append(ADR_ABB(tuple(bb["left"].T), self.gridInfo, self.precisionPolicy, 1.0))
+            self.BCs.append(ADR_ABB(tuple(bb["right"].T), self.gridInfo, self.precisionPolicy, 0.0))
+            self.BCs.append(ADR_BB(tuple(bb["bottom"].T), self.gridInfo, self.precisionPolicy))
+            self.BCs.append(ADR_BB(tuple(bb["top"].T), self.gridInfo, self.precisionPolicy))
+            block = [(x, y) for x in range(6, 9) for y in range(5, 11)] + [(x, 5) for x in range(9, 11)]
+            self.BCs.append(ADR_ABB(tuple(np.array(block).T), self.gridInfo, self.precisionPolicy, 0.3))
+
+        def initialize_macroscopic_fields(self):
+            x = (np.arange(self.nx)[:, None, None] - 0.5) / (self.nx - 2)
+            return np.clip(1.0 - x, 0.0, 1.0) * np.ones((1, self.ny, 1)), np.zeros((self.nx, self.ny, 2))
+
+    K = lognormal_field(nx, ny, 1.0, 4.0, 2)
+    return Obstacle(**{"omega": (1.0 / (3.0 * K + 0.5)).reshape(nx, ny, 1), "nx": nx, "ny": ny, "nz": 0,
+                       "precision": precision, "lattice": m["D2Q9"](precision), "print_info_rate": 0})
+
+
 def darcy_equilibrium(kind, n=16, precision="f64/f64"):
     """examples/darcy/dirichlet_eq.py-like: equilibrium walls with a manufactured source."""
     m = modules(kind)

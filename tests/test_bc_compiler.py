@@ -47,6 +47,7 @@ def _pairs():
     yield "consolidation", lambda k: P.biot_consolidation(k, nx=16, levels=3)
     yield "darcy_c2", lambda k: P.darcy_c2(k, n=10)
     yield "darcy_eq", lambda k: P.darcy_equilibrium(k, n=8)
+    yield "darcy_obstacle", lambda k: P.darcy_obstacle(k)
 
 
 @pytest.mark.parametrize("name,make", list(_pairs()))

@@ -143,6 +143,26 @@ def test_darcy_steps(precision, hetero, source):
         assert rel(g.gradient(rg, scale), o.gradient(ro, scale)) < TOL[precision]
 
 
+@pytest.mark.parametrize("threads", [0, 256, 704])
+def test_darcy_interior_obstacle(threads):
+    """An arbitrary solid voxel set (an L-shaped obstacle with a Dirichlet surface)
+    inside the C2 walls: single steps, then tile launches of 4 steps, against the
+    oracle's literal boundary passes."""
+    say("darcy interior obstacle, tile threads", threads)
+    g, o = P.darcy_obstacle("gpu"), P.darcy_obstacle("oracle")
+    fg, fo = g.assign_fields_sharded(), o.assign_fields_sharded()
+    for t in range(12):
+        fg, _ = g.step(fg, t)
+        fo, _ = o.step(fo, t)
+    assert rel(fg, fo) < 1e-12
+    fs = g._soa(fg, 9)
+    for _ in range(3):
+        fs = g._stepn_into(fs, torch.empty_like(fs), 4, threads=threads)
+    for t in range(12, 24):
+        fo, _ = o.step(fo, t)
+    assert rel(fs.permute(1, 2, 0), fo) < 1e-12
+
+
 def test_darcy_equilibrium_walls():
     say("darcy equilibrium walls")
     g, o = P.darcy_equilibrium("gpu"), P.darcy_equilibrium("oracle")
