@@ -173,6 +173,36 @@ def dirichlet_elasticity(kind, n=16, levels=1, lam=0.5, mu=0.4, precision="f64/f
                         "print_info_rate": 0})
 
 
+def obstacle_elasticity(kind, n=16, lam=0.5, mu=0.4, precision="f64/f64"):
+    """The four full-way Dirichlet walls of ``dirichlet_elasticity`` around an interior
+    L-shaped solid obstacle held at zero displacement (halfway bounce-back on solid
+    cells: the fluid cells around it take their missing directions from the grid
+    mask, diagonals included); single grid."""
+    m = modules(kind)
+    EBBF, EBB = m["EBBF"], m["EBB"]
+
+    class Obstacle(m["Elasticity"]):
+        def get_force(self):
+            return _wall_force(self)
+
+        def set_boundary_conditions(self):
+            ew = np.concatenate((sum_function(self.nx, self.ny, [[], [2, 2], [], []], lambda k, l: 9e-4, True),
+                                 sum_function(self.nx, self.ny, [[], [], [2, 2], []], lambda k, l: 7e-4, True)), -1)
+            bb = self.boundingBoxIndices
+            self.BCs.append(EBBF(tuple(bb["left"].T), self.gridInfo, self.precisionPolicy, ew[-1, ...]))
+            self.BCs.append(EBBF(tuple(bb["right"].T), self.gridInfo, self.precisionPolicy, ew[0, ...]))
+            self.BCs.append(EBBF(tuple(bb["bottom"].T), self.gridInfo, self.precisionPolicy, ew[:, -1, ...]))
+            self.BCs.append(EBBF(tuple(bb["top"].T), self.gridInfo, self.precisionPolicy, ew[:, 0, ...]))
+            block = [(x, y) for x in range(6, 9) for y in range(6, 12)] + [(x, 6) for x in range(9, 12)]
+            # one displacement vector for the whole obstacle (the per-cell form indexes the
+            # given rows with the shifted cells' positions, elasticity.py:1604-1608)
+            self.BCs.append(EBB(tuple(np.array(block).T), self.gridInfo, self.precisionPolicy,
+                                np.array([2e-4, -1e-4])))
+
+    return Obstacle(**{"lambda": lam, "mu": mu, "nx": n + 2, "ny": n + 2, "nz": 0, "precision": precision,
+                       "lattice": m["D2Q8"](precision), "print_info_rate": 0})
+
+
 def mixed_halfway_elasticity(kind, nx=18, ny=12, precision="f64/f64", lam=1.0, mu=0.7):
     """Halfway Dirichlet (``ElasticityBounceBack``) on the left, traction on the right,
     periodic in y: exercises the duplicate-index scatters of the halfway bounce-back

@@ -48,6 +48,7 @@ def _pairs():
     yield "darcy_c2", lambda k: P.darcy_c2(k, n=10)
     yield "darcy_eq", lambda k: P.darcy_equilibrium(k, n=8)
     yield "darcy_obstacle", lambda k: P.darcy_obstacle(k)
+    yield "elastic_obstacle", lambda k: P.obstacle_elasticity(k)
 
 
 @pytest.mark.parametrize("name,make", list(_pairs()))

@@ -63,6 +63,29 @@ def test_elasticity_periodic_nonsquare_and_ragged():
         assert rel(fg, fo) < 1e-12, (nx, ny)
 
 
+@pytest.mark.parametrize("threads", [0, 256, 704])
+def test_elasticity_interior_obstacle(threads):
+    """An interior L-shaped solid obstacle with a prescribed displacement (halfway
+    bounce-back on solid cells) inside full-way Dirichlet walls: single steps, then
+    tile launches of 4 steps, against the oracle's literal boundary passes."""
+    say("elasticity interior obstacle, tile threads", threads)
+    g, o = P.obstacle_elasticity("gpu"), P.obstacle_elasticity("oracle")
+    rng = np.random.default_rng(9)
+    f0 = rng.standard_normal((g.nx, g.ny, 8)) * 1e-3
+    fg = torch.as_tensor(f0, device="cuda")
+    fo = torch.as_tensor(f0)
+    for t in range(12):
+        fg, _ = g.step(fg, t)
+        fo, _ = o.step(fo, t)
+    assert rel(fg, fo) < 1e-12
+    fs = g._soa(fg, 8)
+    for _ in range(3):
+        fs = g._stepn_into(fs, torch.empty_like(fs), 4, threads=threads)
+    for t in range(12, 24):
+        fo, _ = o.step(fo, t)
+    assert rel(fs.permute(1, 2, 0), fo) < 1e-12
+
+
 @pytest.mark.parametrize("name,precision", [("neumann", "f64/f64"), ("dirichlet", "f64/f64"),
                                             ("mixed", "f64/f64"), ("neumann", "f32/f32"), ("mixed", "f32/f32")])
 def test_elasticity_walls_steps(name, precision):
