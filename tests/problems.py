@@ -321,6 +321,37 @@ def darcy_equilibrium(kind, n=16, precision="f64/f64"):
 # ---------------------------------------------------------------------------
 # Biot
 
+def mean_rate(darcy):
+    """The Darcy solver's mean relaxation rate: the scalar ``omega``, or the mean of a
+    per-cell ``omega`` field (the solver keeps the mean in ``omega``, the oracle keeps
+    the field itself)."""
+    om = darcy.omega
+    if isinstance(om, float):
+        return om
+    return float(np.asarray(om.detach().cpu() if hasattr(om, "detach") else om, dtype=np.float64).mean())
+
+
+def step_rel(out, ref, against_state=()):
+    """Relative error of each output of a coupled Biot step against the oracle's:
+    ``max|a - b| / max|b|``.  An output that vanishes in exact arithmetic (``max|b|``
+    below 1e-8 of the populations' scale ``S = max(max|f_el|, max|f_darcy|)``) is
+    measured against ``S`` instead: its entries are round-off of sums of
+    populations.  So are the outputs listed in ``against_state`` (indices into the
+    step's 7-tuple), such as the divergence rate (output 4), which is formed from
+    the difference of two consecutive states and so carries their rounding."""
+    import torch
+    cpu = [torch.as_tensor(x).detach().to("cpu", torch.float64) for x in out]
+    ref = [torch.as_tensor(x).detach().to("cpu", torch.float64) for x in ref]
+    S = max(ref[0].abs().max().item(), ref[1].abs().max().item())
+    errs = []
+    for k, (a, b) in enumerate(zip(cpu, ref)):
+        assert a.shape == b.shape, (a.shape, b.shape)
+        m = b.abs().max().item()
+        den = m if m > 1e-8 * S and k not in against_state else S
+        errs.append((a - b).abs().max().item() / (den if den > 0 else 1.0))
+    return errs
+
+
 def biot_periodic(kind, nx=32, levels=1, steps=1, precision="f64/f64", coupling="centered", alt=False,
                   lam=1.0, mu=1.0, alpha=1.0, K=1.0, K_field=None, k_smooth=None, gamma=None, device_hooks=False):
     """examples/poroelasticity/periodic.py: manufactured time-dependent solution.
@@ -352,7 +383,7 @@ def biot_periodic(kind, nx=32, levels=1, steps=1, precision="f64/f64", coupling=
             return pats[name]
 
         def get_elastic_force(self, timestep):
-            Kd = (1.0 / self.darcy.omega - 0.5) / 3
+            Kd = (1.0 / mean_rate(self.darcy) - 0.5) / 3
             tf = 1 - np.exp(-8 * np.pi ** 2 * Kd * timestep / self.elasticity.nx ** 2)
             if device_hooks:
                 import torch
@@ -367,7 +398,7 @@ def biot_periodic(kind, nx=32, levels=1, steps=1, precision="f64/f64", coupling=
             return np.concatenate((fx, np.zeros_like(fx)), -1)
 
         def get_darcy_source(self, timestep):
-            Kd = (1.0 / self.darcy.omega - 0.5) / 3
+            Kd = (1.0 / mean_rate(self.darcy) - 0.5) / 3
             tf = np.exp(-8 * np.pi ** 2 * Kd * timestep / self.elasticity.nx ** 2)
             factor = -np.pi ** 3 * Kd / self.darcy.nx ** 2 * (
                 128 * self.alpha * tf + (128 * self.elasticity.lamda + 240 * self.elasticity.mu) / self.alpha)

@@ -26,7 +26,7 @@ def _single(step, f, n):
 
 @pytest.mark.parametrize("n,precision,hetero", [(514, "f64/f64", True), (70, "f32/f32", True), (16, "f64/f64", False),
                                                (5, "f64/f64", False)])
-@pytest.mark.parametrize("m", [1, 3, 4, 8])
+@pytest.mark.parametrize("m", [1, 3, 4, 6, 8])
 @pytest.mark.parametrize("threads", [256, 512, 704])
 def test_darcy_stepn_matches_single_steps(n, precision, hetero, m, threads):
     say("darcy", n, precision, "hetero", hetero, "steps per launch", m, "threads per tile", threads)

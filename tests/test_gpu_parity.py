@@ -214,8 +214,29 @@ def test_biot_periodic_steps(levels, coupling, alt, fused):
     for t in range(3):
         outg = g.step(fe, fd, dde, t)
         outo = o.step(oe, od, odde, t)
-        for k, (a, b) in enumerate(zip(outg, outo)):
-            assert rel(a, b, 1e-3) < 1e-11, (t, k, rel(a, b, 1e-3))
+        errs = P.step_rel(outg, outo)
+        assert max(errs) < 1e-12, (t, errs)
+        fe, fd, dde = outg[0], outg[1], outg[4]
+        oe, od, odde = outo[0], outo[1], outo[4]
+
+
+def test_biot_consolidation_mg_walls_fp32():
+    # the consolidation set-up in FP32 state (the precision of the gradient
+    # workload) against the FP32 oracle: 1e-5; the divergence rate is measured
+    # against the populations' scale (it is formed from a difference of states)
+    say("biot consolidation MG walls, FP32")
+    g = P.biot_consolidation("gpu", nx=32, levels=3, steps=2, precision="f32/f32")
+    o = P.biot_consolidation("oracle", nx=32, levels=3, steps=2, precision="f32/f32")
+    sg = g.initial_state()
+    so = o.initial_state()
+    fe, fd, dde = (t.permute(1, 2, 0) for t in sg)
+    oe, od, odde = so
+    for t in range(3):
+        outg = g.step(fe, fd, dde, t)
+        outo = o.step(oe, od, odde, t)
+        errs = P.step_rel(outg, outo, against_state=(4,))
+        say("  step", t, "relative errors", [f"{e:.1e}" for e in errs])
+        assert max(errs) < 1e-5, (t, errs)
         fe, fd, dde = outg[0], outg[1], outg[4]
         oe, od, odde = outo[0], outo[1], outo[4]
 
@@ -233,8 +254,8 @@ def test_biot_consolidation_mg_walls(fused):
     for t in range(3):
         outg = g.step(fe, fd, dde, t)
         outo = o.step(oe, od, odde, t)
-        for k, (a, b) in enumerate(zip(outg, outo)):
-            assert rel(a, b, 1e-3) < 1e-11, (t, k, rel(a, b, 1e-3))
+        errs = P.step_rel(outg, outo)
+        assert max(errs) < 1e-12, (t, errs)
         fe, fd, dde = outg[0], outg[1], outg[4]
         oe, od, odde = outo[0], outo[1], outo[4]
 
@@ -292,8 +313,8 @@ def test_biot_residual_driven_inner_loop_matches_oracle():
         o.elasticitySteps = g.inner_counts[-1]
         outo = o.step(oe, od, odde, t)
         assert g.inner_counts[-1] > 1
-        for k, (a, b) in enumerate(zip(outg, outo)):
-            assert rel(a, b, 1e-3) < 1e-11, (t, k)
+        errs = P.step_rel(outg, outo)
+        assert max(errs) < 1e-12, (t, errs)
         fe, fd, dde = outg[0], outg[1], outg[4]
         oe, od, odde = outo[0], outo[1], outo[4]
 

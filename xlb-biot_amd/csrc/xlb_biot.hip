@@ -16,6 +16,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <exception>
+#include <new>
 #include <type_traits>
 #include <vector>
 
@@ -39,6 +41,22 @@ int check_launch(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(XB_EHIP, "%s: %s", what, hipGetErrorString(e));
   return XB_OK;
+}
+
+// Runs `body` (returning an int status) and turns any C++ exception into an error
+// status, so none crosses the C ABI (an exception leaving an extern "C" function
+// would end the process).  Used by the entry points that build host-side tables
+// in std containers; the others call only the HIP C API.
+template <class F> int guarded(const char *what, F &&body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return fail(XB_EHIP, "%s: host memory allocation failed", what);
+  } catch (const std::exception &e) {
+    return fail(XB_EHIP, "%s: %s", what, e.what());
+  } catch (...) {
+    return fail(XB_EHIP, "%s: unexpected C++ exception", what);
+  }
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -3195,8 +3213,10 @@ int64_t xb_batch_table_bytes(int32_t kind, int32_t dtype, const void *jobs, int3
   if (check_dtype(dtype) != XB_OK || jobs == nullptr || njobs <= 0) return -1;
   if (kind < XB_BATCH_ELAST_STEP || kind > XB_BATCH_ELAST_SMOOTHN) return -1;
   int64_t need = -1;
-  const int r = dtype == XB_F64 ? batch_prepare_t<double>(kind, jobs, nullptr, njobs, nullptr, 0, nullptr, nullptr, &need)
-                                : batch_prepare_t<float>(kind, jobs, nullptr, njobs, nullptr, 0, nullptr, nullptr, &need);
+  const int r = guarded("xb_batch_table_bytes", [&] {
+    return dtype == XB_F64 ? batch_prepare_t<double>(kind, jobs, nullptr, njobs, nullptr, 0, nullptr, nullptr, &need)
+                           : batch_prepare_t<float>(kind, jobs, nullptr, njobs, nullptr, 0, nullptr, nullptr, &need);
+  });
   return r == XB_OK ? need : -1;
 }
 
@@ -3210,8 +3230,10 @@ int xb_batch_prepare(int32_t kind, int32_t dtype, const void *jobs, const int32_
       static_cast<const xb_elast_args *>(jobs)[0].dtype != dtype)
     return fail(XB_EINVAL, "xb_batch_prepare: job dtype differs from the batch dtype");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  return dtype == XB_F64 ? batch_prepare_t<double>(kind, jobs, inst, njobs, table, table_bytes, plan, st, nullptr)
-                         : batch_prepare_t<float>(kind, jobs, inst, njobs, table, table_bytes, plan, st, nullptr);
+  return guarded("xb_batch_prepare", [&] {
+    return dtype == XB_F64 ? batch_prepare_t<double>(kind, jobs, inst, njobs, table, table_bytes, plan, st, nullptr)
+                           : batch_prepare_t<float>(kind, jobs, inst, njobs, table, table_bytes, plan, st, nullptr);
+  });
 }
 
 int xb_batch_launch(const xb_batch_plan *plan, const int32_t *active, void *stream) {
@@ -3534,10 +3556,11 @@ int xb_vcycle_tail_prepare(int32_t dtype, double lam, double mu, double gamma, i
       D.resid = static_cast<T *>(L.resid);
       D.pc = static_cast<T *>(L.scratch);
     }
-    return hipMemcpy(table, v.data(), sizeof(TailLevelDev<T>) * v.size(), hipMemcpyHostToDevice);
+    const hipError_t e = hipMemcpy(table, v.data(), sizeof(TailLevelDev<T>) * v.size(), hipMemcpyHostToDevice);
+    return e == hipSuccess ? XB_OK : fail(XB_EHIP, "xb_vcycle_tail_prepare: %s", hipGetErrorString(e));
   };
-  const hipError_t e = dtype == XB_F64 ? fill(double(0)) : fill(float(0));
-  if (e != hipSuccess) return fail(XB_EHIP, "xb_vcycle_tail_prepare: %s", hipGetErrorString(e));
+  if (int r = guarded("xb_vcycle_tail_prepare", [&] { return dtype == XB_F64 ? fill(double(0)) : fill(float(0)); }))
+    return r;
   plan->dtype = dtype;
   plan->nlevels = nlevels;
   plan->k_smooth = k_smooth;

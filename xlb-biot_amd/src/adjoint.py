@@ -32,6 +32,7 @@ import math
 import torch
 
 from . import _native as nat
+from .base import graph_capture
 
 
 def _f64_zeros(shape, device):
@@ -216,7 +217,7 @@ class CheckpointedSolve:
         fn()  # eager warm-up outside the capture
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with graph_capture(g):
             fn()
         return g
 

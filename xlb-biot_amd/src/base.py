@@ -11,6 +11,8 @@ solver uses (construction :44-146, boundary data :351-372,
 ``distributed_array_init`` :391-411, grid mask :413-449, bounding box
 :451-485, precisions :487-510, ``assign_fields_sharded`` :528-559).
 """
+import contextlib
+import gc
 import time
 
 import numpy as np
@@ -93,6 +95,44 @@ def view(soa):
     return soa.permute(1, 2, 0)
 
 
+def capturing():
+    """True while the current stream records a HIP graph."""
+    return torch.cuda.is_available() and torch.cuda.is_current_stream_capturing()
+
+
+def no_capture(what):
+    """Raise if called while a graph is being captured (for set-up work that copies
+    host data synchronously or allocates outside the capture's memory pool)."""
+    if capturing():
+        raise RuntimeError(f"{what} must be created before a graph capture, not inside one")
+
+
+@contextlib.contextmanager
+def graph_capture(graph):
+    """``torch.cuda.graph(graph)`` with Python's cyclic garbage collector held off.
+
+    Solvers cache runners, adjoint sessions and V-cycle graphs that refer back to
+    them; such reference cycles are freed only by the cyclic collector, whenever an
+    allocation happens to trigger it.  Run inside a capture, that collection would
+    destroy ``CUDAGraph`` objects (HIP graph executables, their memory pools and
+    the device buffers they own) while a global-mode stream capture is in
+    progress, which the HIP runtime does not support.  So pending cycles are
+    collected first, outside the capture, and the collector stays disabled until
+    the capture has ended."""
+    if capturing():
+        raise RuntimeError("graph captures do not nest")
+    gc.collect()
+    torch.cuda.synchronize()
+    was_enabled = gc.isenabled()
+    gc.disable()
+    try:
+        with torch.cuda.graph(graph):
+            yield graph
+    finally:
+        if was_enabled:
+            gc.enable()
+
+
 class GraphedSteps:
     """``n`` (even) ping-pong steps ``launch(src, dst)`` captured once into a HIP graph
     (through ``torch.cuda.CUDAGraph``) and replayed: one launch for the whole chunk.
@@ -104,7 +144,7 @@ class GraphedSteps:
         self.n = n
         self.buf_a, self.buf_b = buf_a, buf_b
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
+        with graph_capture(self.graph):
             for _ in range(n // 2):
                 launch(buf_a, buf_b)
                 launch(buf_b, buf_a)
@@ -162,6 +202,7 @@ class NormWorkspace:
     """FP64 partial sums + the finalised (norm, non-finite flag) pair."""
 
     def __init__(self, n, device):
+        no_capture("NormWorkspace")
         self.partials = torch.zeros(max(int(n), 1), dtype=torch.float64, device=device)
         self.count = nat.c_int32(0)
 

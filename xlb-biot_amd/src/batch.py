@@ -31,6 +31,7 @@ import torch
 
 from . import _native as nat
 from . import autotune
+from .base import graph_capture
 
 
 class BatchPlan:
@@ -180,7 +181,7 @@ class ElasticityBatch:
         self.graphs = []
         for src in (0, 1):
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
+            with graph_capture(g):
                 self._run_schedule(src)
             self.graphs.append(g)
         torch.cuda.synchronize()

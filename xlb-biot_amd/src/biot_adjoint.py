@@ -35,6 +35,7 @@ derivative); the initial state is taken as given (not differentiated).
 import torch
 
 from . import _native as nat
+from .base import graph_capture
 
 
 def _zeros(shape, dtype, device):
@@ -102,7 +103,7 @@ class BiotAdjointSession:
         body()  # eager once: the gradient wall tables of the coarse grids exist before capture
         torch.cuda.synchronize()
         self.coarse_graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.coarse_graph):
+        with graph_capture(self.coarse_graph):
             body()
         # the coarse solve of a recomputed V-cycle (residual of the pre-smoothed state,
         # its restriction, the coarse V-cycle from zero) from static inputs
@@ -113,7 +114,7 @@ class BiotAdjointSession:
         el._coarse_solve_nl(self.cs_f, self.cs_div, 0, None, self.cs_base, self.nlf)
         torch.cuda.synchronize()
         self.solve_graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.solve_graph):
+        with graph_capture(self.solve_graph):
             self.cs_out, _ = el._coarse_solve_nl(self.cs_f, self.cs_div, 0, None, self.cs_base, self.nlf)
 
     # ---- parameters

@@ -206,6 +206,9 @@ class RuleTable:
     ``xb_bc_table`` struct handed to the kernels."""
 
     def __init__(self, nx, ny, q, slots, code_pc, const_pc, code_ps, const_ps, dtype, device):
+        if torch.cuda.is_available() and torch.cuda.is_current_stream_capturing():
+            # the tables are copied from host memory: build them before any capture
+            raise RuntimeError("RuleTable must be created before a graph capture, not inside one")
         self.nx, self.ny, self.q = nx, ny, q
         self.ncells = int(len(slots))
         # cells whose rule rows (codes and constants, compared bit for bit) coincide

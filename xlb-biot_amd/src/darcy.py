@@ -9,6 +9,7 @@ Extension: ``omega`` may also be a per-cell field (shape ``(nx, ny)`` or
 reproduces the reference exactly.  Advection (``vel``) is not on this path.
 """
 import time
+import weakref
 
 import numpy as np
 import torch
@@ -197,11 +198,14 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         if choice == 0:
             choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
         m = autotune.steps_per_launch(choice)
+        # the launchers are kept by runners the solver owns: a weak proxy keeps them
+        # from forming a reference cycle with it
+        me = weakref.proxy(self)
         if choice > autotune.TILE_BASE:
             nt = autotune.tile_threads(choice)
-            return choice, m, (lambda s, d: self._stepn_into(s, d, m, threads=nt))
+            return choice, m, (lambda s, d: me._stepn_into(s, d, m, threads=nt))
         if choice == autotune.SWEEP2:
-            return choice, 2, (lambda s, d: self._step2_into(s, d))
+            return choice, 2, (lambda s, d: me._step2_into(s, d))
         return choice, 1, None
 
     def _step_soa(self, f, ext_source=None, fpc=False, out=None):
@@ -218,7 +222,8 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         key = ("steps", chunk, choice)
         runner = self._runners.get(key)
         if runner is None:
-            runner = self._runners[key] = StepRunner(lambda s, d: self._step_soa(s, None, False, out=d), fs, chunk,
+            me = weakref.proxy(self)
+            runner = self._runners[key] = StepRunner(lambda s, d: me._step_soa(s, None, False, out=d), fs, chunk,
                                                      multi, m)
         return view(runner.run(fs, n))
 

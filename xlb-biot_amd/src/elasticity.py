@@ -9,13 +9,15 @@ signatures and return values.  Every step runs on the GPU through the C ABI of
 """
 import ctypes
 import time
+import weakref
 
 import numpy as np
 import torch
 
 from . import _native as nat
 from . import autotune
-from .base import NormWorkspace, StepRunner, PrecisionPolicy, as_soa, default_device, torch_dtype, view
+from .base import (NormWorkspace, StepRunner, PrecisionPolicy, as_soa, default_device, graph_capture, no_capture,
+                   torch_dtype, view)
 from .boundary_conditions import BoundaryCondition, build_rule_table, configure_bcs, create_grid_mask
 
 
@@ -89,6 +91,13 @@ class LBMElasticity(object):
     def _fields(self):
         return nat.ptr(self._lam_field), nat.ptr(self._mu_field)
 
+    def _drop_plans(self):
+        """Forget captured step loops, coarse-level tails and adjoint sessions: they
+        hold lambda, mu, gamma and k_smooth by value (called by those setters)."""
+        for name in ("_runners", "_tails", "_adjoint_sessions"):
+            if name in self.__dict__:
+                self.__dict__[name] = {}
+
     @property
     def _tdt(self):
         return self.precisionPolicy.torch_compute
@@ -119,6 +128,7 @@ class LBMElasticity(object):
         if not isinstance(value, float):
             raise TypeError("lambda must be a float")
         self._lamda = value
+        self._drop_plans()
 
     @property
     def mu(self):
@@ -131,6 +141,7 @@ class LBMElasticity(object):
         if not isinstance(value, float):
             raise TypeError("mu must be a float")
         self._mu = value
+        self._drop_plans()
 
     @property
     def levels(self):
@@ -162,6 +173,7 @@ class LBMElasticity(object):
         if value < 0.0 or value > 1.0:
             raise ValueError("gamma must be between 0 and 1")
         self._gamma = value
+        self._drop_plans()
 
     @property
     def k_smooth(self):
@@ -176,6 +188,7 @@ class LBMElasticity(object):
         if value < 1:
             raise ValueError("k_smooth must be positive")
         self._k_smooth = value
+        self._drop_plans()
 
     @property
     def mg_BCs(self):
@@ -453,11 +466,14 @@ class LBMElasticity(object):
         if choice == 0:
             choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
         m = autotune.steps_per_launch(choice)
+        # the launchers are kept by runners the solver owns: a weak proxy keeps them
+        # from forming a reference cycle with it
+        me = weakref.proxy(self)
         if choice > autotune.TILE_BASE:
             nt = autotune.tile_threads(choice)
-            return choice, m, (lambda s, d: self._stepn_into(s, d, m, ext, nt))
+            return choice, m, (lambda s, d: me._stepn_into(s, d, m, ext, nt))
         if choice == autotune.SWEEP2:
-            return choice, 2, (lambda s, d: self._step2_into(s, d, ext))
+            return choice, 2, (lambda s, d: me._step2_into(s, d, ext))
         return choice, 1, None
 
     def run_steps(self, f, n, chunk=64, external_force=None):
@@ -472,7 +488,8 @@ class LBMElasticity(object):
         key = ("steps", chunk, external_force is None, choice)
         runner = self._runners.get(key)
         if runner is None or external_force is not None:
-            runner = StepRunner(lambda s, d: self._step_into(s, d, ext), fs, chunk, multi, m)
+            me = weakref.proxy(self)
+            runner = StepRunner(lambda s, d: me._step_into(s, d, ext), fs, chunk, multi, m)
             if external_force is None:
                 self._runners[key] = runner
         return view(runner.run(fs, n))
@@ -593,7 +610,10 @@ class LBMElasticity(object):
         """The tail that solves level ``level`` and below, if it starts there."""
         if self._tail_start() != level:
             return None
-        key = (level, None if nlf is None else float(nlf.scale))
+        # the plan holds lambda, mu, gamma, k_smooth and the coupling's scale by value
+        # and the Darcy solver's gradient wall tables by address
+        key = (level, None if nlf is None else (id(nlf.darcy), float(nlf.scale)), self.lamda, self.mu, self.gamma,
+               self.k_smooth)
         tails = self.__dict__.setdefault("_tails", {})
         if key not in tails:
             tails[key] = VCycleTail(self, level, nlf)
@@ -871,6 +891,7 @@ class VCycleTail:
     ``start - 1`` is written into :attr:`resid`; the result is :attr:`result`."""
 
     def __init__(self, el, start, nlf):
+        no_capture("VCycleTail")  # its device table is copied from host memory
         lib = nat.lib()
         coupled = nlf is not None
         self.levels = list(range(start, el.levels))
@@ -878,6 +899,8 @@ class VCycleTail:
         dt, dev = el._tdt, el.device
         arr = (nat.TailLevel * n)()
         self._keep = []
+        # the rule tables whose device addresses the plan's table holds
+        self._tables = []
         for i, lvl in enumerate(self.levels):
             nx, ny = el.grid_size(lvl)
             x0, x1, y0, y1 = el.interior_box(lvl)
@@ -890,6 +913,7 @@ class VCycleTail:
                 b["div1"] = torch.zeros((1, nx, ny), dtype=dt, device=dev)
             table = el._tables.get(lvl)
             gt = nlf.darcy._grad_table(nx, ny) if coupled else None
+            self._tables += [table, gt]
             arr[i] = nat.TailLevel(nx, ny, x0, x1, y0, y1, table.ref() if table is not None else None,
                                    gt.ref() if gt is not None else None, nat.ptr(b["f0"]), nat.ptr(b["f1"]),
                                    nat.ptr(b.get("div0")), nat.ptr(b.get("div1")), nat.ptr(b["resid"]),
@@ -926,6 +950,9 @@ class VCycleGraph:
         self.buf = [torch.zeros(shape, dtype=el._tdt, device=el.device) for _ in range(2)]
         self.ext = None if external_force is None else external_force.clone()
         nlf = nonlinear_force
+        # the captured kernels read the coupling's Darcy gradient wall tables and the
+        # coarse tails by address: hold their owners as long as the graphs live
+        self.nlf = nlf
         self.fused = el._fusable(nlf)
         self.div = [torch.zeros((1, el.nx, el.ny), dtype=el._tdt, device=el.device) for _ in range(2)]
 
@@ -939,14 +966,16 @@ class VCycleGraph:
             else:
                 _, norm = el._vcycle(self.buf[src], 0, None, self.ext, nlf, out=self.buf[1 - src])
             return norm
-        # eager warm-up: rule tables, norm workspaces and gradient tables exist before capture
+        # eager warm-up: rule tables, norm workspaces, gradient tables and coarse tails
+        # exist before capture
         cycle(0)
         torch.cuda.synchronize()
+        self.tails = dict(el.__dict__.get("_tails", {}))
         self.graphs, self.norms, self.coarse_bufs = [], [], []
         for src in (0, 1):
             g = torch.cuda.CUDAGraph()
             kept = [] if keep_coarse else None
-            with torch.cuda.graph(g):
+            with graph_capture(g):
                 norm = cycle(src, kept)
             self.graphs.append(g)
             self.norms.append(norm)

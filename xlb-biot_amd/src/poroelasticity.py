@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .base import as_soa, view
+from .base import as_soa, graph_capture, view
 
 
 class BiotNonlinearForce:
@@ -409,7 +409,7 @@ class _PlainInnerGraph:
         self._one(self.a, self.da, self.b, self.db)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
+        with graph_capture(self.graph):
             for _ in range(chunk // 2):
                 self._one(self.a, self.da, self.b, self.db)
                 self._one(self.b, self.db, self.a, self.da)
