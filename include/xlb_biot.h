@@ -491,6 +491,23 @@ int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converg
  * state this is the residual scale |S(0)| of each instance. */
 int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream);
 
+/* Capacity of the static scratch of xb_elast_residual_noalloc, in FP64 partials
+ * (one per wavefront item of the residual sweep: 1 MiB per GPU). */
+#define XB_STATIC_PARTIALS (1 << 17)
+
+/* Allocation-free residual sweep: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid it).  The same sweep as xb_elast_step in XB_MODE_RESIDUAL
+ * (restricted residual into coarse_out, with the same arguments and checks)
+ * followed by xb_norm_finalize, writing (norm, non-finite flag) to norm_out[0..1];
+ * the squared-norm partials live in a static device array of XB_STATIC_PARTIALS
+ * doubles held by each GPU the library runs on, instead of a caller workspace, so
+ * args->norm_partials must be null.  A sweep of more wavefront items than that is
+ * rejected with XB_EINVAL.  Calls no allocation routine.  Calls that overlap on one
+ * GPU (several streams or host threads) race on the shared array; the stream must
+ * belong to the current device. */
+int xb_elast_residual_noalloc(const xb_elast_args *args, double *norm_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

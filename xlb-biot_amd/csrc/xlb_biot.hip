@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <exception>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -3658,6 +3659,71 @@ int xb_elast_smoothn(const xb_elast_args *a, int32_t steps, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (a->dtype == XB_F64) return elast_smoothn_t<double>(a, steps, st);
   return elast_smoothn_t<float>(a, steps, st);
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free residual sweep.  The squared-norm partials of the residual sweep
+// (one per wavefront item) live in this file-scope device array instead of a
+// caller workspace: 8 B x XB_STATIC_PARTIALS = 1 MiB, held by every GPU the
+// library runs on until the process ends.  Every call on one GPU shares that
+// GPU's copy; the sweep writes each partial the finalisation reads, so nothing of
+// it has to start at zero, and calls that overlap on one GPU (different streams,
+// or host threads) race on it.
+static __device__ __align__(256) double scratch[XB_STATIC_PARTIALS];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of the current device's copy of `scratch`, looked up once per
+// device (with that device current).  In host code the array's name is a host-side
+// stand-in, never a device address: kernels and copies use this pointer.
+int scratch_address(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+    return fail(XB_EHIP, "xb_elast_residual_noalloc: no usable current device (%d)", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(XB_EHIP, "xb_elast_residual_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return XB_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int xb_elast_residual_noalloc(const xb_elast_args *a, double *norm_out, void *stream) {
+  if (a == nullptr || norm_out == nullptr) return fail(XB_EINVAL, "xb_elast_residual_noalloc: null argument");
+  if (a->mode != XB_MODE_RESIDUAL) return fail(XB_EINVAL, "xb_elast_residual_noalloc: residual mode only");
+  if (a->norm_partials != nullptr)
+    return fail(XB_EINVAL, "xb_elast_residual_noalloc: norm_partials must be null (the partials live in the "
+                           "library's static scratch)");
+  double *partials = nullptr;
+  if (int r = guarded("xb_elast_residual_noalloc", [&] { return scratch_address(&partials); })) return r;
+  xb_elast_args b = *a;
+  b.norm_partials = partials;
+  if (int r = validate_elast(&b, "xb_elast_residual_noalloc")) return r;
+  SweepGeom g;
+  elast_geom(&b, g);
+  if ((int64_t)g.nitems > (int64_t)XB_STATIC_PARTIALS)
+    return fail(XB_EINVAL, "xb_elast_residual_noalloc: the sweep has %d wavefront items, the static scratch holds "
+                "%d partials (more rows per wavefront, or xb_elast_step with a workspace)", g.nitems,
+                (int)XB_STATIC_PARTIALS);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (int r = b.dtype == XB_F64 ? elast_step_t<double>(&b, st) : elast_step_t<float>(&b, st)) return r;
+  hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(NORM_THREADS), 0, st, partials, g.nitems, norm_out);
+  return check_launch("xb_elast_residual_noalloc");
 }
 
 } // extern "C"

@@ -178,7 +178,11 @@ SIGNATURES = {
     "xb_batch_converge": (c_int32, [ctypes.POINTER(BatchPlan), c_int32, ctypes.POINTER(ConvergeState), c_double,
                                     c_int32, c_void_p]),
     "xb_batch_norms": (c_int32, [ctypes.POINTER(BatchPlan), c_void_p, c_void_p]),
+    "xb_elast_residual_noalloc": (c_int32, [ctypes.POINTER(ElastArgs), c_void_p, c_void_p]),
 }
+
+# capacity of the static partials of xb_elast_residual_noalloc (XB_STATIC_PARTIALS)
+STATIC_PARTIALS = 1 << 17
 
 _lib = None
 
