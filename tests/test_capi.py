@@ -32,6 +32,30 @@ def test_library_exports_every_declared_symbol():
     assert lib.xb_abi_version() == 1
 
 
+def _integration_stub():
+    """Namespace left by the reference-side ctypes binding shown in INTEGRATION.md."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    section = text.split("## 3. The reference-side binding", 1)[1].split("\n## ", 1)[0]
+    code = re.search(r"```python\n(.*?)```", section, flags=re.S).group(1)
+    ns = {}
+    cwd = os.getcwd()
+    os.chdir(ROOT)  # the stub loads the library by its path from the repository root
+    try:
+        exec(compile(code, "INTEGRATION.md", "exec"), ns)
+    finally:
+        os.chdir(cwd)
+    return ns
+
+
+def test_integration_stub_declares_the_whole_argument_struct():
+    # the documented binding must lay out xb_elast_args exactly as the library reads it
+    stub = _integration_stub()["ElastArgs"]
+    assert ctypes.sizeof(stub) == ctypes.sizeof(nat.ElastArgs)
+    got = [(n, getattr(stub, n).offset, getattr(stub, n).size) for n, _ in stub._fields_]
+    want = [(n, getattr(nat.ElastArgs, n).offset, getattr(nat.ElastArgs, n).size) for n, _ in nat.ElastArgs._fields_]
+    assert got == want
+
+
 STRUCTS = (("xb_elast_args", nat.ElastArgs), ("xb_darcy_args", nat.DarcyArgs), ("xb_bc_table", nat.BCTable),
            ("xb_prolong_job", nat.ProlongJob), ("xb_fill_job", nat.FillJob), ("xb_norm_job", nat.NormJob),
            ("xb_batch_plan", nat.BatchPlan), ("xb_converge_state", nat.ConvergeState),

@@ -9,67 +9,85 @@ grid, one ``<prefix>_<timestep:07d>.vtk`` per call) as legacy binary VTK without
 pyvista; ``save_error_history`` stores the error histories of the validation
 drivers as ``.npz`` like ``examples/poroelasticity/periodic.py:243-259``.
 """
+import functools
 import os
+from typing import Dict, NamedTuple
 
 import numpy as np
 
 
-class Parameters:
-    def __init__(self, units=None, parameters=None):
-        self._units = {}
-        self._params = {}
-        self.add_units(**(units or {}))
-        self.add_parameters(**(parameters or {}))
+class _Quantity(NamedTuple):
+    value: float          # in the base units the quantity was given in (SI in the drivers)
+    dims: Dict[str, int]  # base unit name -> integer power
 
+
+class Parameters:
+    """Physical parameters with their dimensions, and their values in a scaled unit
+    system (reference ``src/utils.py:17-76``: same methods, arguments and errors).
+
+    Base units (length, time, pressure, ...) are positive scale factors; a parameter
+    carries a value and the integer powers of the base units it is measured in.  Its
+    nondimensional value divides the value by each base unit raised to its power, one
+    unit at a time in the order the powers were given (the drivers' lattice
+    parameters come out of this, so the order of the divisions is kept)."""
+
+    def __init__(self, units=None, parameters=None):
+        self._scale: Dict[str, float] = {}
+        self._quantities: Dict[str, _Quantity] = {}
+        self.add_units(**dict(units or {}))
+        self.add_parameters(**dict(parameters or {}))
+
+    # ---- base units
     def set_unit(self, name, value):
         if not isinstance(value, float) or value <= 0.0:
             raise TypeError(f"Unit '{name}' must be a positive float.")
-        self._units[name] = value
+        self._scale[name] = value
 
-    def add_units(self, **kwargs):
-        for k, v in kwargs.items():
-            self.set_unit(k, v)
+    def add_units(self, **units):
+        for name in units:
+            self.set_unit(name, units[name])
 
+    def get_unit(self, key):
+        return self._scale[key]
+
+    # ---- parameters
     def set_parameter(self, name, value, unit):
         if not isinstance(unit, dict):
             raise TypeError("unit must be a dict.")
         if not isinstance(value, float):
             raise TypeError("value must be a float.")
-        for k, v in unit.items():
-            if k not in self._units:
-                raise TypeError(f"unit['{k}'] must exist as base unit.")
-            if not isinstance(v, int):
-                raise TypeError(f"unit['{k}'] must be an int.")
-        self._params[name] = (value, unit)
+        # the first offending base unit, in the order the powers are given
+        bad = next(((k, k not in self._scale) for k, p in unit.items()
+                    if k not in self._scale or not isinstance(p, int)), None)
+        if bad is not None:
+            raise TypeError(f"unit['{bad[0]}'] must exist as base unit." if bad[1] else f"unit['{bad[0]}'] must be an int.")
+        self._quantities[name] = _Quantity(value, unit)
 
-    def add_parameters(self, **kwargs):
-        for k, v in kwargs.items():
-            self.set_parameter(k, *v)
-
-    def get_unit(self, key):
-        return self._units[key]
+    def add_parameters(self, **parameters):
+        for name, spec in parameters.items():
+            self.set_parameter(name, *spec)
 
     def get(self, key):
-        return self._params[key][0]
+        return self._quantities[key].value
+
+    def _scaled(self, q):
+        return functools.reduce(lambda acc, d: acc / self._scale[d[0]] ** d[1], q.dims.items(), q.value)
 
     def get_nondimensional(self, key):
-        value, unit = self._params[key]
-        for k, v in unit.items():
-            value /= self._units[k] ** v
-        return value
+        return self._scaled(self._quantities[key])
+
+    # ---- report
+    @staticmethod
+    def _dims_label(dims):
+        return " ".join(k if p == 1 else f"{k}**{p}" for k, p in dims.items()) or "---"
 
     def print(self, nondimensional=True):
-        print(f"{'Parameter':>20} | {'Unit': ^20} | {'Value'}")
-        print('-' * 60)
-        for k, v in self._units.items():
-            print(f"{k:>20} | {'< UNIT >': ^20} | {v}")
-        for k, (v, u) in self._params.items():
-            unit = ""
-            for uk, uv in u.items():
-                if nondimensional:
-                    v /= self._units[uk] ** uv
-                unit += (" " if unit else "") + uk + (f"**{uv}" if uv != 1 else "")
-            print(f"{k:>20} | {unit or '---': ^20} | {v}")
+        rows = [("Parameter", "Unit", "Value"), None]
+        rows += [(k, "< UNIT >", s) for k, s in self._scale.items()]
+        rows += [(k, self._dims_label(q.dims), self._scaled(q) if nondimensional else q.value)
+                 for k, q in self._quantities.items()]
+        for row in rows:
+            print("-" * 60 if row is None else f"{row[0]:>20} | {row[1]: ^20} | {row[2]}")
 
 
 def downsample_field(field, factor, method="bicubic"):
