@@ -41,7 +41,9 @@ class BatchPlan:
                   nat.XB_BATCH_FILL: nat.FillJob, nat.XB_BATCH_NORM: nat.NormJob,
                   nat.XB_BATCH_ELAST_SMOOTHN: nat.ElastArgs}
 
-    def __init__(self, kind, dtype, jobs, inst, device):
+    def __init__(self, kind, dtype, jobs, inst, device, tag=None):
+        # tag: (role, level[, steps]) of the launch in a V-cycle schedule
+        self.tag = tag
         lib = nat.lib()
         n = len(jobs)
         if n == 0:
@@ -283,7 +285,7 @@ class ElasticityBatch:
                 cur[i, level] = (of, od)
             if jobs:
                 kind = nat.XB_BATCH_ELAST_SMOOTHN if m > 1 else nat.XB_BATCH_ELAST_STEP
-                plans.append(BatchPlan(kind, dt, jobs, ids, self.device))
+                plans.append(BatchPlan(kind, dt, jobs, ids, self.device, tag=("smooth", level, m)))
 
         def smooth_phase(level, post):
             m = self._tile_steps(level)
@@ -315,8 +317,8 @@ class ElasticityBatch:
                 fills.append(nat.FillJob(d0.numel(), nat.ptr(d0), 0.0))
                 fill_ids += [i, i]
                 cur[i, level + 1] = (c0, d0)
-            plans.append(BatchPlan(nat.XB_BATCH_ELAST_STEP, dt, jobs, deeper, self.device))
-            plans.append(BatchPlan(nat.XB_BATCH_FILL, dt, fills, fill_ids, self.device))
+            plans.append(BatchPlan(nat.XB_BATCH_ELAST_STEP, dt, jobs, deeper, self.device, tag=("residual", level)))
+            plans.append(BatchPlan(nat.XB_BATCH_FILL, dt, fills, fill_ids, self.device, tag=("fill", level + 1)))
         for level in range(L - 1, -1, -1):
             deeper = [i for i, I in enumerate(self.inst) if I.levels - 1 > level]
             if deeper:
@@ -330,7 +332,7 @@ class ElasticityBatch:
                     x0, x1, y0, y1 = el.interior_box(level)
                     jobs.append(nat.ProlongJob(nx, ny, cnx, cny, x0, x1, y0, y1, el.lamda, el.mu, nat.ptr(f),
                                                nat.ptr(fc), nat.ptr(d) if self.fused else None))
-                plans.append(BatchPlan(nat.XB_BATCH_PROLONG_DIV, dt, jobs, deeper, self.device))
+                plans.append(BatchPlan(nat.XB_BATCH_PROLONG_DIV, dt, jobs, deeper, self.device, tag=("prolong", level)))
             smooth_phase(level, True)
         if len(norm_jobs) != self.n:
             raise RuntimeError("every instance needs a finest-level residual")
