@@ -413,6 +413,9 @@ def c5_kernel_time(torch, bat, reps):
     launches between two HIP events on the stream they are launched on."""
     plan = c5_dominant_plan(bat)
     active = torch.ones(bat.batch.n, dtype=torch.int32, device="cuda")
+    # with every instance active a solve's cycles launch one workgroup per tile (the
+    # work-list form takes over once most of the work has converged)
+    plan.use_list(False)
     plan.launch(active)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -451,6 +451,18 @@ typedef struct xb_batch_plan { /* filled by xb_batch_prepare                    
   void *table;
   int64_t table_bytes;
   int64_t jobs_offset, blocks_offset, insts_offset;
+  /* per job: first block, block count, instance (3 x njobs int32) */
+  int64_t jobinfo_offset;
+  /* work list written by xb_batch_compact: active jobs (njobs), prefix of their block
+   * counts (njobs + 1), count of active jobs (1); 0: the kind has none (NORM) */
+  int64_t work_offset;
+  /* set by the caller: 1 = launch over the work list (a grid of at most as many
+   * workgroups as the GPU holds at once, each taking listed blocks in turn) instead
+   * of one workgroup per table block */
+  int32_t use_list;
+  /* set by the caller: workgroups per resident slot of a list launch (0 or 1: as many
+   * as the GPU holds at once; more: hardware scheduling balances the blocks) */
+  int32_t list_waves;
 } xb_batch_plan;
 
 /* Per-instance convergence state (device arrays of ninst entries, nactive one int).
@@ -474,8 +486,19 @@ int64_t xb_batch_table_bytes(int32_t kind, int32_t dtype, const void *jobs, int3
 int xb_batch_prepare(int32_t kind, int32_t dtype, const void *jobs, const int32_t *inst, int32_t njobs,
                      void *table, int64_t table_bytes, xb_batch_plan *plan, void *stream);
 
-/* One launch of every job of the plan (kinds ELAST_STEP, PROLONG_DIV, FILL). */
+/* One launch of every job of the plan (kinds ELAST_STEP, ELAST_SMOOTHN, PROLONG_DIV,
+ * FILL).  With plan->use_list the launch covers the blocks of the jobs listed by the
+ * last xb_batch_compact of that plan (whatever `active` says now); otherwise every
+ * table block is a workgroup and blocks of inactive instances return at once. */
 int xb_batch_launch(const xb_batch_plan *plan, const int32_t *active, void *stream);
+
+/* Work lists of many plans in one launch: for each plan of `plans` (a DEVICE array of
+ * nplans xb_batch_plan structs, copies of prepared plans), the jobs whose instance is
+ * active (active[inst] != 0; all when active is null), in job order, and the prefix
+ * of their block counts, written into that plan's table.  Capture-safe (one kernel).
+ * No single reference method: the per-instance convergence masks of the batched
+ * loop over solver objects, turned into the work of the next launches. */
+int xb_batch_compact(const xb_batch_plan *plans, int32_t nplans, const int32_t *active, void *stream);
 
 /* Convergence update after one V-cycle of every active instance (plan of kind NORM):
  * r = sqrt(sum of its partials) (same order as xb_norm_finalize); cycles += 1;

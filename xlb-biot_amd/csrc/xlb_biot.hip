@@ -506,27 +506,64 @@ template <typename T, class Coll> struct SweepJob {
   int32_t block0; // first block of this job in the launch
 };
 
+// Work-list launches of the batched kernels.  xb_batch_compact lists, per plan, the
+// jobs of active instances and the prefix of their block counts; a launch of G
+// workgroups (G a multiple of 8, at most what the GPU holds at once) then covers
+// exactly those blocks.  Workgroups are dealt round-robin to the 8 XCDs, so
+// workgroup i serves XCD i % 8: each XCD takes a contiguous run of the listed blocks
+// (neighbouring tiles share halo lines in that XCD's L2) and each of its G / 8
+// workgroups every (G / 8)-th block of the run, in order.  Idle instances cost
+// nothing: no workgroup is dispatched for them.
+struct WorkListDev {
+  const int32_t *ajob = nullptr; // active jobs; null: one workgroup per table block
+  const int32_t *aoff = nullptr; // prefix of their block counts (naj + 1 entries)
+  const int32_t *naj = nullptr;  // number of active jobs
+};
+
+template <class F> __device__ __forceinline__ void for_listed_blocks(const WorkListDev &wl, F &&body) {
+  const int na = *wl.naj;
+  const int total = wl.aoff[na];
+  const int x = (int)(blockIdx.x & 7), k = (int)(blockIdx.x >> 3), g8 = (int)(gridDim.x >> 3);
+  const int lo = (int)(((int64_t)total * x) >> 3), hi = (int)(((int64_t)total * (x + 1)) >> 3);
+  int a = 0;
+  for (int w = lo + k; w < hi; w += g8) { // w is uniform per workgroup
+    while (wl.aoff[a + 1] <= w) ++a;      // monotone in w
+    body(wl.ajob[a], w - wl.aoff[a]);
+  }
+}
+
+// block `local` of one batched sweep job
+template <typename T, class Coll, int MODE, int BC, bool HAS_RES, bool HAS_NL>
+__device__ __forceinline__ void sweep_batch_block(const SweepJob<T, Coll> *__restrict__ jobs, int j, int local) {
+  const SweepJob<T, Coll> J = jobs[j];
+  const int item = local * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
+  if (item >= J.g.nitems) return; // uniform per wavefront
+  if constexpr (BC == 2) {
+    __shared__ BCShared<T, Coll::Q> sh[WAVES_PER_BLOCK];
+    sweep_item<T, Coll, MODE, BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
+                                                   J.coarse, J.partials, J.nl, &sh[threadIdx.x >> 6], item);
+  } else {
+    sweep_item<T, Coll, MODE, BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
+                                                   J.coarse, J.partials, J.nl, nullptr, item);
+  }
+}
+
 template <typename T, class Coll, int MODE, int BC, bool HAS_RES, bool HAS_NL>
 __global__ void __launch_bounds__(BLOCK)
 k_sweep_batch(const SweepJob<T, Coll> *__restrict__ jobs, const int32_t *__restrict__ block_job,
-              const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
+              const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active, const WorkListDev wl) {
+  if (wl.ajob != nullptr) {
+    // the waves of a workgroup share nothing but the rule staging of their own LDS
+    // regions, so consecutive blocks need no barrier between them
+    for_listed_blocks(wl, [&](int j, int local) { sweep_batch_block<T, Coll, MODE, BC, HAS_RES, HAS_NL>(jobs, j, local); });
+    return;
+  }
   const int b = xcd_block_id();
   const int j = block_job[b];
   const int i = block_inst[b];
   if (j < 0) return;                               // uniform per block
   if (active != nullptr && active[i] == 0) return; // uniform per block
-  const SweepJob<T, Coll> J = jobs[j];
-  const int item = (b - J.block0) * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
-  if constexpr (BC == 2) {
-    __shared__ BCShared<T, Coll::Q> sh[WAVES_PER_BLOCK];
-    if (item >= J.g.nitems) return;
-    sweep_item<T, Coll, MODE, BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
-                                                   J.coarse, J.partials, J.nl, &sh[threadIdx.x >> 6], item);
-  } else {
-    if (item >= J.g.nitems) return; // uniform per wavefront
-    sweep_item<T, Coll, MODE, BC, HAS_RES, HAS_NL>(J.g, J.coll, J.fin, J.fout, J.fpc, J.res, J.bc, J.omg, J.gm,
-                                                   J.coarse, J.partials, J.nl, nullptr, item);
-  }
+  sweep_batch_block<T, Coll, MODE, BC, HAS_RES, HAS_NL>(jobs, j, b - jobs[j].block0);
 }
 
 template <typename T, class Coll, int MODE>
@@ -1315,20 +1352,32 @@ template <typename T> struct ProlongJob {
   int32_t inst, block0, nblocks;
 };
 
+template <typename T> __device__ __forceinline__ void prolong_batch_block(const ProlongJob<T> &J, int local);
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_prolong_batch(const ProlongJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
-                const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
+                const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active, const WorkListDev wl) {
+  if (wl.ajob != nullptr) {
+    for_listed_blocks(wl, [&](int j, int local) { prolong_batch_block<T>(jobs[j], local); });
+    return;
+  }
   const int b = blockIdx.x;
   const int j = block_job[b];
   const int i = block_inst[b];
   if (j < 0) return;
   if (active != nullptr && active[i] == 0) return;
-  const ProlongJob<T> J = jobs[j];
+  prolong_batch_block<T>(jobs[j], b - jobs[j].block0);
+}
+
+// block `local` of one batched prolongation job (a grid-stride loop over the job's
+// fine interior with the job's block count as the stride)
+template <typename T> __device__ __forceinline__ void prolong_batch_block(const ProlongJob<T> &Jr, int local) {
+  const ProlongJob<T> J = Jr;
   const int nxi = J.ix1 - J.ix0, nyi = J.iy1 - J.iy0;
   const int64_t n = (int64_t)nxi * nyi;
   const size_t N = (size_t)J.nx * J.ny, Nc = (size_t)J.cnx * J.cny;
-  for (int64_t t = (int64_t)(b - J.block0) * 256 + threadIdx.x; t < n; t += (int64_t)J.nblocks * 256) {
+  for (int64_t t = (int64_t)local * 256 + threadIdx.x; t < n; t += (int64_t)J.nblocks * 256) {
     const int xi = (int)(t / nyi), yi = (int)(t % nyi);
     const size_t cell = (size_t)(J.ix0 + xi) * J.ny + (J.iy0 + yi);
     const size_t cc = (size_t)(J.ix0 + (xi >> 1)) * J.cny + (J.iy0 + (yi >> 1));
@@ -1349,18 +1398,25 @@ template <typename T> struct FillJob {
   int32_t inst, block0, nblocks;
 };
 
+template <typename T> __device__ __forceinline__ void fill_batch_block(const FillJob<T> &Jr, int local) {
+  const FillJob<T> J = Jr;
+  for (int64_t i = (int64_t)local * 256 + threadIdx.x; i < J.n; i += (int64_t)J.nblocks * 256) J.out[i] = J.value;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_fill_batch(const FillJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
-             const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
+             const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active, const WorkListDev wl) {
+  if (wl.ajob != nullptr) {
+    for_listed_blocks(wl, [&](int j, int local) { fill_batch_block<T>(jobs[j], local); });
+    return;
+  }
   const int b = blockIdx.x;
   const int j = block_job[b];
   const int i = block_inst[b];
   if (j < 0) return;
   if (active != nullptr && active[i] == 0) return;
-  const FillJob<T> J = jobs[j];
-  for (int64_t i = (int64_t)(b - J.block0) * 256 + threadIdx.x; i < J.n; i += (int64_t)J.nblocks * 256)
-    J.out[i] = J.value;
+  fill_batch_block<T>(jobs[j], b - jobs[j].block0);
 }
 
 struct NormJob {
@@ -1437,7 +1493,12 @@ inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 struct BlockTable {
   std::vector<int32_t> block_job;  // job of each block (-1: padding)
   std::vector<int32_t> block_inst; // instance of each block's job (read beside block_job)
+  // per job (in job order): first block, block count, instance (the work lists)
+  std::vector<int32_t> job_block0, job_nblocks, job_inst;
   void add(int job, int inst, int nblocks) {
+    job_block0.push_back((int32_t)block_job.size());
+    job_nblocks.push_back(nblocks);
+    job_inst.push_back(inst);
     block_job.insert(block_job.end(), (size_t)nblocks, job);
     block_inst.insert(block_inst.end(), (size_t)nblocks, inst);
   }
@@ -1734,7 +1795,14 @@ template <typename T> struct TileSmoothJob {
 template <typename T, int BC, bool HAS_RES, bool HAS_NL, bool GWALLS>
 __global__ void __launch_bounds__(SMOOTH_THREADS, HAS_RES ? 2 : 1)
 k_tile_smooth_batch(const TileSmoothJob<T> *__restrict__ jobs, const int32_t *__restrict__ block_job,
-                    const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active) {
+                    const int32_t *__restrict__ block_inst, const int32_t *__restrict__ active, const WorkListDev wl) {
+  if (wl.ajob != nullptr) {
+    for_listed_blocks(wl, [&](int j, int local) {
+      tile_smooth_run<T, BC, HAS_RES, HAS_NL, GWALLS>(jobs[j].A, local);
+      __syncthreads(); // the next tile reuses the LDS arrays
+    });
+    return;
+  }
   const int b = xcd_block_id();
   const int j = block_job[b];
   if (j < 0) return;                                         // uniform per block
@@ -1827,13 +1895,51 @@ template <typename T> int build_smoothn_jobs(const xb_elast_args *args, const in
   return XB_OK;
 }
 
+// Work list of a plan launched over it (empty: one workgroup per table block).  The
+// list region holds the active jobs (njobs), the prefix of their block counts
+// (njobs + 1) and their number.
+WorkListDev plan_worklist(const xb_batch_plan *p) {
+  WorkListDev wl;
+  if (p->use_list && p->work_offset > 0) {
+    const int32_t *w = reinterpret_cast<const int32_t *>(static_cast<const char *>(p->table) + p->work_offset);
+    wl.ajob = w;
+    wl.aoff = w + p->njobs;
+    wl.naj = w + 2 * p->njobs + 1;
+  }
+  return wl;
+}
+
+// Workgroups of a batched launch: one per table block, or over a work list as many
+// as the GPU holds at once for this kernel (a multiple of 8, at most the table's
+// block count): each takes listed blocks in turn.
+int batch_grid(const xb_batch_plan *p, const WorkListDev &wl, const void *kernel, int threads) {
+  if (wl.ajob == nullptr) return p->nblocks;
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  const int waves = std::max(1, std::min(p->list_waves, 64));
+  const int g = (int)std::min<int64_t>((int64_t)per_cu * cus * waves, p->nblocks);
+  return std::max(8, (g + 7) / 8 * 8);
+}
+
+template <typename T, int BC, bool RES, bool NL, bool GW>
+void launch_smoothn_batch_k(const xb_batch_plan *p, const TileSmoothJob<T> *jobs, const int32_t *bj,
+                            const int32_t *bi, const int32_t *active, const WorkListDev &wl, hipStream_t st) {
+  const int grid = batch_grid(p, wl, reinterpret_cast<const void *>(&k_tile_smooth_batch<T, BC, RES, NL, GW>),
+                              SMOOTH_THREADS);
+  hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, NL, GW>), dim3(grid), dim3(SMOOTH_THREADS), 0, st, jobs, bj, bi,
+                     active, wl);
+}
+
 template <typename T, int BC, bool RES>
-void launch_smoothn_batch_nl(const TileSmoothJob<T> *jobs, const int32_t *bj, const int32_t *bi,
-                             const int32_t *active, dim3 grid, hipStream_t st, bool nl, bool gw) {
-  const dim3 block(SMOOTH_THREADS);
-  if (!nl) hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, false, false>), grid, block, 0, st, jobs, bj, bi, active);
-  else if (gw) hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, true, true>), grid, block, 0, st, jobs, bj, bi, active);
-  else hipLaunchKernelGGL((k_tile_smooth_batch<T, BC, RES, true, false>), grid, block, 0, st, jobs, bj, bi, active);
+void launch_smoothn_batch_nl(const xb_batch_plan *p, const TileSmoothJob<T> *jobs, const int32_t *bj,
+                             const int32_t *bi, const int32_t *active, const WorkListDev &wl, hipStream_t st, bool nl,
+                             bool gw) {
+  if (!nl) launch_smoothn_batch_k<T, BC, RES, false, false>(p, jobs, bj, bi, active, wl, st);
+  else if (gw) launch_smoothn_batch_k<T, BC, RES, true, true>(p, jobs, bj, bi, active, wl, st);
+  else launch_smoothn_batch_k<T, BC, RES, true, false>(p, jobs, bj, bi, active, wl, st);
 }
 
 template <typename T> int launch_smoothn_batch(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
@@ -1841,15 +1947,15 @@ template <typename T> int launch_smoothn_batch(const xb_batch_plan *p, const int
   const auto *jobs = reinterpret_cast<const TileSmoothJob<T> *>(base + p->jobs_offset);
   const int32_t *bj = reinterpret_cast<const int32_t *>(base + p->blocks_offset);
   const int32_t *bi = reinterpret_cast<const int32_t *>(base + p->insts_offset);
-  const dim3 grid(p->nblocks);
+  const WorkListDev wl = plan_worklist(p);
   const int bcm = (p->flags & 1) ? ((p->flags & 8) ? 2 : 1) : 0;
   const bool res = p->flags & 2, nl = p->flags & 4, gw = p->flags & 16;
-  if (bcm == 2 && res) launch_smoothn_batch_nl<T, 2, true>(jobs, bj, bi, active, grid, st, nl, gw);
-  else if (bcm == 2) launch_smoothn_batch_nl<T, 2, false>(jobs, bj, bi, active, grid, st, nl, gw);
-  else if (bcm == 1 && res) launch_smoothn_batch_nl<T, 1, true>(jobs, bj, bi, active, grid, st, nl, gw);
-  else if (bcm == 1) launch_smoothn_batch_nl<T, 1, false>(jobs, bj, bi, active, grid, st, nl, gw);
-  else if (res) launch_smoothn_batch_nl<T, 0, true>(jobs, bj, bi, active, grid, st, nl, gw);
-  else launch_smoothn_batch_nl<T, 0, false>(jobs, bj, bi, active, grid, st, nl, gw);
+  if (bcm == 2 && res) launch_smoothn_batch_nl<T, 2, true>(p, jobs, bj, bi, active, wl, st, nl, gw);
+  else if (bcm == 2) launch_smoothn_batch_nl<T, 2, false>(p, jobs, bj, bi, active, wl, st, nl, gw);
+  else if (bcm == 1 && res) launch_smoothn_batch_nl<T, 1, true>(p, jobs, bj, bi, active, wl, st, nl, gw);
+  else if (bcm == 1) launch_smoothn_batch_nl<T, 1, false>(p, jobs, bj, bi, active, wl, st, nl, gw);
+  else if (res) launch_smoothn_batch_nl<T, 0, true>(p, jobs, bj, bi, active, wl, st, nl, gw);
+  else launch_smoothn_batch_nl<T, 0, false>(p, jobs, bj, bi, active, wl, st, nl, gw);
   return check_launch("batched smoothing tile kernel");
 }
 
@@ -1922,9 +2028,15 @@ int build_norm_jobs(const xb_norm_job *args, const int32_t *inst, int32_t n, std
 template <typename J>
 int finish_table(const std::vector<J> &jobs, const BlockTable &bt, int32_t kind, int32_t dtype, int32_t mode,
                  int32_t flags, void *table, int64_t bytes, xb_batch_plan *plan, hipStream_t st, int64_t *need) {
+  // layout: jobs | block_job | block_inst | per-job info (3 x njobs) | work list
+  // (2 x njobs + 2: active jobs, prefix of their block counts, their number)
+  const bool listed = !bt.job_block0.empty(); // every kind with blocks (not NORM)
+  const int64_t nj = (int64_t)bt.job_block0.size();
   const int64_t jb = align256((int64_t)(jobs.size() * sizeof(J)));
   const int64_t ib = jb + align256((int64_t)bt.block_job.size() * 4);
-  const int64_t total = ib + (int64_t)bt.block_inst.size() * 4;
+  const int64_t fb = ib + align256((int64_t)bt.block_inst.size() * 4);
+  const int64_t wb = fb + align256(3 * nj * 4);
+  const int64_t total = listed ? wb + (2 * nj + 2) * 4 : ib + (int64_t)bt.block_inst.size() * 4;
   if (need != nullptr) {
     *need = std::max<int64_t>(total, 256);
     return XB_OK;
@@ -1942,7 +2054,24 @@ int finish_table(const std::vector<J> &jobs, const BlockTable &bt, int32_t kind,
                        hipMemcpyHostToDevice, st) != hipSuccess)
       return fail(XB_EHIP, "xb_batch_prepare: block table upload failed");
   }
+  std::vector<int32_t> info, work;
+  if (listed) {
+    info.reserve((size_t)(3 * nj));
+    info.insert(info.end(), bt.job_block0.begin(), bt.job_block0.end());
+    info.insert(info.end(), bt.job_nblocks.begin(), bt.job_nblocks.end());
+    info.insert(info.end(), bt.job_inst.begin(), bt.job_inst.end());
+    work.assign((size_t)(2 * nj + 2), 0); // nothing listed until the first xb_batch_compact
+    if (hipMemcpyAsync(static_cast<char *>(table) + fb, info.data(), info.size() * 4, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(static_cast<char *>(table) + wb, work.data(), work.size() * 4, hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+      return fail(XB_EHIP, "xb_batch_prepare: work list upload failed");
+  }
   if (hipStreamSynchronize(st) != hipSuccess) return fail(XB_EHIP, "xb_batch_prepare: synchronisation failed");
+  plan->jobinfo_offset = listed ? fb : 0;
+  plan->work_offset = listed ? wb : 0;
+  plan->use_list = 0;
+  plan->list_waves = 1;
   plan->kind = kind;
   plan->dtype = dtype;
   plan->mode = mode;
@@ -2001,9 +2130,14 @@ int launch_elast_batch(const xb_batch_plan *p, const int32_t *active, hipStream_
   const J *jobs = reinterpret_cast<const J *>(static_cast<const char *>(p->table) + p->jobs_offset);
   const int32_t *bj = reinterpret_cast<const int32_t *>(static_cast<const char *>(p->table) + p->blocks_offset);
   const int32_t *bi = reinterpret_cast<const int32_t *>(static_cast<const char *>(p->table) + p->insts_offset);
-  const dim3 grid(p->nblocks), block(BLOCK);
-#define XB_SB(BC, RES, NL) \
-  hipLaunchKernelGGL((k_sweep_batch<T, ElastColl<T>, MODE, BC, RES, NL>), grid, block, 0, st, jobs, bj, bi, active)
+  const WorkListDev wl = plan_worklist(p);
+  const dim3 block(BLOCK);
+#define XB_SB(BC, RES, NL)                                                                                          \
+  hipLaunchKernelGGL(                                                                                              \
+      (k_sweep_batch<T, ElastColl<T>, MODE, BC, RES, NL>),                                                         \
+      dim3(batch_grid(p, wl, reinterpret_cast<const void *>(&k_sweep_batch<T, ElastColl<T>, MODE, BC, RES, NL>),  \
+                      BLOCK)),                                                                                     \
+      block, 0, st, jobs, bj, bi, active, wl)
   const int bcm = (p->flags & 1) ? ((p->flags & 8) ? 2 : 1) : 0;
 #define XB_SB_BC(RES, NL)                                                                                           \
   do {                                                                                                             \
@@ -2026,6 +2160,7 @@ template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *
   const char *base = static_cast<const char *>(p->table);
   const int32_t *bj = reinterpret_cast<const int32_t *>(base + p->blocks_offset);
   const int32_t *bi = reinterpret_cast<const int32_t *>(base + p->insts_offset);
+  const WorkListDev wl = plan_worklist(p);
   switch (p->kind) {
   case XB_BATCH_ELAST_STEP:
     if (p->mode == XB_MODE_STEP) return launch_elast_batch<T, XB_MODE_STEP>(p, active, st);
@@ -2034,13 +2169,73 @@ template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *
   case XB_BATCH_ELAST_SMOOTHN:
     return launch_smoothn_batch<T>(p, active, st);
   case XB_BATCH_PROLONG_DIV:
-    hipLaunchKernelGGL(k_prolong_batch<T>, dim3(p->nblocks), dim3(256), 0, st,
-                       reinterpret_cast<const ProlongJob<T> *>(base + p->jobs_offset), bj, bi, active);
+    hipLaunchKernelGGL(k_prolong_batch<T>, dim3(batch_grid(p, wl, reinterpret_cast<const void *>(&k_prolong_batch<T>), 256)),
+                       dim3(256), 0, st, reinterpret_cast<const ProlongJob<T> *>(base + p->jobs_offset), bj, bi, active,
+                       wl);
     return check_launch("batched prolongation kernel");
   default:
-    hipLaunchKernelGGL(k_fill_batch<T>, dim3(p->nblocks), dim3(256), 0, st,
-                       reinterpret_cast<const FillJob<T> *>(base + p->jobs_offset), bj, bi, active);
+    hipLaunchKernelGGL(k_fill_batch<T>, dim3(batch_grid(p, wl, reinterpret_cast<const void *>(&k_fill_batch<T>), 256)),
+                       dim3(256), 0, st, reinterpret_cast<const FillJob<T> *>(base + p->jobs_offset), bj, bi, active, wl);
     return check_launch("batched fill kernel");
+  }
+}
+
+// Work lists of many plans (one workgroup per plan): the jobs whose instance is active,
+// in job order, and the exclusive prefix of their block counts (a block-wide scan of
+// 256 jobs at a time: ballot-free shuffle scans per wavefront, wavefront totals in LDS).
+__global__ void __launch_bounds__(256) k_batch_compact(const xb_batch_plan *__restrict__ plans,
+                                                       const int32_t *__restrict__ active) {
+  const xb_batch_plan P = plans[blockIdx.x];
+  if (P.work_offset <= 0 || P.jobinfo_offset <= 0) return; // uniform per workgroup
+  const int32_t *jnb = reinterpret_cast<const int32_t *>(static_cast<const char *>(P.table) + P.jobinfo_offset) +
+                       P.njobs;
+  const int32_t *jin = jnb + P.njobs;
+  int32_t *ajob = reinterpret_cast<int32_t *>(static_cast<char *>(P.table) + P.work_offset);
+  int32_t *aoff = ajob + P.njobs;
+  int32_t *naj = aoff + P.njobs + 1;
+  __shared__ int32_t s_cnt[4], s_nb[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int carry_cnt = 0, carry_nb = 0;
+  for (int j0 = 0; j0 < P.njobs; j0 += 256) {
+    const int j = j0 + (int)threadIdx.x;
+    const bool on = j < P.njobs && jnb[j] > 0 && (active == nullptr || active[jin[j]] != 0);
+    const int c = on ? 1 : 0, nb = on ? jnb[j] : 0;
+    int ic = c, inb = nb; // inclusive scans over the wavefront
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int tc = __shfl_up(ic, o, 64), tn = __shfl_up(inb, o, 64);
+      if (lane >= o) {
+        ic += tc;
+        inb += tn;
+      }
+    }
+    if (lane == 63) {
+      s_cnt[wv] = ic;
+      s_nb[wv] = inb;
+    }
+    __syncthreads();
+    int wc = 0, wn = 0, tc = 0, tn = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      if (w < wv) {
+        wc += s_cnt[w];
+        wn += s_nb[w];
+      }
+      tc += s_cnt[w];
+      tn += s_nb[w];
+    }
+    if (on) {
+      const int pos = carry_cnt + wc + ic - 1;
+      ajob[pos] = j;
+      aoff[pos] = carry_nb + wn + inb - nb;
+    }
+    carry_cnt += tc;
+    carry_nb += tn;
+    __syncthreads(); // s_cnt / s_nb are rewritten by the next chunk
+  }
+  if (threadIdx.x == 0) {
+    aoff[carry_cnt] = carry_nb;
+    *naj = carry_cnt;
   }
 }
 
@@ -3244,6 +3439,14 @@ int xb_batch_launch(const xb_batch_plan *plan, const int32_t *active, void *stre
   if (plan->nblocks == 0) return XB_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   return plan->dtype == XB_F64 ? batch_launch_t<double>(plan, active, st) : batch_launch_t<float>(plan, active, st);
+}
+
+int xb_batch_compact(const xb_batch_plan *plans, int32_t nplans, const int32_t *active, void *stream) {
+  // `plans` is a device array: its contents are read by the kernel, not here
+  if (plans == nullptr || nplans <= 0 || nplans > 65535)
+    return fail(XB_EINVAL, "xb_batch_compact: needs a device array of 1 to 65535 plans");
+  hipLaunchKernelGGL(k_batch_compact, dim3(nplans), dim3(256), 0, static_cast<hipStream_t>(stream), plans, active);
+  return check_launch("xb_batch_compact");
 }
 
 int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converge_state *state, double tol,

@@ -107,7 +107,9 @@ class NormJob(ctypes.Structure):
 class BatchPlan(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("dtype", c_int32), ("mode", c_int32), ("flags", c_int32),
                 ("njobs", c_int32), ("nblocks", c_int32), ("table", c_void_p), ("table_bytes", c_int64),
-                ("jobs_offset", c_int64), ("blocks_offset", c_int64), ("insts_offset", c_int64)]
+                ("jobs_offset", c_int64), ("blocks_offset", c_int64), ("insts_offset", c_int64),
+                ("jobinfo_offset", c_int64), ("work_offset", c_int64), ("use_list", c_int32),
+                ("list_waves", c_int32)]
 
 
 class ConvergeState(ctypes.Structure):
@@ -178,6 +180,7 @@ SIGNATURES = {
     "xb_batch_converge": (c_int32, [ctypes.POINTER(BatchPlan), c_int32, ctypes.POINTER(ConvergeState), c_double,
                                     c_int32, c_void_p]),
     "xb_batch_norms": (c_int32, [ctypes.POINTER(BatchPlan), c_void_p, c_void_p]),
+    "xb_batch_compact": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     "xb_elast_residual_noalloc": (c_int32, [ctypes.POINTER(ElastArgs), c_void_p, c_void_p]),
 }
 
