@@ -16,7 +16,8 @@ Arrays use the reference layout ``(nx, ny, q)`` (cell-major), in the dtype of th
 precision policy (float64 by default).  Every function cites the reference
 file:line it restates.
 
-Parity pins (what checks each part; see ``tests/test_oracle_*.py``):
+Parity pins (what checks each part; ``tests/test_oracle.py``,
+``tests/test_bc_tables.py``, ``tests/test_orders.py``):
 
 * lattice tables: closed-form constants, moment identities;
 * transfer matrices: ``(m_to_f @ f_to_m)[:5] = I``;
@@ -25,13 +26,18 @@ Parity pins (what checks each part; see ``tests/test_oracle_*.py``):
 * streaming: ``numpy.roll`` brute force, per-direction sums conserved;
 * restrict/prolong: 2x2 sum / injection, ``restrict(prolong(c)) = 4c``;
 * V-cycle: ``levels=1`` reduces to ``_step``; residual equals ``|f - S(f)|``;
-* elasticity: manufactured periodic solution converges at O(nx^-2);
+* elasticity: manufactured periodic solution converges at O(nx^-2); the
+  reference drivers' Dirichlet walls at second order (displacement) and first
+  (stress), traction walls at first order;
 * Darcy: mass conservation, ``rho*w`` fixed point, analytic mode decay, O(nx^-2);
+  the Dirichlet-wall driver at first order;
 * gradient: exact on linear fields / spectral symbol of the D2Q9 stencil;
-* boundary conditions: hand-derived missing-direction tables on small grids,
-  Dirichlet/Neumann convergence orders of the reference drivers;
-* Biot: manufactured periodic solution (``examples/poroelasticity/periodic.py``)
-  converges at O(dx^2).
+* boundary conditions: hand-derived wall tables (missing directions, their
+  partners and the added traction / pressure terms) of the elastic and ADR
+  anti-bounce-back conditions on 4 x 6 grids;
+* Biot: the manufactured periodic solution (``examples/poroelasticity/periodic.py``)
+  converges at O(dx^2), Terzaghi's consolidation (``Terzaghi.py``) at O(dx); a
+  wrong sign planted in the lattice gradient breaks the former.
 
 The JAX out-of-bounds semantics used when fine-level Darcy boundary indices are
 applied to coarse multigrid arrays (clamp on gather, drop on scatter) are taken

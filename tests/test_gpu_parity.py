@@ -334,6 +334,27 @@ def test_run_loop_matches_oracle(levels):
     assert rel(fg, fo) < 1e-12
 
 
+@pytest.mark.parametrize("io", [0, 2])
+def test_poroelasticity_run_matches_oracle(io):
+    # LBMPoroelasticity.run (poroelasticity.py:208-301): t_max + 1 physical steps; with
+    # io_rate 0 the inner iterations replay captured graphs, with io_rate 2 every second
+    # step hands its seven fields to output_data
+    say("poroelasticity run(), io rate", io)
+    g = P.biot_periodic("gpu", nx=32, levels=2, steps=3)
+    o = P.biot_periodic("oracle", nx=32, levels=2, steps=3)
+    seen = {"gpu": [], "oracle": []}
+    for kind, b in (("gpu", g), ("oracle", o)):
+        b.ioRate = io
+        b.output_data = (lambda kind: lambda **kw: seen[kind].append(
+            (kw["timestep"], torch.as_tensor(kw["pressure"]).detach().cpu().double().clone())))(kind)
+    ge, gd = g.run(5)
+    oe, od = o.run(5)
+    assert rel(ge, oe) < 1e-12 and rel(gd, od) < 1e-12
+    assert [t for t, _ in seen["gpu"]] == [t for t, _ in seen["oracle"]] == ([0, 2, 4, 5] if io else [])
+    for (_, a), (_, b) in zip(seen["gpu"], seen["oracle"]):
+        assert rel(a, b, 1e-3) < 1e-12
+
+
 def test_darcy_run_loop_matches_oracle():
     say("darcy run()")
     g = P.darcy_c2("gpu", n=30)
