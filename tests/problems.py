@@ -173,13 +173,16 @@ def dirichlet_elasticity(kind, n=16, levels=1, lam=0.5, mu=0.4, precision="f64/f
                         "print_info_rate": 0})
 
 
-def obstacle_elasticity(kind, n=16, lam=0.5, mu=0.4, precision="f64/f64"):
+def obstacle_elasticity(kind, n=16, lam=0.5, mu=0.4, precision="f64/f64", levels=1, k_smooth=2):
     """The four full-way Dirichlet walls of ``dirichlet_elasticity`` around an interior
-    L-shaped solid obstacle held at zero displacement (halfway bounce-back on solid
-    cells: the fluid cells around it take their missing directions from the grid
-    mask, diagonals included); single grid."""
+    L-shaped solid obstacle held at a prescribed displacement (halfway bounce-back on
+    solid cells: the fluid cells around it take their missing directions from the grid
+    mask, diagonals included).  With ``levels`` > 1 every coarse level carries its own
+    walls and the obstacle's coarse voxels (interior cell i -> i // 2^level), both
+    homogeneous, as ``BCs_MG`` (elasticity.py:396-436)."""
     m = modules(kind)
     EBBF, EBB = m["EBBF"], m["EBB"]
+    block = [(x, y) for x in range(6, 9) for y in range(6, 12)] + [(x, 6) for x in range(9, 12)]
 
     class Obstacle(m["Elasticity"]):
         def get_force(self):
@@ -193,14 +196,24 @@ def obstacle_elasticity(kind, n=16, lam=0.5, mu=0.4, precision="f64/f64"):
             self.BCs.append(EBBF(tuple(bb["right"].T), self.gridInfo, self.precisionPolicy, ew[0, ...]))
             self.BCs.append(EBBF(tuple(bb["bottom"].T), self.gridInfo, self.precisionPolicy, ew[:, -1, ...]))
             self.BCs.append(EBBF(tuple(bb["top"].T), self.gridInfo, self.precisionPolicy, ew[:, 0, ...]))
-            block = [(x, y) for x in range(6, 9) for y in range(6, 12)] + [(x, 6) for x in range(9, 12)]
             # one displacement vector for the whole obstacle (the per-cell form indexes the
             # given rows with the shifted cells' positions, elasticity.py:1604-1608)
             self.BCs.append(EBB(tuple(np.array(block).T), self.gridInfo, self.precisionPolicy,
                                 np.array([2e-4, -1e-4])))
+            for level in range(1, self.levels):
+                gi = self.grid_info(level)
+                bbl = self.bounding_box_indices(level)
+                coarse = sorted({((x - 1) // 2 ** level + 1, (y - 1) // 2 ** level + 1) for x, y in block})
+                self.BCs_MG[level] = [EBBF(tuple(bbl[s].T), gi, self.precisionPolicy)
+                                      for s in ("left", "right", "bottom", "top")]
+                self.BCs_MG[level].append(EBB(tuple(np.array(coarse).T), gi, self.precisionPolicy))
 
-    return Obstacle(**{"lambda": lam, "mu": mu, "nx": n + 2, "ny": n + 2, "nz": 0, "precision": precision,
-                       "lattice": m["D2Q8"](precision), "print_info_rate": 0})
+    kw = {"lambda": lam, "mu": mu, "nx": n + 2, "ny": n + 2, "nz": 0, "precision": precision,
+          "lattice": m["D2Q8"](precision), "print_info_rate": 0}
+    if levels > 1:
+        kw.update({"MG_levels": levels, "MG_n_smoothing": k_smooth,
+                   "MG_BCs": {"top": True, "bottom": True, "left": True, "right": True}})
+    return Obstacle(**kw)
 
 
 def mixed_halfway_elasticity(kind, nx=18, ny=12, precision="f64/f64", lam=1.0, mu=0.7):

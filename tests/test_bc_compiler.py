@@ -49,6 +49,7 @@ def _pairs():
     yield "darcy_eq", lambda k: P.darcy_equilibrium(k, n=8)
     yield "darcy_obstacle", lambda k: P.darcy_obstacle(k)
     yield "elastic_obstacle", lambda k: P.obstacle_elasticity(k)
+    yield "elastic_obstacle_mg", lambda k: P.obstacle_elasticity(k, levels=3)
 
 
 @pytest.mark.parametrize("name,make", list(_pairs()))

@@ -120,15 +120,18 @@ def test_biot_batch_matches_sequential_bitwise(scale):
         say("inner residuals", [p.inner_residuals for p in seq], "cycles", [p.inner_counts for p in seq])
 
 
-WALLS = [("neumann", 16, 2), ("neumann", 32, 3), ("dirichlet", 16, 2), ("dirichlet", 32, 2)]
+WALLS = [("neumann", 16, 2), ("neumann", 32, 3), ("dirichlet", 16, 2), ("dirichlet", 32, 2), ("obstacle", 16, 3),
+         ("obstacle", 32, 3)]
 
 
 def _walled(kind, n, levels):
-    return (P.neumann_elasticity if kind == "neumann" else P.dirichlet_elasticity)("gpu", n=n, levels=levels)
+    make = {"neumann": P.neumann_elasticity, "dirichlet": P.dirichlet_elasticity,
+            "obstacle": P.obstacle_elasticity}[kind]
+    return make("gpu", n=n, levels=levels)
 
 
 def test_elasticity_batch_with_walls_matches_sequential_bitwise():
-    # traction / Dirichlet walls and multigrid wall layers: rule tables read from
+    # traction / Dirichlet walls, multigrid wall layers and coarse obstacles: rule tables read from
     # global memory (per-cell traction constants) and staged in LDS (coarse levels)
     from src.batch import ElasticityBatch
     say("elasticity batch with walls", WALLS)

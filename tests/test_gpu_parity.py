@@ -103,13 +103,16 @@ def test_elasticity_walls_steps(name, precision):
     assert rel(fg, fo) < TOL[precision]
 
 
-@pytest.mark.parametrize("name,levels", [("periodic", 3), ("neumann", 2), ("neumann", 3), ("dirichlet", 3)])
+@pytest.mark.parametrize("name,levels", [("periodic", 3), ("neumann", 2), ("neumann", 3), ("dirichlet", 3),
+                                         ("obstacle", 2), ("obstacle", 3)])
 def test_elasticity_vcycle(name, levels):
+    # "obstacle": an interior L-shaped obstacle on every level (BCs_MG with coarse voxels)
     say("elasticity V-cycle", name, levels)
     if name == "periodic":
         g, o = P.periodic_elasticity("gpu", nx=32, levels=levels), P.periodic_elasticity("oracle", nx=32, levels=levels)
     else:
-        make = {"neumann": P.neumann_elasticity, "dirichlet": P.dirichlet_elasticity}[name]
+        make = {"neumann": P.neumann_elasticity, "dirichlet": P.dirichlet_elasticity,
+                "obstacle": P.obstacle_elasticity}[name]
         g, o = make("gpu", n=16, levels=levels), make("oracle", n=16, levels=levels)
     fg, fo = g.assign_fields_sharded(), o.assign_fields_sharded()
     for t in range(4):
