@@ -173,57 +173,49 @@ def bench_large_elasticity(torch, n, reps, warmup=1, precision="f64/f64", chunk=
                 tile_threads=threads)
 
 
-def residual_solve(torch, d, tol, max_steps, check_every=1000, chunk=64):
-    """Steps until ||f_{k+1}-f_k|| / ||f_1-f_0|| < tol; returns (steps, seconds)."""
-    f = d.assign_fields_sharded()
+def c2_time_to_solution(torch, n=512, tol=1e-8, check_every=1000, checks=10, max_steps=4000000):
+    """C2 (FP64) from rest to ``|f_{k+1} - f_k| < tol |f_1 - f_0|``, the test every
+    ``check_every`` steps on the device (``run_steps_to``: captured blocks, one host
+    read per ``checks`` tests); the launch shape is the tuned one."""
+    d = darcy_workload(n, "f64/f64")
+    d.autotune()
+    f0 = d.assign_fields_sharded()
+    # buffers, tuned launches and the captured blocks are built before the clock starts
+    d.run_steps_to(f0, tol, check_every, max_steps=check_every * checks + 1, checks=checks)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    f1 = d.run_steps(f, 1)
-    r0 = torch.linalg.norm(f1 - f).item()
-    f = f1
-    k = 1
-    while k < max_steps:
-        f = d.run_steps(f, check_every - 1, chunk)
-        fn = d.run_steps(f, 1)
-        r = torch.linalg.norm(fn - f).item()
-        f = fn
-        k += check_every
-        if r / r0 < tol:
-            break
+    f, k, ratio = d.run_steps_to(f0, tol, check_every, max_steps=max_steps, checks=checks)
     torch.cuda.synchronize()
-    return k, time.perf_counter() - t0, r / r0
+    sec = time.perf_counter() - t0
+    cells = d.nx * d.ny
+    rec = {"grid": [d.nx, d.ny], "tol": tol, "steps": k, "seconds": round(sec, 4), "rel_residual": ratio,
+           "check_every": check_every, "host_reads_every_steps": check_every * checks}
+    if k > 0:
+        rec["mlups"] = round(cells * k / sec / 1e6, 1)
+    return rec
 
 
-def bench_c1(torch, tol=1e-10, check_every=10, max_steps=20000):
+def bench_c1(torch, tol=1e-10, check_every=10, checks=10, max_steps=20000):
     """C1: periodic 64^2 elasticity, FP64, plain steps until the relative displacement
-    residual ||f_{k+1}-f_k|| / ||f_1-f_0|| < tol; error to the closed form."""
+    residual ||f_{k+1}-f_k|| / ||f_1-f_0|| < tol (tested every ``check_every`` steps on
+    the device, ``run_steps_to``); error to the closed form."""
     import numpy as np
     import problems as P
     el = P.periodic_elasticity("gpu", nx=64)
     f0 = el.assign_fields_sharded()
-    chunk = check_every - 1 if check_every % 2 else check_every - 2
-    # the step runners (buffers and captured graphs) are built before the clock starts
-    el.run_steps(f0, 1)
-    el.run_steps(f0, check_every - 1, chunk=chunk)
+    # the runner (buffers, captured blocks) is built before the clock starts
+    el.run_steps_to(f0, tol, check_every, max_steps=check_every * checks + 1, checks=checks)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    f1 = el.run_steps(f0, 1)
-    r0 = torch.linalg.norm(f1 - f0).item()
-    f, k, r = f1, 1, r0
-    while k < max_steps:
-        f = el.run_steps(f, check_every - 1, chunk=chunk)
-        fn = el.run_steps(f, 1)
-        r = torch.linalg.norm(fn - f).item()
-        f, k = fn, k + check_every
-        if r / r0 < tol:
-            break
+    f, k, ratio = el.run_steps_to(f0, tol, check_every, max_steps=max_steps, checks=checks)
     torch.cuda.synchronize()
     el_s = time.perf_counter() - t0
     eta, _ = el.update_macroscopic(f)
     ue = P.manufactured_c1(64) / (8 * np.pi ** 2)
     err = float(np.linalg.norm(eta.cpu().numpy() - ue) / np.linalg.norm(ue))
-    return {"steps": k, "seconds": round(el_s, 4), "mlups": round(64 * 64 * k / el_s / 1e6, 1),
-            "rel_residual": r / r0, "rel_error_closed_form": round(err, 6)}
+    return {"steps": k, "seconds": round(el_s, 5), "ms": round(el_s * 1e3, 3),
+            "mlups": round(64 * 64 * k / el_s / 1e6, 1), "rel_residual": ratio, "rel_error_closed_form": round(err, 6),
+            "check_every": check_every, "host_reads_every_steps": check_every * checks}
 
 
 def bench_c3(torch, n=256, levels=4, steps=20, tol=1e-8, lam=1.0, mu=1.0, K=1.0, max_cycles=300):
@@ -511,7 +503,7 @@ def bench_c5(torch, steps=10, warmup=2, n_inst=64, tol=1e-8, seed=3, max_cycles=
                 "per_instance": [[s_["n"], L, [int(c) for c in cs]] for s_, L, cs in zip(specs, levels, timed_counts)]}}
 
 
-def c5_cpu_baseline(torch, r, seconds_cap=30.0):
+def c5_cpu_baseline(torch, r, seconds_cap=30.0, threads=None):
     """The CPU restatement timed on a bounded sample of the timed workload.  For the
     first instance of every (size, depth) pair of the set, the oracle runs the first
     timed physical step from the GPU's state twice, with 1 and with 3 V-cycles; the
@@ -523,7 +515,7 @@ def c5_cpu_baseline(torch, r, seconds_cap=30.0):
     b = r["build"]
     specs, levels, fields = b["specs"], b["levels"], b["fields"]
     warmup = r["extra"]["warmup_physical_steps"]
-    threads = os.cpu_count() or 1
+    threads = threads or os.cpu_count() or 1
     torch.set_num_threads(threads)
     cost = {}
     spent = 0.0
@@ -730,11 +722,12 @@ def profiled_traffic(kernel, blocks, cells):
     return None if rec is None else rec["read_bytes"] + rec["write_bytes"]
 
 
-def c2_cpu_record(n, precision, seconds):
-    """The CPU restatement's C2 rate (Darcy steps on the CPU, all host cores)."""
+def c2_cpu_record(n, precision, seconds, threads=None):
+    """The CPU restatement's C2 rate (Darcy steps on the CPU; all host cores unless
+    ``threads`` is given)."""
     import torch
     import problems as P
-    threads = os.cpu_count() or 1
+    threads = threads or os.cpu_count() or 1
     torch.set_num_threads(threads)
     o = P.darcy_c2("oracle", n=n, precision=precision, heterogeneous=True, seed=0, variance=1.0, corr=16.0)
     f = o.assign_fields_sharded()
@@ -881,14 +874,15 @@ def main():
         from src import autotune
         log("C2 plain Darcy 512^2, FP64 and FP32")
         extra("c2_darcy_512", lambda: c2_record(torch, args.n))
+        extra("c2_darcy_512_to_1e-8_fp64", lambda: c2_time_to_solution(torch, args.n))
         log("C1 64^2 to 1e-10")
         extra("c1_elasticity_64_fp64", lambda: bench_c1(torch))
         log("C3 Biot 256^2 MG (4 levels, lambda = mu = 1), inner 1e-8")
-        extra("c3_biot_256_mg4_fp64", lambda: bench_c3(torch, steps=5, max_cycles=60))
+        extra("c3_biot_256_mg4_fp64", lambda: bench_c3(torch, steps=200, max_cycles=60))
         lam, mu = 0.11 * 0.8 / (1 - 0.8 ** 2), 0.5 * 0.11 / 1.8
         log("Biot 256^2 MG (8 levels, material of the reference periodic driver), inner 1e-8")
         extra("c3_biot_256_mg8_ref_material_fp64",
-              lambda: bench_c3(torch, levels=8, steps=10, lam=lam, mu=mu, K=0.1))
+              lambda: bench_c3(torch, levels=8, steps=200, lam=lam, mu=mu, K=0.1))
         log("adjoint: consolidation geometry 1024, FP32, 1000 pseudo-steps, checkpoints every 50")
         extra("adjoint_1024_fp32", lambda: bench_adjoint(torch))
         log("C4: gradient through the coupled multigrid consolidation, 1024 x 1026, FP32, 50 steps")
@@ -924,6 +918,10 @@ def main():
         if not args.no_extras:
             log(f"cpu: the oracle on C2 ({args.cpu_seconds:.0f} s)")
             extra("c2_cpu_oracle", lambda: c2_cpu_record(args.n, "f64/f64", args.cpu_seconds))
+            # one thread: the CPU restatement without the host's parallelism
+            extra("c2_cpu_oracle_1_thread", lambda: c2_cpu_record(args.n, "f64/f64", args.cpu_seconds / 2, threads=1))
+            extra("c5_cpu_oracle_1_thread", lambda: c5_cpu_baseline(torch, r, seconds_cap=40.0, threads=1))
+            torch.set_num_threads(os.cpu_count() or 1)
     line["extra"] = extras
     print(json.dumps(line), flush=True)
 

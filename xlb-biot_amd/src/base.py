@@ -12,6 +12,7 @@ solver uses (construction :44-146, boundary data :351-372,
 :451-485, precisions :487-510, ``assign_fields_sharded`` :528-559).
 """
 import contextlib
+import ctypes
 import gc
 import time
 
@@ -196,6 +197,110 @@ class StepRunner:
             self.a, self.b = self.b, self.a
             self.a.copy_(self.b)
         return self.a.clone()
+
+
+class ConvergedStepRunner:
+    """Step loop to a relative step-to-step residual, tested on the device.
+
+    From ``f_0`` it takes one step, records ``r_0 = |f_1 - f_0|`` and then, in blocks of
+    ``check_every`` steps, ``r_k = |f_k - f_{k-1}|`` at ``k = 1 + j check_every``; the
+    first ``k`` with ``r_k < tol r_0`` (or a non-finite ``r_k``) latches ``k``, ``r_k / r_0``
+    and ``f_k`` into device buffers.  ``checks`` blocks are one captured graph; the host
+    replays it and reads the latch once per replay, so a solve costs one
+    synchronisation per ``checks * check_every`` steps and runs at most that many
+    steps past the latched one (their results are discarded).  The norms are the
+    library's FP64 partial sums (``xb_sqdiff_partials`` + ``xb_norm_finalize``).
+
+    ``launch(src, dst)`` is one step, ``multi(src, dst)`` (optional) ``m`` steps."""
+
+    def __init__(self, launch, like, check_every=10, checks=10, multi=None, m=1):
+        if check_every < 2 or checks < 1:
+            raise ValueError("check_every must be >= 2 and checks >= 1")
+        no_capture("ConvergedStepRunner")
+        if multi is None:
+            m = 1
+        self.launch, self.multi, self.m = launch, multi, m
+        self.ce, self.checks = int(check_every), int(checks)
+        dev = like.device
+        self.a = torch.empty_like(like)
+        self.b = torch.empty_like(like)
+        self.out = torch.empty_like(like)
+        self.n = like.numel()
+        self.dtype_code = nat.dtype_code(like.dtype)
+        self.partials = torch.zeros(2048, dtype=torch.float64, device=dev)
+        self.count = nat.c_int32(0)
+        self.rv = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.r0 = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.tol = torch.zeros((), dtype=torch.float64, device=dev)
+        self.k = torch.zeros((), dtype=torch.int64, device=dev)
+        self.done = torch.full((), -1, dtype=torch.int64, device=dev)
+        self.rel = torch.zeros((), dtype=torch.float64, device=dev)
+        self.a.copy_(like)
+        self._block()  # warm-up outside the capture (buffers, kernels)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with graph_capture(self.graph):
+            for _ in range(self.checks):
+                self._block()
+
+    def _norm(self, x, y, out):
+        lib = nat.lib()
+        nat.check(lib.xb_sqdiff_partials(self.n, self.dtype_code, nat.ptr(x), nat.ptr(y), nat.ptr(self.partials),
+                                         2048, ctypes.byref(self.count), nat.stream()), "xb_sqdiff_partials")
+        nat.check(lib.xb_norm_finalize(nat.ptr(self.partials), int(self.count.value), nat.ptr(out), nat.stream()),
+                  "xb_norm_finalize")
+
+    def _advance(self, n):
+        """``n`` steps from buffer a; returns the buffer holding the result."""
+        src, dst = self.a, self.b
+        done = 0
+        if self.multi is not None:
+            while n - done >= self.m:
+                self.multi(src, dst)
+                src, dst = dst, src
+                done += self.m
+        while done < n:
+            self.launch(src, dst)
+            src, dst = dst, src
+            done += 1
+        return src
+
+    def _block(self):
+        """``check_every`` steps from buffer a back into buffer a, the last one tested."""
+        x = self._advance(self.ce - 1)
+        y = self.b if x is self.a else self.a
+        self.launch(x, y)
+        self._norm(x, y, self.rv)
+        self.k.add_(self.ce)
+        hit = ((self.rv[0] / self.r0[0] < self.tol) | (self.rv[1] != 0)) & (self.done < 0)
+        self.done.copy_(torch.where(hit, self.k, self.done))
+        self.rel.copy_(torch.where(hit, self.rv[0] / self.r0[0], self.rel))
+        self.out.copy_(torch.where(hit, y, self.out))
+        if y is not self.a:
+            nat.check(nat.lib().xb_lincomb(self.n, self.dtype_code, 1.0, nat.ptr(y), 0.0, nat.ptr(y), nat.ptr(self.a),
+                                           nat.stream()), "xb_lincomb")
+
+    def run(self, f, tol, max_steps):
+        """``(f_k, k, r_k / r_0)``; ``k`` is -1 when ``max_steps`` passed first."""
+        self.a.copy_(f)
+        self.launch(self.a, self.b)
+        self._norm(self.a, self.b, self.r0)
+        self.a.copy_(self.b)
+        self.tol.fill_(float(tol))
+        self.k.fill_(1)
+        self.done.fill_(-1)
+        self.out.copy_(self.a)
+        self.rel.fill_(1.0)
+        steps = 1
+        while steps < max_steps:
+            self.graph.replay()
+            steps += self.checks * self.ce
+            if int(self.done.item()) >= 0:
+                break
+        k = int(self.done.item())
+        if k < 0:
+            return self.a.clone(), -1, float(self.rv[0].item() / self.r0[0].item())
+        return self.out.clone(), k, float(self.rel.item())
 
 
 class NormWorkspace:

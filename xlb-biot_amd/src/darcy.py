@@ -16,7 +16,7 @@ import torch
 
 from . import _native as nat
 from . import autotune
-from .base import LBMBase, StepRunner, as_soa, view
+from .base import ConvergedStepRunner, LBMBase, StepRunner, as_soa, view
 from .boundary_conditions import BoundaryCondition, BounceBackHalfway, RuleTable, build_rule_table, compile_rules
 
 
@@ -226,6 +226,22 @@ class AdvectionDiffusionReactionBGK(LBMBase):
             runner = self._runners[key] = StepRunner(lambda s, d: me._step_soa(s, None, False, out=d), fs, chunk,
                                                      multi, m)
         return view(runner.run(fs, n))
+
+    def run_steps_to(self, f, tol, check_every=10, max_steps=1000000, checks=10):
+        """Steps from ``f`` until ``|f_k - f_{k-1}| < tol |f_1 - f_0|``, tested every
+        ``check_every`` steps on the device (``k = 1 + j check_every``), the host
+        reading the device-side latch once per ``checks`` tests.  Returns ``(f_k, k,
+        ratio)``, ``k = -1`` if ``max_steps`` came first.  No source, no I/O."""
+        fs = self._soa(f, 9)
+        choice, m, multi = self._multi()
+        key = ("steps_to", int(check_every), int(checks), choice)
+        runner = self._runners.get(key)
+        if runner is None:
+            me = weakref.proxy(self)
+            runner = self._runners[key] = ConvergedStepRunner(lambda s, d: me._step_soa(s, None, False, out=d), fs,
+                                                              check_every, checks, multi, m)
+        out, k, ratio = runner.run(fs, tol, max_steps)
+        return view(out), k, ratio
 
     def collision(self, f, external_source=None):
         fs = self._soa(f, 9)

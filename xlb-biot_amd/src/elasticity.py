@@ -16,7 +16,7 @@ import torch
 
 from . import _native as nat
 from . import autotune
-from .base import (NormWorkspace, StepRunner, PrecisionPolicy, as_soa, default_device, graph_capture, no_capture,
+from .base import (ConvergedStepRunner, NormWorkspace, StepRunner, PrecisionPolicy, as_soa, default_device, graph_capture, no_capture,
                    torch_dtype, view)
 from .boundary_conditions import BoundaryCondition, build_rule_table, configure_bcs, create_grid_mask
 
@@ -493,6 +493,25 @@ class LBMElasticity(object):
             if external_force is None:
                 self._runners[key] = runner
         return view(runner.run(fs, n))
+
+    def run_steps_to(self, f, tol, check_every=10, max_steps=1000000, checks=10):
+        """Single-grid steps from ``f`` until ``|f_k - f_{k-1}| < tol |f_1 - f_0|``, tested
+        every ``check_every`` steps on the device (``k = 1 + j check_every``; the test
+        and the state at the first such ``k`` are latched on the device and the host
+        reads the latch once per ``checks`` tests).  Returns ``(f_k, k, ratio)``, with
+        ``k = -1`` if ``max_steps`` came first.  No external force, no I/O."""
+        if self.levels != 1:
+            raise ValueError("run_steps_to is the single-grid loop; multigrid uses step() / run()")
+        fs = self._soa(f, 8)
+        choice, m, multi = self._multi(None)
+        key = ("steps_to", int(check_every), int(checks), choice)
+        runner = self._runners.get(key)
+        if runner is None:
+            me = weakref.proxy(self)
+            runner = ConvergedStepRunner(lambda s, d: me._step_into(s, d, None), fs, check_every, checks, multi, m)
+            self._runners[key] = runner
+        out, k, ratio = runner.run(fs, tol, max_steps)
+        return view(out), k, ratio
 
     def _smooth(self, f, level, resid, ext, out=None):
         out = torch.empty_like(f) if out is None else out
