@@ -35,6 +35,13 @@ class PrecisionPolicy:
         return None if x is None else x.to(self.output_dtype)
 
 
+def gathered(x):
+    """A field as the run loops hand it to ``output_data``: on the host with the leading
+    process axis of ``process_allgather`` (elasticity.py:1378-1401, darcy.py:200-229,
+    poroelasticity.py:273-277; one process)."""
+    return x.detach().cpu().numpy()[None]
+
+
 def stream(f, lattice):
     """``streaming_p`` (elasticity.py:768-803): ``f_i(x) <- f_i(x - c_i)`` periodic roll.
 

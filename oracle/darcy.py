@@ -11,7 +11,7 @@ import numpy as np
 import torch
 
 from . import bc as obc
-from .bc import PrecisionPolicy, stream
+from .bc import PrecisionPolicy, gathered, stream
 
 
 class OracleDarcy:
@@ -167,14 +167,14 @@ class OracleDarcy:
     def run(self, t_max):
         f = self.assign_fields_sharded()
         if self.ioRate > 0:
-            self.output_data(timestep=0, rho=self.update_macroscopic(f), rho_prev=None,
+            self.output_data(timestep=0, rho=gathered(self.update_macroscopic(f)), rho_prev=None,
                              f_poststreaming=f, f_postcollision=None)
         for timestep in range(1, t_max + 1):
             io_flag = self.ioRate > 0 and (timestep % self.ioRate == 0 or timestep == t_max)
             if io_flag:
-                rho_prev = self.update_macroscopic(f)
+                rho_prev = gathered(self.update_macroscopic(f))
             f, fstar = self.step(f, timestep, return_fpost=self.returnFpost)
             if io_flag:
-                self.output_data(timestep=timestep, rho=self.update_macroscopic(f), rho_prev=rho_prev,
+                self.output_data(timestep=timestep, rho=gathered(self.update_macroscopic(f)), rho_prev=rho_prev,
                                  f_poststreaming=f, f_postcollision=fstar)
         return f

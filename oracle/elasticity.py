@@ -9,7 +9,7 @@ import numpy as np
 import torch
 
 from . import bc as obc
-from .bc import PrecisionPolicy, stream
+from .bc import PrecisionPolicy, gathered, stream
 
 
 class OracleElasticity:
@@ -293,10 +293,10 @@ class OracleElasticity:
         for timestep in range(0, t_max + 1):
             io_flag = self.ioRate > 0 and (timestep % self.ioRate == 0 or timestep == t_max)
             if io_flag:
-                eta_prev, sigma_prev = self.update_macroscopic(f)
+                eta_prev, sigma_prev = (gathered(x) for x in self.update_macroscopic(f))
             f, fstar = self.step(f, timestep, return_fpost=self.returnFpost)
             if io_flag:
-                eta, sigma = self.update_macroscopic(f)
+                eta, sigma = (gathered(x) for x in self.update_macroscopic(f))
                 self.output_data(timestep=timestep, eta=eta, eta_prev=eta_prev, sigma=sigma,
                                  sigma_prev=sigma_prev, f_poststreaming=f, f_postcollision=fstar)
         return f

@@ -4,6 +4,7 @@ Restates ``src/poroelasticity.py:44-339`` (construction, ``run`` initialisation,
 the coupled ``step``) on torch-CPU arrays.
 """
 import torch
+from .bc import gathered
 
 
 class OraclePoroelasticity:
@@ -103,7 +104,8 @@ class OraclePoroelasticity:
         for timestep in range(0, t_max + 1):
             f_el, f_d, eta, sigma, dt_div_eta, pressure, pressure_grad = self.step(f_el, f_d, dt_div_eta, timestep)
             if self.ioRate > 0 and (timestep % self.ioRate == 0 or timestep == t_max):
-                self.output_data(timestep=timestep, eta=eta, sigma=sigma, dt_div_eta=dt_div_eta,
-                                 pressure=pressure, pressure_grad=pressure_grad,
+                self.output_data(timestep=timestep, eta=gathered(eta), sigma=gathered(sigma),
+                                 dt_div_eta=gathered(dt_div_eta), pressure=gathered(pressure),
+                                 pressure_grad=gathered(pressure_grad),
                                  f_poststreaming_elasticity=f_el, f_poststreaming_darcy=f_d)
         return f_el, f_d

@@ -72,7 +72,7 @@ def darcy_dirichlet(kind, n, precision="f64/f64"):
             self.BCs.append(ABB(tuple(walls.T), self.gridInfo, self.precisionPolicy))
 
         def output_data(self, **kw):
-            p = _host(kw["rho"])
+            p = np.array(kw["rho"])[0]
             t = kw["timestep"]
             Kt = (1.0 / self.omega - 0.5) / 3 * t / self.nx ** 2
             pe = P.sum_function(self.nx, self.ny, kls,
@@ -137,9 +137,9 @@ def biot_periodic(kind, n, precision="f64/f64", coupling="centered", ne=None):
         t = kw["timestep"]
         Kd = (1.0 / P.mean_rate(b.darcy) - 0.5) / 3
         tf = 1 - np.exp(-8 * np.pi ** 2 * Kd * t / n ** 2)
-        eta = _host(kw["eta"])
-        sigma = _host(kw["sigma"]) / b.epsilon
-        p = _host(kw["pressure"])
+        eta = np.array(kw["eta"])[0]
+        sigma = np.array(kw["sigma"])[0] / b.epsilon
+        p = np.array(kw["pressure"])[0]
         eta_ex = np.concatenate((S([[], [2, 2], [], []], lambda k, l: 4.5 * tf),
                                  S([[], [], [2, 2], []], lambda k, l: 3.5 * tf)), -1)
         sig_ex = np.concatenate((S([[], [], [], [2, 2]], lambda k, l: -np.pi * tf * (16 * lam + 18 * mu)),
@@ -159,12 +159,12 @@ def biot_periodic(kind, n, precision="f64/f64", coupling="centered", ne=None):
 # ---------------------------------------------------------------------------
 # Terzaghi's consolidation
 
-def _xcorner_fix(Base, remove, mask=True):
+def _xcorner_fix(Base, remove):
     """``get_missing_indices`` of the Terzaghi driver's BC subclasses
     (Terzaghi.py:22-112): every direction listed, paired with its opposite; the
     directions of ``remove[(right_wall, top)]`` cleared from the boundary mask in
-    place at that corner cell of a left / right wall; with ``mask`` the directions
-    outside the mask then zeroed (ADR_AntiBB keeps all of them)."""
+    place at that corner cell of a left / right wall; the directions outside the
+    mask then zeroed (ADR_BB and ADR_AntiBB share the ADR table)."""
 
     class Fixed(Base):
         def get_missing_indices(self, boundaryMask):
@@ -172,8 +172,6 @@ def _xcorner_fix(Base, remove, mask=True):
             q = self.lattice.q
             imissing = np.vstack([np.arange(q, dtype="uint8")] * nbd)
             iknown = np.vstack([np.asarray(self.lattice.opp_indices)] * nbd)
-            if not mask:
-                return imissing, iknown
             for i in range(nbd):
                 x, y = int(self.indices[0][i]), int(self.indices[1][i])
                 for (right, top), dirs in remove.items():
@@ -188,8 +186,8 @@ def _xcorner_fix(Base, remove, mask=True):
     return Fixed
 
 
-# Terzaghi.py:23-50 (ADR_BB) and 82-109 (ElasticityAntiBB): directions removed at
-# the lower/upper corners of the left (False) / right (True) wall
+# Terzaghi.py:23-50 (ADR_BB), 53-80 (ADR_AntiBB) and 84-111 (ElasticityAntiBB):
+# directions removed at the lower/upper corners of the left (False) / right (True) wall
 TERZAGHI_ADR = {(False, False): (1, 4), (False, True): (2, 8), (True, False): (1, 7), (True, True): (2, 5)}
 TERZAGHI_EL = {(False, False): (1, 5), (False, True): (3, 6), (True, False): (1, 4), (True, True): (3, 7)}
 
@@ -235,7 +233,7 @@ def terzaghi(kind, n, precision="f64/f64", n_exact=100, ne=None, multigrid=False
         ny = n
         ne = ne or 1
     ADR_BB = _xcorner_fix(m["ADR_BB"], TERZAGHI_ADR)
-    ADR_AntiBB = _xcorner_fix(m["ADR_ABB"], {}, mask=False)
+    ADR_AntiBB = _xcorner_fix(m["ADR_ABB"], TERZAGHI_ADR)
     ElAntiBB = _xcorner_fix(m["EABB"], TERZAGHI_EL)
     EBBF = m["EBBF"]
     load = -s["force"]
@@ -272,8 +270,8 @@ def terzaghi(kind, n, precision="f64/f64", n_exact=100, ne=None, multigrid=False
 
         def output_data(self, **kw):
             t = kw["timestep"]
-            eta = _host(kw["eta"])
-            p = _host(kw["pressure"])
+            eta = np.array(kw["eta"])[0]
+            p = np.array(kw["pressure"])[0]
             nx, ny = p.shape[0], p.shape[1]
             if t > 0:
                 tc = -0.25 * np.pi ** 2 * t * s["dt"]

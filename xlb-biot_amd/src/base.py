@@ -96,6 +96,19 @@ def view(soa):
     return soa.permute(1, 2, 0)
 
 
+def io_field(x, factor=1):
+    """A macroscopic field as the run loops hand it to ``output_data``: down-sampled by
+    ``factor`` and gathered to host memory with a leading process axis, as
+    ``process_allgather`` returns it (one process: ``(1, nx, ny, c)``; elasticity.py:
+    1378-1401, darcy.py:200-229, poroelasticity.py:273-277); drivers read
+    ``np.array(kwargs["eta"])[0]``."""
+    from .utils import downsample_field
+    x = downsample_field(x, factor)
+    if isinstance(x, torch.Tensor):
+        x = x.detach().cpu().numpy()
+    return np.asarray(x)[None]
+
+
 def capturing():
     """True while the current stream records a HIP graph."""
     return torch.cuda.is_available() and torch.cuda.is_current_stream_capturing()

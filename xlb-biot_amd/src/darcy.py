@@ -16,7 +16,7 @@ import torch
 
 from . import _native as nat
 from . import autotune
-from .base import ConvergedStepRunner, LBMBase, StepRunner, as_soa, view
+from .base import ConvergedStepRunner, LBMBase, StepRunner, as_soa, io_field, view
 from .boundary_conditions import BoundaryCondition, BounceBackHalfway, RuleTable, build_rule_table, compile_rules
 
 
@@ -281,7 +281,7 @@ class AdvectionDiffusionReactionBGK(LBMBase):
     def run(self, t_max):
         f = self.assign_fields_sharded()
         if self.ioRate > 0:
-            rho = self.update_macroscopic(f).cpu()
+            rho = io_field(self.update_macroscopic(f), self.downsamplingFactor)
             self.handle_io_timestep(0, f, None, rho, None)
         start = time.time()
         if self.ioRate == 0 and self.printInfoRate == 0 and not self.returnFpost and t_max >= 2:
@@ -296,12 +296,12 @@ class AdvectionDiffusionReactionBGK(LBMBase):
             io_flag = self.ioRate > 0 and (timestep % self.ioRate == 0 or timestep == t_max)
             print_iter_flag = self.printInfoRate > 0 and timestep % self.printInfoRate == 0
             if io_flag:
-                rho_prev = self.update_macroscopic(f).cpu()
+                rho_prev = io_field(self.update_macroscopic(f), self.downsamplingFactor)
             f, fstar = self.step(f, timestep, return_fpost=self.returnFpost)
             if print_iter_flag:
                 print(f"Timestep {timestep} of {t_max} completed")
             if io_flag:
-                rho = self.update_macroscopic(f).cpu()
+                rho = io_field(self.update_macroscopic(f), self.downsamplingFactor)
                 self.handle_io_timestep(timestep, f, fstar, rho, rho_prev)
             if self.computeMLUPS and timestep == 1:
                 torch.cuda.synchronize()

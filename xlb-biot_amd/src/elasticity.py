@@ -16,7 +16,7 @@ import torch
 
 from . import _native as nat
 from . import autotune
-from .base import (ConvergedStepRunner, NormWorkspace, StepRunner, PrecisionPolicy, as_soa, default_device, graph_capture, no_capture,
+from .base import (ConvergedStepRunner, NormWorkspace, StepRunner, PrecisionPolicy, as_soa, default_device, graph_capture, io_field, no_capture,
                    torch_dtype, view)
 from .boundary_conditions import BoundaryCondition, build_rule_table, configure_bcs, create_grid_mask
 
@@ -868,12 +868,12 @@ class LBMElasticity(object):
             io_flag = self.ioRate > 0 and (timestep % self.ioRate == 0 or timestep == t_max)
             print_iter_flag = self.printInfoRate > 0 and timestep % self.printInfoRate == 0
             if io_flag:
-                eta_prev, sigma_prev = (x.cpu() for x in self.update_macroscopic(f))
+                eta_prev, sigma_prev = (io_field(x, self.downsamplingFactor) for x in self.update_macroscopic(f))
             f, fstar = self.step(f, timestep, return_fpost=self.returnFpost)
             if print_iter_flag:
                 print(f"Timestep {timestep} of {t_max} completed")
             if io_flag:
-                eta, sigma = (x.cpu() for x in self.update_macroscopic(f))
+                eta, sigma = (io_field(x, self.downsamplingFactor) for x in self.update_macroscopic(f))
                 self.handle_io_timestep(timestep, f, fstar, eta, sigma, eta_prev, sigma_prev)
             if self.computeMLUPS and timestep == 1:
                 torch.cuda.synchronize()

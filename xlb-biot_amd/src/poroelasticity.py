@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .base import as_soa, graph_capture, view
+from .base import as_soa, graph_capture, io_field, view
 
 
 class BiotNonlinearForce:
@@ -357,8 +357,9 @@ class LBMPoroelasticity(object):
             if print_iter_flag:
                 print(f"Timestep {timestep} of {t_max} completed")
             if io_flag:
-                self.handle_io_timestep(timestep, view(fe), view(fd), view(eta).cpu(), view(sigma).cpu(),
-                                        view(pressure).cpu(), view(pressure_grad).cpu(), view(dde).cpu())
+                g = lambda x: io_field(view(x), self.downsamplingFactor)
+                self.handle_io_timestep(timestep, view(fe), view(fd), g(eta), g(sigma), g(pressure), g(pressure_grad),
+                                        g(dde))
             if self.computeMLUPS and timestep == 1:
                 torch.cuda.synchronize()
                 start = time.time()
