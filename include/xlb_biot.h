@@ -15,6 +15,13 @@
  *    The reference's (nx, ny, c) arrays are the permuted views of these.
  *  - dtype: XB_F64 or XB_F32; all operands of one call share it (the reference's
  *    "f64/f64" and "f32/f32" policies).  Norms are always accumulated in FP64.
+ *    XB_F64_S32 is the reference's "f64/f32" (elasticity.py:606-629: FP64
+ *    arithmetic, FP32 storage): FP64 buffers and arithmetic, and every population
+ *    value rounded to FP32 where the reference stores it in its FP32 arrays (the
+ *    collision's cast_to_output, each boundary-rule stage, the sweep's output, the
+ *    prolongation's sum).  Only xb_elast_step, xb_elast_partials_count,
+ *    xb_darcy_step, xb_elast_prolong_add and xb_elast_prolong_add_div take it;
+ *    every other entry point returns XB_EUNSUPPORTED for it.
  *  - Every call takes the HIP stream to launch on (hipStream_t as void*) and
  *    returns an int status: XB_OK, or a negative XB_E* code; the message of the
  *    last failure on the calling thread is returned by xb_last_error().
@@ -33,7 +40,7 @@ extern "C" {
 
 #define XB_ABI_VERSION 1
 
-enum { XB_F64 = 0, XB_F32 = 1 };
+enum { XB_F64 = 0, XB_F32 = 1, XB_F64_S32 = 2 };
 
 enum {
   XB_OK = 0,

@@ -122,8 +122,13 @@ def test_solver_validation_errors_match_reference():
         LBMElasticity(**{**kw, "MG_BCs": {"left": 1}})
     with pytest.raises(ValueError):
         LBMElasticity(**{**kw, "lattice": LatticeD2Q9("f64/f64")})
-    with pytest.raises(NotImplementedError):
-        LBMElasticity(**{**kw, "precision": "f64/f32"})
+    # elasticity.py:606-629: "f64/f32" is FP64 arithmetic with FP32 storage; half
+    # precision storage is not carried by these kernels
+    mixed = LBMElasticity(**{**kw, "precision": "f64/f32"}).precisionPolicy
+    assert mixed.mixed and mixed.compute_dtype is np.float64 and mixed.output_dtype is np.float32
+    for prec in ("f32/f16", "f64/f16", "f16/f16"):
+        with pytest.raises(NotImplementedError):
+            LBMElasticity(**{**kw, "precision": prec})
     with pytest.raises(TypeError):
         P.biot_periodic("gpu", nx=16, coupling="sideways")
 

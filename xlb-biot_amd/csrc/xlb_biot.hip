@@ -68,7 +68,14 @@ struct SweepGeom {
   int rows;               // rows per wavefront item
   int nstrips, nitems;
   int ix0, ix1, iy0, iy1; // box carrying the subtracted residual
+  int round32;            // XB_F64_S32: population values rounded to FP32 where stored
 };
+
+// FP32 storage of FP64 arithmetic (XB_F64_S32): the value an FP32 array would hold
+template <typename T, int Q> __device__ __forceinline__ void store32(T (&v)[Q]) {
+#pragma unroll
+  for (int d = 0; d < Q; ++d) v[d] = T(float(v[d]));
+}
 
 // Rows per wavefront: enough items to give every SIMD several waves, at most 64
 // and at least min_rows (2 for the residual sweep, whose restriction pairs rows).
@@ -204,9 +211,13 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
     } else {
       coll.collide(pre, rw.aux, pc);
     }
+    if (g.round32) store32(pc);
     if constexpr (HAS_BC) {
       stage_if_needed(rw.slot);
-      if (rw.slot >= 0 && bc.code_pc != nullptr) rules_pc(pc, pre, rw.slot);
+      if (rw.slot >= 0 && bc.code_pc != nullptr) {
+        rules_pc(pc, pre, rw.slot);
+        if (g.round32) store32(pc);
+      }
     }
   };
 
@@ -232,7 +243,10 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
       v[d] = s;
     }
     if constexpr (HAS_BC) {
-      if (S1 >= 0 && bc.code_ps != nullptr) rules_ps(v, R1, S1);
+      if (S1 >= 0 && bc.code_ps != nullptr) {
+        rules_ps(v, R1, S1);
+        if (g.round32) store32(v);
+      }
     }
     const size_t cell = (size_t)x * g.ny + y;
     if constexpr (MODE == XB_MODE_STEP) {
@@ -251,22 +265,24 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
       if (out_lane) {
         T o[Q];
 #pragma unroll
-        for (int d = 0; d < Q; ++d) {
-          o[d] = omg * F1[d] + gamma * v[d];
-          fout[d * N + cell] = o[d];
-        }
+        for (int d = 0; d < Q; ++d) o[d] = omg * F1[d] + gamma * v[d];
+        if (g.round32) store32(o);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) fout[d * N + cell] = o[d];
         if constexpr (HAS_NL && Q == 8) {
           if (nl.div_out != nullptr) nl.div_out[cell] = elast_div_of<T>(o, K1);
         }
       }
     } else {
       // residual r = f - S(f - r_in) on the interior box, 2x2 sum, squared norm
-      T pair[Q];
+      T pair[Q], r[Q];
+#pragma unroll
+      for (int d = 0; d < Q; ++d) r[d] = F1[d] - v[d];
+      if (g.round32) store32(r);
 #pragma unroll
       for (int d = 0; d < Q; ++d) {
-        const T r = F1[d] - v[d];
-        if (out_lane) nrm += (double)r * (double)r;
-        pair[d] = r + from_next_lane(r);
+        if (out_lane) nrm += (double)r[d] * (double)r[d];
+        pair[d] = r[d] + from_next_lane(r[d]);
       }
       if (((x - g.ox) & 1) == 0) {
 #pragma unroll
@@ -624,23 +640,7 @@ inline dim3 ew_grid(int64_t n) {
 
 template <typename T>
 __global__ void k_prolong_add(int nx, int ny, T *__restrict__ f, const T *__restrict__ c, int cnx, int cny, int ix0,
-                              int ix1, int iy0, int iy1) {
-  const int nxi = ix1 - ix0, nyi = iy1 - iy0;
-  const int64_t n = (int64_t)nxi * nyi;
-  const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-    const int xi = (int)(t / nyi), yi = (int)(t % nyi);
-    const size_t cell = (size_t)(ix0 + xi) * ny + (iy0 + yi);
-    const size_t cc = (size_t)(ix0 + (xi >> 1)) * cny + (iy0 + (yi >> 1));
-#pragma unroll
-    for (int d = 0; d < 8; ++d) f[d * N + cell] = f[d * N + cell] + c[d * Nc + cc];
-  }
-}
-
-template <typename T>
-__global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__restrict__ c, int cnx, int cny,
-                                  int ix0, int ix1, int iy0, int iy1, T k, T *__restrict__ div,
-                                  const T *__restrict__ lamf, const T *__restrict__ muf) {
+                              int ix1, int iy0, int iy1, int round32) {
   const int nxi = ix1 - ix0, nyi = iy1 - iy0;
   const int64_t n = (int64_t)nxi * nyi;
   const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
@@ -650,10 +650,30 @@ __global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__
     const size_t cc = (size_t)(ix0 + (xi >> 1)) * cny + (iy0 + (yi >> 1));
     T v[8];
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      v[d] = f[d * N + cell] + c[d * Nc + cc];
-      f[d * N + cell] = v[d];
-    }
+    for (int d = 0; d < 8; ++d) v[d] = f[d * N + cell] + c[d * Nc + cc];
+    if (round32) store32(v);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) f[d * N + cell] = v[d];
+  }
+}
+
+template <typename T>
+__global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__restrict__ c, int cnx, int cny,
+                                  int ix0, int ix1, int iy0, int iy1, T k, T *__restrict__ div,
+                                  const T *__restrict__ lamf, const T *__restrict__ muf, int round32) {
+  const int nxi = ix1 - ix0, nyi = iy1 - iy0;
+  const int64_t n = (int64_t)nxi * nyi;
+  const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int xi = (int)(t / nyi), yi = (int)(t % nyi);
+    const size_t cell = (size_t)(ix0 + xi) * ny + (iy0 + yi);
+    const size_t cc = (size_t)(ix0 + (xi >> 1)) * cny + (iy0 + (yi >> 1));
+    T v[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) v[d] = f[d * N + cell] + c[d * Nc + cc];
+    if (round32) store32(v);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) f[d * N + cell] = v[d];
     div[cell] = elast_div_of<T>(v, lamf != nullptr ? elast_kdiv<T>(lamf[cell], muf[cell]) : k);
   }
 }
@@ -1285,13 +1305,14 @@ template <typename T> void elast_coll(const xb_elast_args *a, ElastColl<T> &c, N
   }
 }
 
-template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st) {
+template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st, int round32 = 0) {
   ElastColl<T> c;
   NLDev<T> nl;
   elast_coll<T>(a, c, nl);
   const NLDev<T> *nlp = a->div_in != nullptr ? &nl : nullptr;
   SweepGeom geo;
   elast_geom(a, geo);
+  geo.round32 = round32;
   switch (a->mode) {
   case XB_MODE_STEP:
     return launch_sweep<T, ElastColl<T>, XB_MODE_STEP>(geo, c, a->f_in, a->f_out, a->fpc_out, a->resid, a->bc, 1.0,
@@ -3122,8 +3143,15 @@ int64_t xb_elast_partials_count(const xb_elast_args *a) {
 }
 
 int xb_elast_step(const xb_elast_args *a, void *stream) {
-  if (int r = validate_elast(a, "xb_elast_step")) return r;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (a != nullptr && a->dtype == XB_F64_S32) {
+    // FP64 buffers and arithmetic, FP32 storage: the FP64 sweep with its stores rounded
+    xb_elast_args b = *a;
+    b.dtype = XB_F64;
+    if (int r = validate_elast(&b, "xb_elast_step")) return r;
+    return elast_step_t<double>(&b, st, 1);
+  }
+  if (int r = validate_elast(a, "xb_elast_step")) return r;
   return a->dtype == XB_F64 ? elast_step_t<double>(a, st) : elast_step_t<float>(a, st);
 }
 
@@ -3150,6 +3178,8 @@ int xb_elast_step2(const xb_elast_args *a, void *stream) {
 
 int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const void *coarse, int32_t cnx, int32_t cny,
                          int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1, void *stream) {
+  const int round32 = dtype == XB_F64_S32 ? 1 : 0; // FP64 arithmetic, FP32 storage
+  if (round32) dtype = XB_F64;
   if (int r = check_dtype(dtype)) return r;
   if (f == nullptr || coarse == nullptr) return fail(XB_EINVAL, "xb_elast_prolong_add: null pointer");
   if (ix0 < 0 || ix1 > nx || iy0 < 0 || iy1 > ny || ix0 >= ix1 || iy0 >= iy1 || ((ix1 - ix0) & 1) || ((iy1 - iy0) & 1))
@@ -3160,16 +3190,18 @@ int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const v
   const dim3 gr = ew_grid((int64_t)(ix1 - ix0) * (iy1 - iy0));
   if (dtype == XB_F64)
     hipLaunchKernelGGL(k_prolong_add<double>, gr, dim3(256), 0, st, nx, ny, (double *)f, (const double *)coarse, cnx,
-                       cny, ix0, ix1, iy0, iy1);
+                       cny, ix0, ix1, iy0, iy1, round32);
   else
     hipLaunchKernelGGL(k_prolong_add<float>, gr, dim3(256), 0, st, nx, ny, (float *)f, (const float *)coarse, cnx, cny,
-                       ix0, ix1, iy0, iy1);
+                       ix0, ix1, iy0, iy1, 0);
   return check_launch("xb_elast_prolong_add");
 }
 
 int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, void *f,
                              const void *coarse, int32_t cnx, int32_t cny, int32_t ix0, int32_t ix1, int32_t iy0,
                              int32_t iy1, void *div, const void *lam_field, const void *mu_field, void *stream) {
+  const int round32 = dtype == XB_F64_S32 ? 1 : 0; // FP64 arithmetic, FP32 storage
+  if (round32) dtype = XB_F64;
   if (int r = check_dtype(dtype)) return r;
   if (f == nullptr || coarse == nullptr || div == nullptr)
     return fail(XB_EINVAL, "xb_elast_prolong_add_div: null pointer");
@@ -3185,11 +3217,11 @@ int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, 
   if (dtype == XB_F64)
     hipLaunchKernelGGL(k_prolong_add_div<double>, gr, dim3(256), 0, st, nx, ny, (double *)f, (const double *)coarse,
                        cnx, cny, ix0, ix1, iy0, iy1, k, (double *)div, (const double *)lam_field,
-                       (const double *)mu_field);
+                       (const double *)mu_field, round32);
   else
     hipLaunchKernelGGL(k_prolong_add_div<float>, gr, dim3(256), 0, st, nx, ny, (float *)f, (const float *)coarse, cnx,
                        cny, ix0, ix1, iy0, iy1, (float)k, (float *)div, (const float *)lam_field,
-                       (const float *)mu_field);
+                       (const float *)mu_field, 0);
   return check_launch("xb_elast_prolong_add_div");
 }
 
@@ -3261,15 +3293,18 @@ int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, doub
 
 int xb_darcy_step(const xb_darcy_args *a, void *stream) {
   if (a == nullptr) return fail(XB_EINVAL, "xb_darcy_step: null args");
-  if (int r = check_dtype(a->dtype)) return r;
+  const int round32 = a->dtype == XB_F64_S32 ? 1 : 0; // FP64 arithmetic, FP32 storage
+  const int32_t dtype = round32 ? XB_F64 : a->dtype;
+  if (int r = check_dtype(dtype)) return r;
   if (a->nx <= 0 || a->ny <= 0 || a->f_in == nullptr || a->f_out == nullptr)
     return fail(XB_EINVAL, "xb_darcy_step: bad arguments");
   if (int r = check_bc(a->bc, false)) return r;
   if (a->rows != 0 && (a->rows < 1 || a->rows > 64 || (a->rows & (a->rows - 1))))
     return fail(XB_EINVAL, "xb_darcy_step: rows must be 0 or a power of two in [1, 64]");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny, a->rows, STRIP, 1);
-  if (a->dtype == XB_F64) {
+  SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny, a->rows, STRIP, 1);
+  g.round32 = round32;
+  if (dtype == XB_F64) {
     DarcyColl<double> c;
     c.omega = a->omega;
     c.sfac = 1 - 0.5 * a->omega;

@@ -309,6 +309,7 @@ def elasticity_solve(el, lam_field, mu_field, f0, n, force=None, checkpoint_ever
     per-cell Lame fields ``(1, nx, ny)`` (or ``(nx, ny)``), an optional external force
     ``(2, nx, ny)`` and initial populations ``f0`` ``(8, nx, ny)``, all SoA.  Differentiable with respect
     to the fields, the force and ``f0``."""
+    el.precisionPolicy.require_single("elasticity_solve")
     return _ElasticitySolveFn.apply(lam_field, mu_field, force, f0, el, int(n), int(checkpoint_every), use_graphs,
                                     reuse)
 
@@ -353,6 +354,7 @@ def darcy_solve(da, omega_field, f0, n, source=None, checkpoint_every=50, use_gr
     Differentiable with respect to omega, the source and ``f0``.  For the
     permeability, omega = 1 / (3 K + 1/2): pass ``1 / (3 * K + 0.5)`` built from a
     ``K`` that requires grad."""
+    da.precisionPolicy.require_single("darcy_solve")
     return _DarcySolveFn.apply(omega_field, source, f0, da, int(n), int(checkpoint_every), use_graphs, reuse)
 
 
@@ -386,6 +388,7 @@ def elasticity_steady_adjoint(el, lam_field, mu_field, f_star, dJdf, force=None,
     """Gradient of ``J(f*)`` for the steady state ``f* = S(f*)`` of the single-grid
     elasticity step with per-cell Lame fields: returns ``(grads, a, steps, rel)``
     with ``grads["lam"]``, ``grads["mu"]`` (and ``grads["force"]``) FP64 fields."""
+    el.precisionPolicy.require_single("elasticity_steady_adjoint")
     st = ElasticityStepper(el, lam_field, mu_field, force)
     f = _soa(f_star, 8, el.nx, el.ny, el._tdt, el.device)
     return _fixed_point_adjoint(st, f, dJdf, tol, max_steps, check_every, force is not None)
@@ -394,6 +397,7 @@ def elasticity_steady_adjoint(el, lam_field, mu_field, f_star, dJdf, force=None,
 def darcy_steady_adjoint(da, omega_field, f_star, dJdf, source=None, tol=1e-10, max_steps=200000, check_every=50):
     """As :func:`elasticity_steady_adjoint` for the Darcy step: ``grads["omega"]``
     (and ``grads["source"]``)."""
+    da.precisionPolicy.require_single("darcy_steady_adjoint")
     st = DarcyStepper(da, omega_field, source)
     f = _soa(f_star, 9, da.nx, da.ny, da._tdt, da.device)
     return _fixed_point_adjoint(st, f, dJdf, tol, max_steps, check_every, source is not None)

@@ -38,7 +38,9 @@ class PrecisionPolicy:
 
     ``compute_dtype`` / ``output_dtype`` are NumPy dtypes, so driver hooks that pass
     them to NumPy keep working; ``torch_compute`` / ``torch_output`` are the torch ones.
-    This backend computes and stores in one precision: ``"f64/f64"`` or ``"f32/f32"``.
+    Supported: ``"f64/f64"``, ``"f32/f32"`` and ``"f64/f32"`` (``mixed``: FP64 buffers and
+    arithmetic, every population value rounded to FP32 where the reference stores
+    it, the states handed out in FP32).  Half-precision storage raises.
     """
     TABLE = {"f64/f64": (np.float64, np.float64), "f32/f32": (np.float32, np.float32),
              "f32/f16": (np.float32, np.float16), "f16/f16": (np.float16, np.float16),
@@ -47,15 +49,21 @@ class PrecisionPolicy:
     def __init__(self, precision):
         # elasticity.py:606-629: unknown strings fall back to f32/f32
         cdt, odt = self.TABLE.get(precision, (np.float32, np.float32))
-        if cdt is not odt:
+        if np.float16 in (cdt, odt):
             raise NotImplementedError(
-                f"precision {precision!r}: mixed or half precision storage is not implemented on this backend "
-                "(use 'f64/f64' or 'f32/f32')")
+                f"precision {precision!r}: half-precision storage is not implemented on this backend "
+                "(use 'f64/f64', 'f32/f32' or 'f64/f32')")
         self.compute_dtype = cdt
         self.param_dtype = cdt
         self.output_dtype = odt
         self.torch_compute = torch_dtype(cdt)
         self.torch_output = torch_dtype(odt)
+        self.mixed = cdt is not odt
+
+    def require_single(self, what):
+        """Raise for the ``"f64/f32"`` policy in paths that have no FP32-storage form."""
+        if self.mixed:
+            raise NotImplementedError(f"{what} is implemented for 'f64/f64' and 'f32/f32' only")
 
     def cast_to_compute(self, x):
         return None if x is None else x.to(self.torch_compute)

@@ -153,6 +153,7 @@ class ElasticityBatch:
         for el in solvers:
             if el.levels < 2:
                 raise ValueError("batched V-cycles need MG_levels >= 2 for every instance")
+            el.precisionPolicy.require_single("the batched solve")
             if el._tdt != dt:
                 raise ValueError("all instances of a batch share one precision")
             if el.k_smooth != ks:

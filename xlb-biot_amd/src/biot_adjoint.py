@@ -453,4 +453,5 @@ def biot_solve(problem, omega_field, lam_field, mu_field, steps, checkpoint_ever
     with the per-cell Darcy rate ``omega_field`` and Lame fields ``lam_field``,
     ``mu_field`` (each ``(nx, ny)`` or ``(1, nx, ny)``).  Differentiable with respect
     to the three fields; V-cycle checkpoints every ``checkpoint_every`` V-cycles."""
+    problem.elasticity.precisionPolicy.require_single("biot_solve")
     return _BiotSolveFn.apply(omega_field, lam_field, mu_field, problem, int(steps), int(checkpoint_every))

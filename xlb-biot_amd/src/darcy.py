@@ -52,6 +52,19 @@ class AdvectionDiffusionReactionBGK(LBMBase):
     def _tdt(self):
         return self.precisionPolicy.torch_compute
 
+    def _code(self, t):
+        """Kernel dtype of a population buffer: FP32 storage of FP64 arithmetic under
+        the ``"f64/f32"`` policy."""
+        return nat.XB_F64_S32 if self.precisionPolicy.mixed else nat.dtype_code(t.dtype)
+
+    def _out(self, soa):
+        """A population state handed to the caller, in the storage precision."""
+        if soa is None:
+            return None
+        if self.precisionPolicy.mixed:
+            soa = soa.to(self.precisionPolicy.torch_output)
+        return view(soa)
+
     @property
     def omega_field(self):
         """Per-cell relaxation rate ``(nx, ny, 1)`` or None."""
@@ -141,7 +154,7 @@ class AdvectionDiffusionReactionBGK(LBMBase):
 
     def _args(self, f, out, ext_source, fpc, two=False):
         table = self._bc_table
-        return nat.DarcyArgs(self.nx, self.ny, nat.dtype_code(f.dtype), 0, self.omega, nat.ptr(self._omega_field),
+        return nat.DarcyArgs(self.nx, self.ny, self._code(f), 0, self.omega, nat.ptr(self._omega_field),
                              nat.ptr(f), nat.ptr(out), nat.ptr(fpc), nat.ptr(self._source_soa), nat.ptr(ext_source),
                              table.ref() if table is not None else None, autotune.lookup(self._tune_key(two)), 0)
 
@@ -193,7 +206,10 @@ class AdvectionDiffusionReactionBGK(LBMBase):
     def _multi(self):
         """The plain-step launch of step loops (choice, steps per launch,
         launch(src, dst)): ``self.step_launch`` when set (an ``autotune`` choice),
-        else the tuned one."""
+        else the tuned one.  FP32 storage (``"f64/f32"``) is rounded by the single-step
+        sweep only."""
+        if self.precisionPolicy.mixed:
+            return autotune.SWEEP1, 1, None
         choice = getattr(self, "step_launch", None) or autotune.lookup(self._tune_key() + "/multi")
         if choice == 0:
             choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
@@ -225,7 +241,7 @@ class AdvectionDiffusionReactionBGK(LBMBase):
             me = weakref.proxy(self)
             runner = self._runners[key] = StepRunner(lambda s, d: me._step_soa(s, None, False, out=d), fs, chunk,
                                                      multi, m)
-        return view(runner.run(fs, n))
+        return self._out(runner.run(fs, n))
 
     def run_steps_to(self, f, tol, check_every=10, max_steps=1000000, checks=10):
         """Steps from ``f`` until ``|f_k - f_{k-1}| < tol |f_1 - f_0|``, tested every
@@ -241,7 +257,7 @@ class AdvectionDiffusionReactionBGK(LBMBase):
             runner = self._runners[key] = ConvergedStepRunner(lambda s, d: me._step_soa(s, None, False, out=d), fs,
                                                               check_every, checks, multi, m)
         out, k, ratio = runner.run(fs, tol, max_steps)
-        return view(out), k, ratio
+        return self._out(out), k, ratio
 
     def collision(self, f, external_source=None):
         fs = self._soa(f, 9)
@@ -249,19 +265,19 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         a = self._args(fs, out, self._soa(external_source, 1), pc)
         a.bc = None
         nat.check(nat.lib().xb_darcy_step(a, nat.stream()), "xb_darcy_step(collision)")
-        return view(pc)
+        return self._out(pc)
 
     def step(self, f_poststreaming, timestep, return_fpost=False, external_source=None):
         fs = self._soa(f_poststreaming, 9)
         out, pc = self._step_soa(fs, self._soa(external_source, 1), return_fpost)
-        return view(out), (view(pc) if pc is not None else None)
+        return self._out(out), self._out(pc)
 
     # ---- base.py:528-579 (zero-velocity equilibrium rho * w)
     def assign_fields_sharded(self):
         rho0, u0 = self.initialize_macroscopic_fields()
         w = torch.as_tensor(np.asarray(self.w, dtype=np.float64), dtype=self._tdt, device=self.device)
         if rho0 is None or u0 is None:
-            return view(w.view(9, 1, 1).expand(9, self.nx, self.ny).contiguous())
+            return self._out(w.view(9, 1, 1).expand(9, self.nx, self.ny).contiguous())
         return self.initialize_populations(rho0, u0)
 
     def initialize_populations(self, rho0, u0):
@@ -275,7 +291,7 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         for i in range(9):
             nat.check(nat.lib().xb_lincomb(rho.numel(), nat.dtype_code(self._tdt), float(w_host[i]), nat.ptr(rho),
                                            0.0, None, nat.ptr(f[i]), nat.stream()), "xb_lincomb")
-        return view(f)
+        return self._out(f)
 
     # ---- darcy.py:173-252 (t_max steps, from timestep 1)
     def run(self, t_max):

@@ -36,6 +36,11 @@ class LBMElasticity(object):
         self.k_smooth = kwargs.get("MG_n_smoothing", 2)
         self.precision = kwargs.get("precision")
         self.precisionPolicy = PrecisionPolicy(self.precision)
+        if self.precisionPolicy.mixed:
+            # FP32 storage is rounded by the single sweeps: no multi-step smoothing
+            # tiles, no single-workgroup coarse tail
+            self.use_tail = False
+            self.smooth_tile_steps = 1
         self.lattice = kwargs.get("lattice")
         self.downsamplingFactor = kwargs.get("downsampling_factor", 1)
         self.printInfoRate = kwargs.get("print_info_rate", 100)
@@ -101,6 +106,19 @@ class LBMElasticity(object):
     @property
     def _tdt(self):
         return self.precisionPolicy.torch_compute
+
+    def _code(self, t):
+        """Kernel dtype of a population buffer: FP32 storage of FP64 arithmetic under
+        the ``"f64/f32"`` policy."""
+        return nat.XB_F64_S32 if self.precisionPolicy.mixed else nat.dtype_code(t.dtype)
+
+    def _out(self, soa):
+        """A population state handed to the caller, in the storage precision."""
+        if soa is None:
+            return None
+        if self.precisionPolicy.mixed:
+            soa = soa.to(self.precisionPolicy.torch_output)
+        return view(soa)
 
     # ---- validated attributes (elasticity.py:150-354)
     @property
@@ -363,7 +381,7 @@ class LBMElasticity(object):
         if use_force is None:
             use_force = level == 0
         fa = self._force_soa if use_force else None
-        a = nat.ElastArgs(nx, ny, nat.dtype_code(f_in.dtype), mode, self.lamda, self.mu, self.gamma,
+        a = nat.ElastArgs(nx, ny, self._code(f_in), mode, self.lamda, self.mu, self.gamma,
                           nat.ptr(f_in), nat.ptr(f_out), nat.ptr(fpc), nat.ptr(fa), nat.ptr(ext), nat.ptr(resid),
                           x0, x1, y0, y1, table.ref() if table is not None else None,
                           nat.ptr(coarse), nat.ptr(partials),
@@ -462,6 +480,9 @@ class LBMElasticity(object):
         return {key: autotune.tune_multi(key, launches)}
 
     def _multi(self, ext):
+        if self.precisionPolicy.mixed:
+            # FP32 storage is rounded by the single-step sweep only
+            return autotune.SWEEP1, 1, None
         choice = autotune.lookup(self._multi_key(ext is not None))
         if choice == 0:
             choice = autotune.SWEEP2 if min(self.nx, self.ny) >= 2 else autotune.SWEEP1
@@ -492,7 +513,7 @@ class LBMElasticity(object):
             runner = StepRunner(lambda s, d: me._step_into(s, d, ext), fs, chunk, multi, m)
             if external_force is None:
                 self._runners[key] = runner
-        return view(runner.run(fs, n))
+        return self._out(runner.run(fs, n))
 
     def run_steps_to(self, f, tol, check_every=10, max_steps=1000000, checks=10):
         """Single-grid steps from ``f`` until ``|f_k - f_{k-1}| < tol |f_1 - f_0|``, tested
@@ -511,7 +532,7 @@ class LBMElasticity(object):
             runner = ConvergedStepRunner(lambda s, d: me._step_into(s, d, None), fs, check_every, checks, multi, m)
             self._runners[key] = runner
         out, k, ratio = runner.run(fs, tol, max_steps)
-        return view(out), k, ratio
+        return self._out(out), k, ratio
 
     def _smooth(self, f, level, resid, ext, out=None):
         out = torch.empty_like(f) if out is None else out
@@ -539,7 +560,7 @@ class LBMElasticity(object):
         nx, ny = self.grid_size(level)
         cnx, cny = self.grid_size(level + 1)
         x0, x1, y0, y1 = self.interior_box(level)
-        nat.check(nat.lib().xb_elast_prolong_add(nx, ny, nat.dtype_code(f.dtype), nat.ptr(f), nat.ptr(f_c), cnx,
+        nat.check(nat.lib().xb_elast_prolong_add(nx, ny, self._code(f), nat.ptr(f), nat.ptr(f_c), cnx,
                                                  cny, x0, x1, y0, y1, nat.stream()), "xb_elast_prolong_add")
 
     def _add(self, a, b):
@@ -568,7 +589,7 @@ class LBMElasticity(object):
                                                  nat.ptr(eta_s), nat.ptr(sig_s), nat.ptr(self._force_soa),
                                                  nat.ptr(ext), nat.ptr(f), *self._fields(), nat.stream()),
                   "xb_elast_equilibrium")
-        return view(f)
+        return self._out(f) if cast_output else view(f)
 
     def update_macroscopic(self, f, external_force=None):
         fs = self._soa(f, 8)
@@ -602,7 +623,7 @@ class LBMElasticity(object):
                          use_force=use_force)
         a.bc = None
         nat.check(nat.lib().xb_elast_step(a, nat.stream()), "xb_elast_step(collision)")
-        return view(pc)
+        return self._out(pc)
 
     def step(self, f_poststreaming, timestep, return_fpost=False, external_force=None, nonlinear_force=None):
         """elasticity.py:1128-1167."""
@@ -610,7 +631,7 @@ class LBMElasticity(object):
             f = self._soa(f_poststreaming, 8)
             ext = self._ext_soa(self._soa(external_force, 2), nonlinear_force, f, 0)
             out, pc = self._run_step(f, ext, 0, return_fpost)
-            return view(out), (view(pc) if pc is not None else None)
+            return self._out(out), self._out(pc)
         return self._mg_step(f_poststreaming, timestep, external_force=external_force,
                              nonlinear_force=nonlinear_force)
 
@@ -751,7 +772,7 @@ class LBMElasticity(object):
             nxc, nyc = self.grid_size(level + 1)
             nx, ny = self.grid_size(level)
             x0, x1, y0, y1 = self.interior_box(level)
-            nat.check(nat.lib().xb_elast_prolong_add_div(nx, ny, nat.dtype_code(f.dtype), self.lamda, self.mu,
+            nat.check(nat.lib().xb_elast_prolong_add_div(nx, ny, self._code(f), self.lamda, self.mu,
                                                          nat.ptr(f), nat.ptr(f_c), nxc, nyc, x0, x1, y0, y1,
                                                          nat.ptr(div), *(self._fields() if level == 0 else (None, None)),
                                                          nat.stream()), "xb_elast_prolong_add_div")
@@ -778,7 +799,7 @@ class LBMElasticity(object):
         """elasticity.py:1169-1219."""
         f = self._soa(f_poststreaming, 8, level)
         out, pc = self._run_step(f, self._soa(external_force, 2, level), level, return_fpost)
-        return view(out), (view(pc) if pc is not None else None)
+        return self._out(out), self._out(pc)
 
     def _mg_step(self, f_poststreaming, timestep, level=0, residual=None, external_force=None, nonlinear_force=None):
         """elasticity.py:1221-1312; ``residual`` is shaped like the level's interior."""
@@ -789,8 +810,8 @@ class LBMElasticity(object):
             r = as_soa(residual, x1 - x0, y1 - y0, 8, self._tdt, self.device)
         out = self._vcycle(f, level, r, self._soa(external_force, 2, level), nonlinear_force)
         if level == 0:
-            return view(out[0]), out[1]
-        return view(out)
+            return self._out(out[0]), out[1]
+        return self._out(out)
 
     def _vcycle(self, f, level, residual, external_force, nonlinear_force, out=None):
         """One V-cycle from ``level`` down (elasticity.py:1221-1312) on SoA buffers; the
@@ -839,7 +860,7 @@ class LBMElasticity(object):
     def assign_fields_sharded(self):
         eta0, sigma0 = self.initialize_macroscopic_fields()
         if eta0 is None or sigma0 is None:
-            return view(torch.zeros((8, self.nx, self.ny), dtype=self._tdt, device=self.device))
+            return self._out(torch.zeros((8, self.nx, self.ny), dtype=self._tdt, device=self.device))
         return self.initialize_populations(eta0, sigma0)
 
     def initialize_populations(self, eta0, sigma0):
@@ -862,7 +883,7 @@ class LBMElasticity(object):
                 vg.load(self._soa(f, 8))
                 vg.run(t_max - 1)
                 self.last_residual = vg.norm
-                f = view(vg.state.clone())
+                f = self._out(vg.state.clone())
             return self._finish_mlups(f, start, t_max)
         for timestep in range(0, t_max + 1):
             io_flag = self.ioRate > 0 and (timestep % self.ioRate == 0 or timestep == t_max)

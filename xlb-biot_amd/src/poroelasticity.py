@@ -344,7 +344,9 @@ class LBMPoroelasticity(object):
         fe = self.elasticity._soa(f_elasticity, 8)
         fd = self.darcy._soa(f_darcy, 9)
         dde = self._soa(dt_div_eta, 1)
-        return tuple(view(t) for t in self._step_soa(fe, fd, dde, timestep))
+        out = self._step_soa(fe, fd, dde, timestep)
+        # populations in the storage precision, macroscopic fields in the compute one
+        return (self.elasticity._out(out[0]), self.darcy._out(out[1])) + tuple(view(t) for t in out[2:])
 
     # ---- driver (poroelasticity.py:208-301): t_max + 1 physical steps
     def run(self, t_max):
@@ -358,8 +360,8 @@ class LBMPoroelasticity(object):
                 print(f"Timestep {timestep} of {t_max} completed")
             if io_flag:
                 g = lambda x: io_field(view(x), self.downsamplingFactor)
-                self.handle_io_timestep(timestep, view(fe), view(fd), g(eta), g(sigma), g(pressure), g(pressure_grad),
-                                        g(dde))
+                self.handle_io_timestep(timestep, self.elasticity._out(fe), self.darcy._out(fd), g(eta), g(sigma),
+                                        g(pressure), g(pressure_grad), g(dde))
             if self.computeMLUPS and timestep == 1:
                 torch.cuda.synchronize()
                 start = time.time()
@@ -371,7 +373,7 @@ class LBMPoroelasticity(object):
             print(f"Domain: {nx} x {ny}")
             print(f"Number of voxels: {nx * ny}")
             print(f"MLUPS: {self.mlups}")
-        return view(fe), view(fd)
+        return self.elasticity._out(fe), self.darcy._out(fd)
 
     def handle_io_timestep(self, timestep, f_elasticity, f_darcy, eta, sigma, pressure, pressure_grad, dt_div_eta):
         self.output_data(timestep=timestep, eta=eta, sigma=sigma, dt_div_eta=dt_div_eta, pressure=pressure,
