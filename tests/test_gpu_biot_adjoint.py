@@ -172,6 +172,51 @@ def test_biot_gradient_matches_finite_differences_16_directions():
     assert worst < 1e-6
 
 
+def test_c4_geometry_gradient_matches_finite_differences_at_128():
+    # the C4 set-up at 128 x 130, FP64: 7 levels by the rule of
+    # examples/poroelasticity/2D.py:216-219, 8 smoothing steps, 2 V-cycles per step,
+    # log-normal K (variance 2, correlation 32, seed 1), lambda = 2 and mu = 1 held as
+    # fields, target from K perturbed by a log-normal factor (seed 2); gradient of the
+    # misfit with respect to K (through omega = 1 / (3 K + 1/2)) and mu along 16
+    # random directions against central differences; checkpoints every 4 V-cycles
+    from src.biot_adjoint import biot_solve
+    n, steps = 128, 6
+    levels = n.bit_length() - 1
+    while n % 2 ** (levels - 1):
+        levels -= 1
+    say("C4 geometry gradient at", n, "levels", levels)
+    p = P.biot_consolidation("gpu", nx=n, levels=levels, steps=2, k_smooth=8)
+    nx, ny = p.elasticity.nx, p.elasticity.ny
+    Kf = 0.5 * torch.as_tensor(P.lognormal_field(nx, ny, 2.0, 32.0, 1), dtype=torch.float64).cuda()
+    Kt = Kf * torch.as_tensor(P.lognormal_field(nx, ny, 0.1, 32.0, 2), dtype=torch.float64).cuda()
+    lam = torch.full((nx, ny), 2.0, dtype=torch.float64, device="cuda")
+    mu = torch.full((nx, ny), 1.0, dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        te, tp = (t.clone() for t in biot_solve(p, 1.0 / (3.0 * Kt + 0.5), lam, mu, steps, checkpoint_every=4))
+
+    def J(K, m):
+        e, q = biot_solve(p, 1.0 / (3.0 * K + 0.5), lam, m, steps, checkpoint_every=4)
+        return 0.5 * ((e - te) ** 2).sum() + 0.5 * ((q - tp) ** 2).sum()
+
+    Kg, mg = Kf.clone().requires_grad_(True), mu.clone().requires_grad_(True)
+    J(Kg, mg).backward()
+    gK, gm = Kg.grad.clone(), mg.grad.clone()
+    assert torch.isfinite(gK).all() and torch.isfinite(gm).all() and gK.abs().max() > 0
+    g = torch.Generator().manual_seed(7)
+    h = 1e-5
+    worst = 0.0
+    with torch.no_grad():
+        for i in range(16):
+            dK = Kf * torch.randn((nx, ny), generator=g, dtype=torch.float64).cuda()
+            dm = mu * torch.randn((nx, ny), generator=g, dtype=torch.float64).cuda()
+            fd = (J(Kf + h * dK, mu + h * dm).item() - J(Kf - h * dK, mu - h * dm).item()) / (2 * h)
+            an = (gK * dK).sum().item() + (gm * dm).sum().item()
+            worst = max(worst, _rel(fd, an))
+            say(i, fd, an)
+    say("worst relative difference", worst)
+    assert worst < 1e-6
+
+
 @pytest.mark.parametrize("case,precision", [("consolidation", "f64/f64"), ("consolidation", "f32/f32"),
                                             ("periodic", "f64/f64")])
 @pytest.mark.parametrize("level", [1, 2])
