@@ -2356,7 +2356,7 @@ __device__ __forceinline__ ElastMomBar<T> elast_collision_adjoint(const T (&ob)[
   M.B5 = T(0.5) * (ob[2] - ob[0]) + T(0.25) * (ob[4] - ob[5] - ob[6] + ob[7]);
   M.B6 = T(0.5) * (ob[3] - ob[1]) + T(0.25) * (ob[4] + ob[5] - ob[6] - ob[7]);
   // moments of the populations: m10, m01 (+F), m_s * cs, m_d * cd, m_11 * cd
-  const T b10 = M.B0 + M.B5 / T(3), b01 = M.B1 + M.B6 / T(3);
+  const T b10 = M.B0 + div_const(M.B5, T(3), T(1) / T(3)), b01 = M.B1 + div_const(M.B6, T(3), T(1) / T(3));
   const T bs = cs * M.B2, bd = cd * M.B3, b11 = cd * M.B4;
   out[0] = b10 + bs + bd + fpre[0];
   out[1] = b01 + bs - bd + fpre[1];
@@ -2433,8 +2433,8 @@ __global__ void __launch_bounds__(256) k_elast_adjoint(const ElastAdjDev<T> A) {
       if (A.g_mu != nullptr) A.g_mu[i] += gm;
     }
     if (A.g_force != nullptr) {
-      A.g_force[i] += (double)(M.B0 + M.B5 / T(6));
-      A.g_force[N + i] += (double)(M.B1 + M.B6 / T(6));
+      A.g_force[i] += (double)(M.B0 + div_const(M.B5, T(6), T(1) / T(6)));
+      A.g_force[N + i] += (double)(M.B1 + div_const(M.B6, T(6), T(1) / T(6)));
     }
   }
 }
@@ -2512,8 +2512,8 @@ __device__ __forceinline__ void elast_mg_adjoint_cell(const ElastMGAdjDev<T> &A,
       for (int d = 0; d < 8; ++d) A.rbar[d * Ni + ci] = A.rbar[d * Ni + ci] - yv[d];
     }
     if (A.fbar != nullptr) {
-      A.fbar[i] = M.B0 + M.B5 / T(6);
-      A.fbar[N + i] = M.B1 + M.B6 / T(6);
+      A.fbar[i] = M.B0 + div_const(M.B5, T(6), T(1) / T(6));
+      A.fbar[N + i] = M.B1 + div_const(M.B6, T(6), T(1) / T(6));
     }
     if (A.g_lam != nullptr) {
       T f[8];
