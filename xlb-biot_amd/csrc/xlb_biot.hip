@@ -1665,6 +1665,10 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
     if constexpr (GWALLS) gslot[k] = gslot[k] >= 0 ? gslot[k] : A.nl.gbc.nslots;
   }
   for (int s = 0; s < steps; ++s) {
+    // The residual is read again in every step (from L2): a pointer that the compiler
+    // must reload keeps it from hoisting the 32 values of the thread's cells out of
+    // the step loop and holding them in registers (which made this kernel spill).
+    const T *res = HAS_RES ? *reinterpret_cast<const T *const volatile *>(&A.res) : nullptr;
     if constexpr (HAS_NL) {
 #pragma unroll
       for (int k = 0; k < SMOOTH_CPT; ++k) ldiv[threadIdx.x + k * SMOOTH_THREADS] = elast_div_of<T>(f[k], kd[k]);
@@ -1680,7 +1684,7 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
       if constexpr (HAS_RES) {
         if (ri[k] >= 0) {
 #pragma unroll
-          for (int d = 0; d < Q; ++d) pre[d] = pre[d] - A.res[d * Ni + ri[k]];
+          for (int d = 0; d < Q; ++d) pre[d] = pre[d] - res[d * Ni + ri[k]];
         }
       }
       Aux ax = aux[k];
