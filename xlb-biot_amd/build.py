@@ -8,16 +8,22 @@ does not know the feature, which is harmless.  Floating-point contraction is off
 fuse multiply-adds differently in each copy, so results would depend in the last
 bit on the launch shape; without it every cell gets the same arithmetic wherever
 it is computed (the kernels are bandwidth-bound, the extra instructions are free).
+
+The source compiles as ``PARTS`` objects in parallel (``-DXB_PART=k``: each defines
+one group of the C entry points and instantiates only their kernels), linked into
+one shared library.
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = [os.path.join(HERE, "csrc", "xlb_biot.hip")]
 HEADERS = [os.path.join(HERE, "csrc", "xb_device.h"), os.path.join(ROOT, "include", "xlb_biot.h")]
 OUT = os.path.join(HERE, "lib", "libxlb_biot.so")
+PARTS = 11
 
 
 def hipcc():
@@ -27,16 +33,29 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"), "-o", tmp] + SOURCES
+def _run(cmd):
     res = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
+        raise RuntimeError(f"hipcc failed ({res.returncode}): {' '.join(cmd)}\n{err}")
+    return err
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    objdir = os.path.join(HERE, "lib", "obj")
+    os.makedirs(objdir, exist_ok=True)
+    flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-function",
+             "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")]
+    objs = [os.path.join(objdir, f"xlb_biot_part{k}.o") for k in range(PARTS)]
+    cmds = [[hipcc()] + flags + [f"-DXB_PART={k}", "-c", "-o", o] + SOURCES for k, o in enumerate(objs)]
+    jobs = jobs or min(PARTS, max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        errs = list(ex.map(_run, cmds))
+    tmp = OUT + ".tmp"
+    errs.append(_run([hipcc(), "--offload-arch=gfx950", "-fPIC", "-shared", "-o", tmp] + objs))
+    err = "\n".join(e for e in errs if e.strip())
     if verbose and err.strip():
         print(err, file=sys.stderr)
     os.replace(tmp, OUT)

@@ -26,14 +26,34 @@
 
 using namespace xb;
 
-namespace {
+// The library is compiled as one source in several parts (XB_PART = 0, 1, ...: each
+// part defines a disjoint group of the C entry points, so the parts compile in
+// parallel and link into one library); without XB_PART one compilation holds all.
+#ifndef XB_PART
+#define XB_IN_PART(n) 1
+#else
+#define XB_IN_PART(n) (XB_PART == (n))
+#endif
 
-thread_local char g_err[512] = "";
+// batched launches of the sweep and smoothing-tile kinds, each compiled in a part of
+// its own (they instantiate most of the kernels); the dtype is the plan's
+int xb_part_elast_batch_step(const xb_batch_plan *p, const int32_t *active, hipStream_t st);
+int xb_part_elast_batch_smooth(const xb_batch_plan *p, const int32_t *active, hipStream_t st);
+int xb_part_elast_batch_residual(const xb_batch_plan *p, const int32_t *active, hipStream_t st);
+int xb_part_smoothn_batch(const xb_batch_plan *p, const int32_t *active, hipStream_t st);
+
+// the calling thread's last error message (one definition, shared by every part)
+extern thread_local char xb_err_msg[512];
+#if XB_IN_PART(0)
+thread_local char xb_err_msg[512] = "";
+#endif
+
+namespace {
 
 int fail(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  vsnprintf(xb_err_msg, sizeof(xb_err_msg), fmt, ap);
   va_end(ap);
   return code;
 }
@@ -2188,11 +2208,11 @@ template <typename T> int batch_launch_t(const xb_batch_plan *p, const int32_t *
   const WorkListDev wl = plan_worklist(p);
   switch (p->kind) {
   case XB_BATCH_ELAST_STEP:
-    if (p->mode == XB_MODE_STEP) return launch_elast_batch<T, XB_MODE_STEP>(p, active, st);
-    if (p->mode == XB_MODE_SMOOTH) return launch_elast_batch<T, XB_MODE_SMOOTH>(p, active, st);
-    return launch_elast_batch<T, XB_MODE_RESIDUAL>(p, active, st);
+    if (p->mode == XB_MODE_STEP) return xb_part_elast_batch_step(p, active, st);
+    if (p->mode == XB_MODE_SMOOTH) return xb_part_elast_batch_smooth(p, active, st);
+    return xb_part_elast_batch_residual(p, active, st);
   case XB_BATCH_ELAST_SMOOTHN:
-    return launch_smoothn_batch<T>(p, active, st);
+    return xb_part_smoothn_batch(p, active, st);
   case XB_BATCH_PROLONG_DIV:
     hipLaunchKernelGGL(k_prolong_batch<T>, dim3(batch_grid(p, wl, reinterpret_cast<const void *>(&k_prolong_batch<T>), 256)),
                        dim3(256), 0, st, reinterpret_cast<const ProlongJob<T> *>(base + p->jobs_offset), bj, bi, active,
@@ -3132,12 +3152,37 @@ template <typename T> TailDev<T> tail_params(const xb_tail_plan *p) {
 
 } // namespace
 
+#if XB_IN_PART(7)
+int xb_part_elast_batch_step(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
+  return p->dtype == XB_F64 ? launch_elast_batch<double, XB_MODE_STEP>(p, active, st)
+                            : launch_elast_batch<float, XB_MODE_STEP>(p, active, st);
+}
+#endif
+#if XB_IN_PART(8)
+int xb_part_elast_batch_smooth(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
+  return p->dtype == XB_F64 ? launch_elast_batch<double, XB_MODE_SMOOTH>(p, active, st)
+                            : launch_elast_batch<float, XB_MODE_SMOOTH>(p, active, st);
+}
+#endif
+#if XB_IN_PART(9)
+int xb_part_elast_batch_residual(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
+  return p->dtype == XB_F64 ? launch_elast_batch<double, XB_MODE_RESIDUAL>(p, active, st)
+                            : launch_elast_batch<float, XB_MODE_RESIDUAL>(p, active, st);
+}
+#endif
+#if XB_IN_PART(10)
+int xb_part_smoothn_batch(const xb_batch_plan *p, const int32_t *active, hipStream_t st) {
+  return p->dtype == XB_F64 ? launch_smoothn_batch<double>(p, active, st) : launch_smoothn_batch<float>(p, active, st);
+}
+#endif
+
 // ============================================================================
 extern "C" {
 
+#if XB_IN_PART(0)
 int xb_abi_version(void) { return XB_ABI_VERSION; }
 
-const char *xb_last_error(void) { return g_err; }
+const char *xb_last_error(void) { return xb_err_msg; }
 
 int64_t xb_elast_partials_count(const xb_elast_args *a) {
   if (a == nullptr) return -1;
@@ -3146,6 +3191,8 @@ int64_t xb_elast_partials_count(const xb_elast_args *a) {
   return g.nitems;
 }
 
+#endif
+#if XB_IN_PART(1)
 int xb_elast_step(const xb_elast_args *a, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (a != nullptr && a->dtype == XB_F64_S32) {
@@ -3180,6 +3227,8 @@ int xb_elast_step2(const xb_elast_args *a, void *stream) {
   return launch_sweep2<float>(g, c, a->f_in, a->f_out, a->bc, st);
 }
 
+#endif
+#if XB_IN_PART(0)
 int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const void *coarse, int32_t cnx, int32_t cny,
                          int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1, void *stream) {
   const int round32 = dtype == XB_F64_S32 ? 1 : 0; // FP64 arithmetic, FP32 storage
@@ -3295,6 +3344,8 @@ int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, doub
   return check_launch("xb_elast_equilibrium");
 }
 
+#endif
+#if XB_IN_PART(2)
 int xb_darcy_step(const xb_darcy_args *a, void *stream) {
   if (a == nullptr) return fail(XB_EINVAL, "xb_darcy_step: null args");
   const int round32 = a->dtype == XB_F64_S32 ? 1 : 0; // FP64 arithmetic, FP32 storage
@@ -3358,6 +3409,8 @@ int xb_darcy_step2(const xb_darcy_args *a, void *stream) {
   return launch_sweep2<float>(g, c, a->f_in, a->f_out, a->bc, st);
 }
 
+#endif
+#if XB_IN_PART(0)
 int xb_darcy_macro(int32_t nx, int32_t ny, int32_t dtype, double omega, const void *omega_field, const void *f,
                    const void *source_a, const void *source_b, void *p_out, void *grad_out, void *stream) {
   if (int r = check_dtype(dtype)) return r;
@@ -3444,6 +3497,8 @@ int xb_norm_finalize(const double *partials, int32_t n, double *out, void *strea
   return check_launch("xb_norm_finalize");
 }
 
+#endif
+#if XB_IN_PART(3)
 int64_t xb_batch_table_bytes(int32_t kind, int32_t dtype, const void *jobs, int32_t njobs) {
   if (check_dtype(dtype) != XB_OK || jobs == nullptr || njobs <= 0) return -1;
   if (kind < XB_BATCH_ELAST_STEP || kind > XB_BATCH_ELAST_SMOOTHN) return -1;
@@ -3505,6 +3560,8 @@ int xb_batch_converge(const xb_batch_plan *plan, int32_t ninst, const xb_converg
   return check_launch("xb_batch_converge");
 }
 
+#endif
+#if XB_IN_PART(4)
 int xb_elast_adjoint(const xb_elast_adjoint_args *a, void *stream) {
   if (a == nullptr) return fail(XB_EINVAL, "xb_elast_adjoint: null args");
   if (int r = check_dtype(a->dtype)) return r;
@@ -3566,6 +3623,8 @@ int xb_darcy_adjoint(const xb_darcy_adjoint_args *a, void *stream) {
   return check_launch("xb_darcy_adjoint");
 }
 
+#endif
+#if XB_IN_PART(3)
 int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream) {
   if (plan == nullptr || plan->kind != XB_BATCH_NORM || plan->table == nullptr || out == nullptr)
     return fail(XB_EINVAL, "xb_batch_norms: needs a norm plan and an output");
@@ -3576,6 +3635,8 @@ int xb_batch_norms(const xb_batch_plan *plan, double *out, void *stream) {
   return check_launch("xb_batch_norms");
 }
 
+#endif
+#if XB_IN_PART(4)
 int xb_elast_mg_adjoint(const xb_elast_mg_adjoint_args *a, void *stream) {
   if (a == nullptr) return fail(XB_EINVAL, "xb_elast_mg_adjoint: null args");
   if (int r = check_dtype(a->dtype)) return r;
@@ -3750,6 +3811,8 @@ int xb_darcy_macro_adjoint(int32_t nx, int32_t ny, int32_t dtype, double omega, 
   return check_launch("xb_darcy_macro_adjoint");
 }
 
+#endif
+#if XB_IN_PART(0)
 int64_t xb_vcycle_tail_bytes(int32_t nlevels) {
   if (nlevels < 1 || nlevels > TAIL_MAX_LEVELS) return 0;
   return (int64_t)sizeof(TailLevelDev<double>) * nlevels;
@@ -3842,6 +3905,8 @@ int xb_vcycle_tail(const xb_tail_plan *plan, void *stream) {
   return check_launch("xb_vcycle_tail");
 }
 
+#endif
+#if XB_IN_PART(2)
 int xb_darcy_stepn(const xb_darcy_args *a, int32_t steps, void *stream) {
   if (a == nullptr) return fail(XB_EINVAL, "xb_darcy_stepn: null args");
   if (int r = check_dtype(a->dtype)) return r;
@@ -3870,6 +3935,8 @@ int xb_darcy_stepn(const xb_darcy_args *a, int32_t steps, void *stream) {
   return launch_tile_steps<float>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, a->rows, st);
 }
 
+#endif
+#if XB_IN_PART(5)
 int xb_elast_stepn(const xb_elast_args *a, int32_t steps, void *stream) {
   if (a == nullptr) return fail(XB_EINVAL, "xb_elast_stepn: null args");
   {
@@ -3896,6 +3963,8 @@ int xb_elast_stepn(const xb_elast_args *a, int32_t steps, void *stream) {
   return launch_tile_steps<float>(a->nx, a->ny, steps, c, a->f_in, a->f_out, a->bc, a->rows, st);
 }
 
+#endif
+#if XB_IN_PART(6)
 int xb_elast_smoothn(const xb_elast_args *a, int32_t steps, void *stream) {
   if (int r = check_smoothn(a, steps, "xb_elast_smoothn")) return r;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -3903,8 +3972,10 @@ int xb_elast_smoothn(const xb_elast_args *a, int32_t steps, void *stream) {
   return elast_smoothn_t<float>(a, steps, st);
 }
 
+#endif
 } // extern "C"
 
+#if XB_IN_PART(1)
 // ---------------------------------------------------------------------------
 // Allocation-free residual sweep.  The squared-norm partials of the residual sweep
 // (one per wavefront item) live in this file-scope device array instead of a
@@ -3969,3 +4040,4 @@ int xb_elast_residual_noalloc(const xb_elast_args *a, double *norm_out, void *st
 }
 
 } // extern "C"
+#endif // XB_IN_PART(1)
