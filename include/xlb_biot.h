@@ -90,7 +90,9 @@ typedef struct xb_elast_args {
   const void *resid;        /* optional (8, ix1-ix0, iy1-iy0): subtracted on box    */
   int32_t ix0, ix1, iy0, iy1; /* interior box (cells without multigrid BC layers) */
   const xb_bc_table *bc;    /* optional                                             */
-  void *coarse_out;         /* RESIDUAL: (8, (ix1-ix0)/2, (iy1-iy0)/2)              */
+  void *coarse_out;         /* RESIDUAL: (8, (ix1-ix0)/2, (iy1-iy0)/2) restriction out;
+                               xb_elast_smoothn (SMOOTH), optional: the next level's
+                               (8, nx-(ix1-ix0)/2, ny-(iy1-iy0)/2) correction, read   */
   double *norm_partials;    /* RESIDUAL: xb_elast_partials_count() doubles          */
   int32_t rows;             /* launch shape: rows per wavefront (power of 2 in
                                [1, 64]; >= 2 for RESIDUAL) or 0 for the built-in    */
@@ -157,7 +159,13 @@ int xb_elast_stepn(const xb_elast_args *args, int32_t steps, void *stream);
  * 32 x 32 LDS tiles (residual, external force, fused coupling force with div_out,
  * per-cell Lame fields as xb_elast_step; bit-identical to `steps` calls).  steps <= 6
  * (<= 3 with the coupling force, which reaches two cells per step).  A gradient wall
- * table of more than 32 slots: XB_EUNSUPPORTED (use the sweeps). */
+ * table of more than 32 slots: XB_EUNSUPPORTED (use the sweeps).
+ * With coarse_out set, the sweeps start from f_in plus the injection of that coarse
+ * correction over the interior box: the post-smoothing of a V-cycle with its
+ * prolong-add (reference src/elasticity.py:1290-1293, _prolong :878-906) fused in,
+ * bit-identical to xb_elast_prolong_add followed by this call; f_in is not written
+ * and the divergence of the corrected state is formed in the tile (div_in is then
+ * only read as the coupling flag). */
 int xb_elast_smoothn(const xb_elast_args *args, int32_t steps, void *stream);
 
 /* Number of doubles in norm_partials for a RESIDUAL call with these args. */

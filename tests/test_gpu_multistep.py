@@ -107,3 +107,26 @@ def test_coupled_smoothn_matches_sweeps(case, m, precision):
     b, db = el._smoothn_nl(f, div, level, r, base, nlf, m)
     assert torch.equal(a, b), (a - b).abs().max().item()
     assert torch.equal(da, db), (da - db).abs().max().item()
+
+
+@pytest.mark.parametrize("case", ["periodic_biot", "consolidation", "fields", "coarse"])
+@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (3, "f64/f64"), (2, "f32/f32")])
+def test_coupled_smoothn_with_coarse_correction_matches_prolong_then_sweeps(case, m, precision):
+    """The post-smoothing launch with the prolongation fused in (coarse_out read by
+    xb_elast_smoothn) against xb_elast_prolong_add_div followed by single sweeps;
+    f_in is left as it was."""
+    say("coupled smoothing on tiles from f + P c", case, m, precision)
+    el, nlf, level, f, r, base = _smooth_cases(case, precision)
+    g = torch.Generator().manual_seed(6)
+    nxc, nyc = el.grid_size(level + 1)
+    c = (1e-4 * torch.randn((8, nxc, nyc), generator=g, dtype=torch.float64)).to(f.dtype).cuda()
+    div = el._divergence_soa(f)
+    a, da = f.clone(), div.clone()
+    el._prolong_add_div(a, da, c, level)
+    for _ in range(m):
+        a, da = el._smooth_nl(a, da, level, r, base, nlf)
+    f0 = f.clone()
+    b, db = el._smoothn_nl(f, div, level, r, base, nlf, m, coarse=c)
+    assert torch.equal(f, f0)
+    assert torch.equal(a, b), (a - b).abs().max().item()
+    assert torch.equal(da, db), (da - db).abs().max().item()

@@ -1347,7 +1347,9 @@ template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st, i
 }
 
 // Argument checks of one elasticity sweep (xb_elast_step and every job of a batch).
-int validate_elast(const xb_elast_args *a, const char *who) {
+// coarse_add: the launch injects a coarse correction (coarse_out in SMOOTH mode), which
+// only the tile smoothing of xb_elast_smoothn does.
+int validate_elast(const xb_elast_args *a, const char *who, bool coarse_add = false) {
   if (a == nullptr) return fail(XB_EINVAL, "%s: null args", who);
   if (int r = check_dtype(a->dtype)) return r;
   if (a->nx <= 0 || a->ny <= 0) return fail(XB_EINVAL, "%s: bad grid %d x %d", who, a->nx, a->ny);
@@ -1364,6 +1366,9 @@ int validate_elast(const xb_elast_args *a, const char *who) {
     if (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1))
       return fail(XB_EINVAL, "%s: interior box must have even extents for restriction", who);
   }
+  if (a->mode != XB_MODE_RESIDUAL && a->coarse_out != nullptr && !coarse_add)
+    return fail(XB_EINVAL, "%s: coarse_out is written in residual mode; in smoothing mode only xb_elast_smoothn "
+                "reads it (as a coarse correction)", who);
   if (a->rows != 0 && (a->rows < 1 || a->rows > 64 || (a->rows & (a->rows - 1))))
     return fail(XB_EINVAL, "%s: rows must be 0 or a power of two in [1, 64]", who);
   if (a->mode == XB_MODE_RESIDUAL && a->rows == 1)
@@ -1615,6 +1620,11 @@ template <typename T> struct TileSmoothDev {
   const T *fin, *res;
   T *fout;
   BCDev<T> bc;
+  // optional coarse correction of the next level (cnx x cny, Nc cells per direction),
+  // injected into f_in over the interior box as it is loaded (k_prolong_add fused)
+  const T *coarse;
+  int cny;
+  size_t Nc;
 };
 
 template <typename T> struct TileGradRules {
@@ -1666,6 +1676,12 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
     const size_t cell = (size_t)gx * ny + gy;
 #pragma unroll
     for (int d = 0; d < Q; ++d) f[k][d] = A.fin[d * N + cell];
+    if (A.coarse != nullptr && gx >= A.ix0 && gx < A.ix1 && gy >= A.iy0 && gy < A.iy1) {
+      // the prolongation's f + c, the same sum as k_prolong_add
+      const size_t cc = (size_t)(A.ix0 + ((gx - A.ix0) >> 1)) * A.cny + (A.iy0 + ((gy - A.iy0) >> 1));
+#pragma unroll
+      for (int d = 0; d < Q; ++d) f[k][d] = f[k][d] + A.coarse[d * A.Nc + cc];
+    }
     aux[k] = A.coll.load_aux(cell);
     kd[k] = A.coll.kdiv(aux[k], A.nl.kdiv);
     slot[k] = (WALLS && A.bc.slot != nullptr) ? A.bc.slot[cell] : -1;
@@ -1877,6 +1893,9 @@ template <typename T> int make_tile_smooth(const xb_elast_args *a, int steps, Ti
   A.res = static_cast<const T *>(a->resid);
   A.fout = static_cast<T *>(a->f_out);
   A.bc = bcdev<T>(a->bc);
+  A.coarse = static_cast<const T *>(a->coarse_out);
+  A.cny = a->ny - (a->iy1 - a->iy0) / 2;
+  A.Nc = (size_t)(a->nx - (a->ix1 - a->ix0) / 2) * A.cny;
   const int R = a->div_in != nullptr ? 2 : 1;
   const int inner = TILE - 2 * R * steps;
   const int tx = cdiv(a->nx, inner), ty = cdiv(a->ny, inner);
@@ -1903,9 +1922,13 @@ template <typename T> int elast_smoothn_t(const xb_elast_args *a, int steps, hip
 
 
 int check_smoothn(const xb_elast_args *a, int steps, const char *who) {
-  if (int r = validate_elast(a, who)) return r;
+  if (int r = validate_elast(a, who, true)) return r;
   if (a->mode != XB_MODE_SMOOTH) return fail(XB_EINVAL, "%s: needs XB_MODE_SMOOTH", who);
   if (a->f_in == a->f_out) return fail(XB_EINVAL, "%s: f_in and f_out must differ", who);
+  if (a->coarse_out != nullptr && (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1)))
+    return fail(XB_EINVAL, "%s: a coarse correction needs an interior box of even sides", who);
+  if (a->coarse_out != nullptr && (a->coarse_out == a->f_out || a->coarse_out == a->f_in))
+    return fail(XB_EINVAL, "%s: the coarse correction must not alias f_in or f_out", who);
   const int R = a->div_in != nullptr ? 2 : 1;
   if (steps < 1 || TILE - 2 * R * steps < 8)
     return fail(XB_EINVAL, "%s: steps must be in [1, %d]", who, (TILE - 8) / (2 * R));

@@ -318,8 +318,9 @@ class ElasticityBatch:
             k = 1 if f is I.C[level][0] else 0
             return I.C[level][k], I.D[level][k]
 
-        def smooth(level, last_step, m=1):
-            # m steps in one launch: on LDS tiles (m > 1) or one sweep
+        def smooth(level, last_step, m=1, coarse=None):
+            # m steps in one launch: on LDS tiles (m > 1) or one sweep; coarse (tile
+            # launches only): instance -> the next level's correction, injected first
             jobs, ids = [], []
             for i, I in enumerate(self.inst):
                 if I.levels <= level:
@@ -329,6 +330,8 @@ class ElasticityBatch:
                 a = self._sweep_args(I, level, nat.XB_MODE_SMOOTH, f, of, d, od)
                 if m > 1:
                     a.rows = m
+                if coarse is not None and i in coarse:
+                    a.coarse_out = nat.ptr(coarse[i])
                 jobs.append(a)
                 ids.append(i)
                 cur[i, level] = (of, od)
@@ -336,13 +339,13 @@ class ElasticityBatch:
                 kind = nat.XB_BATCH_ELAST_SMOOTHN if m > 1 else nat.XB_BATCH_ELAST_STEP
                 plans.append(BatchPlan(kind, dt, jobs, ids, self.device, tag=("smooth", level, m)))
 
-        def smooth_phase(level, post):
+        def smooth_phase(level, post, coarse=None):
             m = self._tile_steps(level)
             done = 0
             while done < ks:
                 step = min(m, ks - done)
                 done += step
-                smooth(level, post and done == ks, step)
+                smooth(level, post and done == ks, step, coarse if done == step else None)
 
         norm_jobs = []
         for level in range(L):
@@ -370,6 +373,10 @@ class ElasticityBatch:
             plans.append(BatchPlan(nat.XB_BATCH_FILL, dt, fills, fill_ids, self.device, tag=("fill", level + 1)))
         for level in range(L - 1, -1, -1):
             deeper = [i for i, I in enumerate(self.inst) if I.levels - 1 > level]
+            if deeper and ks > 1 and self._tile_steps(level) > 1:
+                # the prolongation rides in the first post-smoothing tile launch
+                smooth_phase(level, True, {i: cur[i, level + 1][0] for i in deeper})
+                continue
             if deeper:
                 jobs = []
                 for i in deeper:

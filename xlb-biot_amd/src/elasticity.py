@@ -408,7 +408,8 @@ class LBMElasticity(object):
             ext = (torch.zeros((2, nx, ny), dtype=self._tdt, device=self.device)
                    if external_force and level == 0 else None)
             for mode in modes:
-                coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=self._tdt, device=self.device)
+                coarse = torch.empty((8, (x1 - x0) // 2, (y1 - y0) // 2), dtype=self._tdt, device=self.device) \
+                    if mode == nat.XB_MODE_RESIDUAL else None
                 a = self._launch(f, o, level, mode, ext=ext, resid=r, coarse=coarse)
                 a.rows = 2  # the most wavefronts, hence the most partial sums
                 ws = NormWorkspace(lib.xb_elast_partials_count(a), self.device)
@@ -734,26 +735,47 @@ class LBMElasticity(object):
                 return 1
         return min(m, 3 if nlf is not None else 6)
 
-    def _smoothn_nl(self, f, div, level, resid, base, nlf, m, out=None, out_div=None):
-        """``m`` coupled smoothing sweeps in one launch (``xb_elast_smoothn``)."""
+    def _smoothn_nl(self, f, div, level, resid, base, nlf, m, out=None, out_div=None, coarse=None):
+        """``m`` coupled smoothing sweeps in one launch (``xb_elast_smoothn``), from
+        ``f`` plus the injection of the next level's correction ``coarse`` if given."""
         out = torch.empty_like(f) if out is None else out
         out_div = torch.empty_like(div) if out_div is None else out_div
         a = self._nl_args(self._launch(f, out, level, nat.XB_MODE_SMOOTH, ext=base, resid=resid), div, out_div,
                           nlf, level)
+        if coarse is not None:
+            a.coarse_out = nat.ptr(coarse)
         nat.check(nat.lib().xb_elast_smoothn(a, int(m), nat.stream()), "xb_elast_smoothn")
         return out, out_div
 
-    def _smooth_seq_nl(self, f, div, level, residual, base, nlf, k, out=None, out_div=None):
+    def _prolong_add_div(self, f, div, f_c, level):
+        """``f += P f_c`` over the interior box in place, with ``div`` its divergence
+        (reference src/elasticity.py:1290-1293)."""
+        nxc, nyc = self.grid_size(level + 1)
+        nx, ny = self.grid_size(level)
+        x0, x1, y0, y1 = self.interior_box(level)
+        nat.check(nat.lib().xb_elast_prolong_add_div(nx, ny, self._code(f), self.lamda, self.mu,
+                                                     nat.ptr(f), nat.ptr(f_c), nxc, nyc, x0, x1, y0, y1,
+                                                     nat.ptr(div), *(self._fields() if level == 0 else (None, None)),
+                                                     nat.stream()), "xb_elast_prolong_add_div")
+
+    def _smooth_seq_nl(self, f, div, level, residual, base, nlf, k, out=None, out_div=None, coarse=None):
         """``k`` coupled smoothing sweeps: tile launches of the tuned number of steps,
-        the remainder as single sweeps; the last one writes ``out`` when given."""
+        the remainder as single sweeps; the last one writes ``out`` when given.
+        ``coarse``: the next level's correction, prolonged into ``f`` first - inside
+        the first tile launch when there is one (``f`` is then left as it was), else
+        by the prolongation kernel in place."""
         m = self._smooth_tile_steps(level, nlf)
+        if coarse is not None and not (m > 1 and k > 1):
+            self._prolong_add_div(f, div, coarse, level)
+            coarse = None
         done = 0
         while done < k:
             step = min(m, k - done) if m > 1 else 1
             last = done + step == k
             o, od = (out, out_div) if last else (None, None)
             if step > 1:
-                f, div = self._smoothn_nl(f, div, level, residual, base, nlf, step, o, od)
+                f, div = self._smoothn_nl(f, div, level, residual, base, nlf, step, o, od, coarse=coarse)
+                coarse = None
             else:
                 f, div = self._smooth_nl(f, div, level, residual, base, nlf, o, od)
             done += step
@@ -769,14 +791,9 @@ class LBMElasticity(object):
             f_c, norm = self._coarse_solve_nl(f, div, level, residual, base, nlf)
             if coarse is not None:
                 coarse.append(f_c)
-            nxc, nyc = self.grid_size(level + 1)
-            nx, ny = self.grid_size(level)
-            x0, x1, y0, y1 = self.interior_box(level)
-            nat.check(nat.lib().xb_elast_prolong_add_div(nx, ny, self._code(f), self.lamda, self.mu,
-                                                         nat.ptr(f), nat.ptr(f_c), nxc, nyc, x0, x1, y0, y1,
-                                                         nat.ptr(div), *(self._fields() if level == 0 else (None, None)),
-                                                         nat.stream()), "xb_elast_prolong_add_div")
-        f, div = self._smooth_seq_nl(f, div, level, residual, base, nlf, self.k_smooth, out, out_div)
+        else:
+            f_c = None
+        f, div = self._smooth_seq_nl(f, div, level, residual, base, nlf, self.k_smooth, out, out_div, coarse=f_c)
         if level == 0:
             return f, div, norm
         return f, div
