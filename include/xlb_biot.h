@@ -331,6 +331,22 @@ int xb_elast_mg_adjoint(const xb_elast_mg_adjoint_args *args, void *stream);
 int xb_elast_smooth_adjoint_small(const xb_elast_mg_adjoint_args *args, int32_t sweeps, double scale,
                                   const xb_bc_table *grad_bc, void *tmp, void *stream);
 
+/* `steps` (1..3) consecutive XB_ADJ_SMOOTH transposed sweeps of one level on 32 x 32
+ * LDS tiles, each followed by xb_elast_coupling_adjoint of its force adjoint (scale,
+ * grad_bc, the args' lam / mu and fields): bit-identical to that sequence of
+ * 2 * steps calls (the abar_out of a sweep the abar_in of the next), with abar_in read
+ * and abar_out written once.  rbar (optional) accumulates the residual's adjoint;
+ * with grad_lam / grad_mu (finest level: the Lame fields, no residual) `states` is a
+ * host array of the sweeps' recorded inputs f (device pointers, in the order the
+ * sweeps are transposed) and the parameter derivatives of the collision and of the
+ * divergence accumulate; base_bar (optional, (2, nx, ny) FP64) += each force adjoint.
+ * No abar_add; force_bar must be null (the force adjoint stays in the tile).  Replaces
+ * the transposed smoothing of the reference's V-cycle (src/elasticity.py:1258-1267,
+ * :1296-1305, differentiated by jax.grad there). */
+int xb_elast_smooth_adjoint_tiles(const xb_elast_mg_adjoint_args *args, int32_t steps, double scale,
+                                  const xb_bc_table *grad_bc, const void *const *states, double *base_bar,
+                                  void *stream);
+
 /* Transposed Biot coupling force F = G_scale(D f) of one sweep (the fused force of
  * xb_elast_args.div_in): abar[d] += k w2_d (G^T force_bar) with w2 = 1,1,1,1,2,2,2,2
  * and k the divergence coefficient; with f_in and the fields, the Lame-parameter

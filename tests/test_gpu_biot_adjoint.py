@@ -247,3 +247,35 @@ def test_small_grid_transposed_smoothing_matches_sweeps(case, precision, level):
         b = s._smooth_adjoint_small(level, ab, sweeps, rb2)
         assert torch.equal(a, b), (sweeps, (a - b).abs().max().item())
         assert torch.equal(rb1, rb2), sweeps
+
+
+@pytest.mark.parametrize("precision", ["f64/f64", "f32/f32"])
+def test_tiled_transposed_smoothing_matches_sweep_pairs(precision):
+    """The gradient with the transposed smoothing on LDS tiles (2 or 3 sweeps per
+    launch, xb_elast_smooth_adjoint_tiles) equals, bit for bit, the gradient with one
+    xb_elast_mg_adjoint + xb_elast_coupling_adjoint pair per sweep: consolidation
+    walls, Darcy wall rules of the coupling gradient, Lame fields, parameter
+    derivatives on the finest level, residual adjoints on a coarse level of 32 x 34."""
+    from src.biot_adjoint import biot_solve
+    nx, levels, k, n_inner, steps = 64, 3, 4, 2, 2
+    ny = nx + 2
+    dt = torch.float64 if precision == "f64/f64" else torch.float32
+    lam, mu, om = _fields(nx, ny, 7)
+    te, tp = _misfit_targets(nx, ny, 8)
+    grads, Js = {}, {}
+    for m in (0, 2, 3):
+        say("transposed smoothing sweeps per launch", m, precision)
+        p = P.biot_consolidation("gpu", nx=nx, levels=levels, steps=n_inner, k_smooth=k, precision=precision)
+        p.adjoint_tile_steps = m
+        ts = [torch.tensor(v, dtype=dt, device="cuda", requires_grad=True) for v in (om, lam, mu)]
+        eta, pr = biot_solve(p, *ts, steps, checkpoint_every=2)
+        J = 0.5 * ((eta - te.to(eta)) ** 2).sum(dtype=torch.float64) + \
+            0.5 * ((pr - tp.to(pr)) ** 2).sum(dtype=torch.float64)
+        J.backward()
+        Js[m] = J.item()
+        grads[m] = [t.grad.clone() for t in ts]
+        say("J", Js[m], "grad norms", [g.double().norm().item() for g in grads[m]])
+    for m in (2, 3):
+        assert Js[m] == Js[0]
+        for name, a, b in zip(("omega", "lambda", "mu"), grads[0], grads[m]):
+            assert torch.equal(a, b), (name, m, (a - b).abs().max().item())

@@ -2955,6 +2955,320 @@ int smooth_adjoint_small_t(const xb_elast_mg_adjoint_args *a, int sweeps, double
 }
 
 // ---------------------------------------------------------------------------
+// Transposed smoothing on LDS tiles: `steps` XB_ADJ_SMOOTH transposed sweeps of one
+// level in one launch, each followed by the transposed coupling force of its force
+// adjoint (the arithmetic of one k_elast_mg_adjoint<SMOOTH> launch and one
+// k_grad_adjoint<COUPLE> launch per sweep, in the same order, so bit-identical to
+// that sequence).  A 32 x 32 region per workgroup: the transposed pull reads the
+// neighbours' vbar at distance one and the transposed gradient their force adjoint,
+// so the exact core shrinks by two cells per side and sweep.  Per sweep:
+//   A  vbar = gamma abar to LDS, zeroed in the directions the cell's own
+//      post-streaming rule replaces (the transposed pull then skips them);
+//   B  gather pb (transposed pull + the rule's own-cell terms), transposed
+//      post-collision rules and collision; o = yv + (1 - gamma) abar; the residual
+//      adjoint (rbar -= yv, output cells, global); the force adjoint Fbar to LDS; the
+//      Lame-parameter derivative at the sweep's recorded state (finest level);
+//   C  qbar = sum of the transposed gradient slots over the neighbours' Fbar (minus
+//      the slots their gradient rules replace, plus the cell's own rule terms);
+//      abar = o + k w2 qbar; the divergence's parameter derivative; base_bar += Fbar.
+// The FP64 accumulators of the output cells stay in registers over the launch.
+constexpr int ADJ_TILE_MAX_STEPS = 3;
+// one cell per thread: the transposed sweep holds more per cell than the forward one
+constexpr int ADJ_TILE_THREADS = TILE * TILE;
+constexpr int ADJ_CPT = 1;
+
+template <typename T> struct TileAdjDev {
+  int nx, ny, steps, tiles_y, ix0, ix1, iy0, iy1;
+  T lam, mu, gamma, m3s, kdiv;
+  const T *lamf, *muf;
+  const T *ab_in;
+  T *ab_out, *rbar;
+  const T *state[ADJ_TILE_MAX_STEPS];
+  double *g_lam, *g_mu, *base_bar;
+  BCDev<T> bc, gbc;
+};
+
+template <typename T, bool HAS_BC, bool HAS_GBC, bool GRADS>
+__global__ void __launch_bounds__(ADJ_TILE_THREADS) k_tile_smooth_adjoint(const TileAdjDev<T> A) {
+  using C = ElastColl<T>;
+  using D = DarcyColl<T>;
+  constexpr int Q = 8;
+  constexpr int PAD = TILE_LDS_PAD(TILE);
+  constexpr int TP = TILE * TILE + 2 * PAD; // one LDS plane with its padding
+  constexpr int R = 2;
+  __shared__ T lv_raw[Q * TP];
+  __shared__ T lf_raw[2 * TP];
+  __shared__ int lgs_raw[HAS_GBC ? TP : 1];
+  const int tile = xcd_block_id();
+  const int nx = A.nx, ny = A.ny, steps = A.steps;
+  const int inner = TILE - 2 * R * steps;
+  const int bx = tile / A.tiles_y, by = tile % A.tiles_y;
+  const int x0 = bx * inner - R * steps, y0 = by * inner - R * steps;
+  const size_t N = (size_t)nx * ny;
+  const int nyi = A.iy1 - A.iy0;
+  const size_t Ni = (size_t)(A.ix1 - A.ix0) * nyi;
+  const int h = R * steps;
+  T ab[ADJ_CPT][Q];
+  T lam[ADJ_CPT], mu[ADJ_CPT];
+  int slot[ADJ_CPT], gslot[ADJ_CPT], ri[ADJ_CPT];
+  size_t cell[ADJ_CPT];
+  bool out[ADJ_CPT];
+  double gl[ADJ_CPT], gm[ADJ_CPT], bb0[ADJ_CPT], bb1[ADJ_CPT];
+  if constexpr (HAS_GBC) {
+    // the padding around the slot plane reads as "no rule": the gathers of the tile's
+    // border cells reach into it (their results are discarded, but a slot read there
+    // indexes the rule table)
+    for (int i = threadIdx.x; i < PAD; i += ADJ_TILE_THREADS) {
+      lgs_raw[i] = -1;
+      lgs_raw[PAD + TILE * TILE + i] = -1;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ADJ_CPT; ++k) {
+    const int c = threadIdx.x + k * ADJ_TILE_THREADS;
+    const int rx = c / TILE, ry = c % TILE;
+    const int gx = wrapm(x0 + rx, nx), gy = wrapm(y0 + ry, ny);
+    cell[k] = (size_t)gx * ny + gy;
+    out[k] = rx >= h && rx < TILE - h && ry >= h && ry < TILE - h && x0 + rx < nx && y0 + ry < ny;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) ab[k][d] = A.ab_in[d * N + cell[k]];
+    lam[k] = A.lamf != nullptr ? A.lamf[cell[k]] : A.lam;
+    mu[k] = A.muf != nullptr ? A.muf[cell[k]] : A.mu;
+    slot[k] = HAS_BC ? A.bc.slot[cell[k]] : -1;
+    gslot[k] = HAS_GBC ? A.gbc.slot[cell[k]] : -1;
+    if constexpr (HAS_GBC) lgs_raw[PAD + c] = gslot[k];
+    ri[k] = -1;
+    if (A.rbar != nullptr && out[k] && gx >= A.ix0 && gx < A.ix1 && gy >= A.iy0 && gy < A.iy1)
+      ri[k] = (gx - A.ix0) * nyi + (gy - A.iy0);
+    gl[k] = gm[k] = bb0[k] = bb1[k] = 0.0;
+    if (out[k]) {
+      if (GRADS) {
+        gl[k] = A.g_lam[cell[k]];
+        gm[k] = A.g_mu[cell[k]];
+      }
+      if (A.base_bar != nullptr) {
+        bb0[k] = A.base_bar[cell[k]];
+        bb1[k] = A.base_bar[N + cell[k]];
+      }
+    }
+  }
+  T *lv = lv_raw + PAD;
+  T *lf = lf_raw + PAD;
+  const int *lgs = lgs_raw + PAD;
+  for (int s = 0; s < steps; ++s) {
+    // A: vbar to LDS, without the directions the cell's own post-streaming rule replaces
+#pragma unroll
+    for (int k = 0; k < ADJ_CPT; ++k) {
+      const int c = threadIdx.x + k * ADJ_TILE_THREADS;
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        T v = A.gamma * ab[k][d];
+        if constexpr (HAS_BC) {
+          if (slot[k] >= 0 && A.bc.code_ps != nullptr && A.bc.code_ps[(size_t)slot[k] * Q + d] != 0) v = T(0);
+        }
+        lv[d * TP + c] = v;
+      }
+    }
+    __syncthreads();
+    // B: transposed pull, rules and collision; the force adjoint to LDS
+    T msv[ADJ_CPT];
+#pragma unroll
+    for (int k = 0; k < ADJ_CPT; ++k) {
+      const int c = threadIdx.x + k * ADJ_TILE_THREADS;
+      T pb[Q];
+#pragma unroll
+      for (int d = 0; d < Q; ++d) pb[d] = lv[d * TP + c + (C::cx(d) * TILE + C::cy(d))];
+      T ob[Q], fpre[Q];
+      if constexpr (HAS_BC) {
+        if (slot[k] >= 0) {
+          if (A.bc.code_ps != nullptr) {
+#pragma unroll
+            for (int d = 0; d < Q; ++d) {
+              const int cc = A.bc.code_ps[(size_t)slot[k] * Q + d];
+              if (cc != 0 && cc != XB_RULE_ZERO) {
+                const T a = A.gamma * ab[k][d];
+                const int kk = (cc > 0 ? cc : -cc) - 1;
+#pragma unroll
+                for (int e = 0; e < Q; ++e)
+                  if (e == kk) pb[e] = cc > 0 ? pb[e] + a : pb[e] - a;
+              }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        ob[d] = pb[d];
+        fpre[d] = T(0);
+      }
+      if constexpr (HAS_BC) {
+        if (slot[k] >= 0 && A.bc.code_pc != nullptr) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) {
+            const int cc = A.bc.code_pc[(size_t)slot[k] * Q + d];
+            if (cc != 0) {
+              ob[d] = T(0);
+              if (cc != XB_RULE_ZERO) {
+                const int kk = (cc > 0 ? cc : -cc) - 1;
+#pragma unroll
+                for (int e = 0; e < Q; ++e)
+                  if (e == kk) fpre[e] = cc > 0 ? fpre[e] + pb[d] : fpre[e] - pb[d];
+              }
+            }
+          }
+        }
+      }
+      T cs, cd, hh, nn;
+      elast_coef<T>(lam[k], mu[k], cs, cd, hh, nn);
+      T yv[Q];
+      const ElastMomBar<T> M = elast_collision_adjoint<T>(ob, fpre, cs, cd, hh, nn, yv);
+#pragma unroll
+      for (int d = 0; d < Q; ++d) ab[k][d] = yv[d] + (T(1) - A.gamma) * ab[k][d];
+      if (ri[k] >= 0) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) A.rbar[d * Ni + ri[k]] = A.rbar[d * Ni + ri[k]] - yv[d];
+      }
+      const T fb0 = M.B0 + div_const(M.B5, T(6), T(1) / T(6));
+      const T fb1 = M.B1 + div_const(M.B6, T(6), T(1) / T(6));
+      lf[c] = fb0;
+      lf[TP + c] = fb1;
+      msv[k] = T(0);
+      if constexpr (GRADS) {
+        if (out[k]) {
+          T f[Q];
+#pragma unroll
+          for (int d = 0; d < Q; ++d) f[d] = A.state[s][d * N + cell[k]];
+          double a, b;
+          elast_param_bar<T>(M, f, lam[k], mu[k], cs, a, b);
+          gl[k] += a;
+          gm[k] += b;
+          T ms = f[0] + f[1] + f[2] + f[3];
+          msv[k] = ms + T(2) * (f[4] + f[5] + f[6] + f[7]);
+        }
+      }
+    }
+    __syncthreads();
+    // C: transposed coupling force
+#pragma unroll
+    for (int k = 0; k < ADJ_CPT; ++k) {
+      const int c = threadIdx.x + k * ADJ_TILE_THREADS;
+      T pb[9];
+#pragma unroll
+      for (int d = 0; d < 9; ++d) {
+        const int j = c + (D::cx(d) * TILE + D::cy(d));
+        const T cw = A.m3s * T(D::wd(d));
+        T v = T(0);
+        if (D::cx(d) != 0) v = v + (D::cx(d) > 0 ? cw : -cw) * lf[j];
+        if (D::cy(d) != 0) v = v + (D::cy(d) > 0 ? cw : -cw) * lf[TP + j];
+        if constexpr (HAS_GBC) {
+          const int sn = lgs[j];
+          if (sn >= 0 && A.gbc.code_ps != nullptr && A.gbc.code_ps[(size_t)sn * 9 + d] != 0) v = T(0);
+        }
+        pb[d] = v;
+      }
+      if constexpr (HAS_GBC) {
+        if (gslot[k] >= 0 && A.gbc.code_ps != nullptr) {
+#pragma unroll
+          for (int d = 0; d < 9; ++d) {
+            const int cc = A.gbc.code_ps[(size_t)gslot[k] * 9 + d];
+            if (cc != 0 && cc != XB_RULE_ZERO) {
+              const T cw = A.m3s * T(D::wd(d));
+              T a = T(0);
+              if (D::cx(d) != 0) a = a + (D::cx(d) > 0 ? cw : -cw) * lf[c];
+              if (D::cy(d) != 0) a = a + (D::cy(d) > 0 ? cw : -cw) * lf[TP + c];
+              const int kk = (cc > 0 ? cc : -cc) - 1;
+#pragma unroll
+              for (int e = 0; e < 9; ++e)
+                if (e == kk) pb[e] = cc > 0 ? pb[e] + a : pb[e] - a;
+            }
+          }
+        }
+      }
+      T q = pb[0];
+#pragma unroll
+      for (int d = 1; d < 9; ++d) q = q + pb[d];
+      T kk = A.kdiv;
+      double dk = 0.0;
+      if (A.lamf != nullptr) {
+        const T lm = lam[k] + mu[k];
+        kk = elast_kdiv<T>(lam[k], mu[k]);
+        const double l3 = 3.0 * (double)lm + 1.0;
+        dk = 4.5 / (l3 * l3);
+      }
+      const T kq = kk * q;
+#pragma unroll
+      for (int d = 0; d < Q; ++d) ab[k][d] = ab[k][d] + (d < 4 ? kq : T(2) * kq);
+      if (out[k]) {
+        if constexpr (GRADS) {
+          const double g = (double)q * (double)msv[k] * dk;
+          gl[k] += g;
+          gm[k] += g;
+        }
+        bb0[k] += (double)lf[c];
+        bb1[k] += (double)lf[TP + c];
+      }
+    }
+    // no barrier here: the next sweep's A writes lv, whose last reads (B) precede the
+    // barrier before C, and its B writes lf only after the barrier that follows A,
+    // which every thread reaches after its reads of lf in C
+  }
+#pragma unroll
+  for (int k = 0; k < ADJ_CPT; ++k) {
+    if (!out[k]) continue;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) A.ab_out[d * N + cell[k]] = ab[k][d];
+    if (GRADS) {
+      A.g_lam[cell[k]] = gl[k];
+      A.g_mu[cell[k]] = gm[k];
+    }
+    if (A.base_bar != nullptr) {
+      A.base_bar[cell[k]] = bb0[k];
+      A.base_bar[N + cell[k]] = bb1[k];
+    }
+  }
+}
+
+template <typename T>
+int smooth_adjoint_tiles_t(const xb_elast_mg_adjoint_args *a, int steps, double scale, const xb_bc_table *gbc,
+                           const void *const *states, double *base_bar, hipStream_t st) {
+  TileAdjDev<T> A{};
+  A.nx = a->nx; A.ny = a->ny; A.steps = steps;
+  A.ix0 = a->ix0; A.ix1 = a->ix1; A.iy0 = a->iy0; A.iy1 = a->iy1;
+  A.lam = (T)a->lam; A.mu = (T)a->mu; A.gamma = (T)a->gamma;
+  A.m3s = T(-3 * scale);
+  A.kdiv = T(-1.5 / (3 * (a->lam + a->mu) + 1));
+  A.lamf = static_cast<const T *>(a->lam_field);
+  A.muf = static_cast<const T *>(a->mu_field);
+  A.ab_in = static_cast<const T *>(a->abar_in);
+  A.ab_out = static_cast<T *>(a->abar_out);
+  A.rbar = static_cast<T *>(a->rbar);
+  for (int s = 0; s < steps; ++s) A.state[s] = states != nullptr ? static_cast<const T *>(states[s]) : nullptr;
+  A.g_lam = a->grad_lam;
+  A.g_mu = a->grad_mu;
+  A.base_bar = base_bar;
+  A.bc = bcdev<T>(a->bc);
+  A.gbc = bcdev<T>(gbc);
+  const int inner = TILE - 4 * steps;
+  A.tiles_y = cdiv(a->ny, inner);
+  const dim3 gr((unsigned)(cdiv(a->nx, inner) * A.tiles_y)), bl(ADJ_TILE_THREADS);
+  const bool hb = A.bc.slot != nullptr, hg = A.gbc.slot != nullptr, gr_ = a->grad_lam != nullptr;
+#define XB_ADJ_TILE(HB, HG, GR) hipLaunchKernelGGL((k_tile_smooth_adjoint<T, HB, HG, GR>), gr, bl, 0, st, A)
+  if (gr_) {
+    if (hb && hg) XB_ADJ_TILE(true, true, true);
+    else if (hb) XB_ADJ_TILE(true, false, true);
+    else if (hg) XB_ADJ_TILE(false, true, true);
+    else XB_ADJ_TILE(false, false, true);
+  } else {
+    if (hb && hg) XB_ADJ_TILE(true, true, false);
+    else if (hb) XB_ADJ_TILE(true, false, false);
+    else if (hg) XB_ADJ_TILE(false, true, false);
+    else XB_ADJ_TILE(false, false, false);
+  }
+#undef XB_ADJ_TILE
+  return check_launch("xb_elast_smooth_adjoint_tiles");
+}
+
+// ---------------------------------------------------------------------------
 // Coarse tail of a V-cycle in one workgroup.  Below a few thousand cells a level's
 // sweep is bound by launch and load latency (a wavefront walks its rows one after
 // another), so the levels L0.. of a V-cycle run as phases of one block separated by
@@ -3709,6 +4023,42 @@ int xb_elast_smooth_adjoint_small(const xb_elast_mg_adjoint_args *a, int32_t swe
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (a->dtype == XB_F64) return smooth_adjoint_small_t<double>(a, sweeps, scale, grad_bc, tmp, st);
   return smooth_adjoint_small_t<float>(a, sweeps, scale, grad_bc, tmp, st);
+}
+
+int xb_elast_smooth_adjoint_tiles(const xb_elast_mg_adjoint_args *a, int32_t steps, double scale,
+                                  const xb_bc_table *grad_bc, const void *const *states, double *base_bar,
+                                  void *stream) {
+  const char *who = "xb_elast_smooth_adjoint_tiles";
+  if (a == nullptr) return fail(XB_EINVAL, "%s: null args", who);
+  if (int r = check_dtype(a->dtype)) return r;
+  if (a->mode != XB_ADJ_SMOOTH) return fail(XB_EINVAL, "%s: needs XB_ADJ_SMOOTH", who);
+  if (a->nx <= 0 || a->ny <= 0) return fail(XB_EINVAL, "%s: bad grid %d x %d", who, a->nx, a->ny);
+  if (steps < 1 || steps > ADJ_TILE_MAX_STEPS)
+    return fail(XB_EINVAL, "%s: steps must be in [1, %d]", who, ADJ_TILE_MAX_STEPS);
+  if (a->abar_in == nullptr || a->abar_out == nullptr || a->abar_in == a->abar_out)
+    return fail(XB_EINVAL, "%s: abar_in and abar_out must be given and differ", who);
+  if (a->abar_add != nullptr || a->force_bar != nullptr)
+    return fail(XB_EUNSUPPORTED, "%s: no abar_add; the force adjoint stays in the tile (force_bar null)", who);
+  if (a->ix0 < 0 || a->ix1 > a->nx || a->iy0 < 0 || a->iy1 > a->ny || a->ix0 >= a->ix1 || a->iy0 >= a->iy1)
+    return fail(XB_EINVAL, "%s: bad interior box", who);
+  if ((a->lam_field == nullptr) != (a->mu_field == nullptr))
+    return fail(XB_EINVAL, "%s: lam_field and mu_field go together", who);
+  if ((a->grad_lam == nullptr) != (a->grad_mu == nullptr))
+    return fail(XB_EINVAL, "%s: grad_lam and grad_mu go together", who);
+  if (a->grad_lam != nullptr) {
+    if (a->lam_field == nullptr || states == nullptr || a->resid != nullptr)
+      return fail(XB_EINVAL, "%s: parameter gradients need the Lame fields and the sweeps' states (and no "
+                  "residual)", who);
+    for (int s = 0; s < steps; ++s)
+      if (states[s] == nullptr) return fail(XB_EINVAL, "%s: null state of sweep %d", who, s);
+  }
+  if (a->rbar != nullptr && (a->rbar == a->abar_in || a->rbar == a->abar_out))
+    return fail(XB_EINVAL, "%s: rbar must not alias abar_in or abar_out", who);
+  if (int r = check_bc(a->bc, false)) return r;
+  if (int r = check_bc(grad_bc, true)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (a->dtype == XB_F64) return smooth_adjoint_tiles_t<double>(a, steps, scale, grad_bc, states, base_bar, st);
+  return smooth_adjoint_tiles_t<float>(a, steps, scale, grad_bc, states, base_bar, st);
 }
 
 int xb_elast_coupling_adjoint(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, const void *lam_field,

@@ -149,6 +149,9 @@ SIGNATURES = {
     "xb_vcycle_tail": (c_int32, [ctypes.POINTER(TailPlan), c_void_p]),
     "xb_elast_smooth_adjoint_small": (c_int32, [ctypes.POINTER(ElastMGAdjointArgs), c_int32, c_double,
                                                 ctypes.POINTER(BCTable), c_void_p, c_void_p]),
+    "xb_elast_smooth_adjoint_tiles": (c_int32, [ctypes.POINTER(ElastMGAdjointArgs), c_int32, c_double,
+                                                ctypes.POINTER(BCTable), ctypes.POINTER(c_void_p), c_void_p,
+                                                c_void_p]),
     "xb_elast_coupling_adjoint": (c_int32, [c_int32, c_int32, c_int32, c_double, c_double, c_void_p, c_void_p,
                                             c_void_p, c_double, ctypes.POINTER(BCTable), c_void_p, c_void_p,
                                             c_void_p, c_void_p, c_void_p, c_void_p]),

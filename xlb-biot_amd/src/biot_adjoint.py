@@ -32,6 +32,8 @@ Parameter derivatives and the adjoint of each step's static force accumulate in
 FP64.  Coarse multigrid levels use the scalar lambda, mu (they carry no parameter
 derivative); the initial state is taken as given (not differentiated).
 """
+import ctypes
+
 import torch
 
 from . import _native as nat
@@ -83,6 +85,9 @@ class BiotAdjointSession:
         # coarse grids of at most this many cells run their transposed smoothing in one
         # workgroup (one launch per run of sweeps instead of two per sweep)
         self.adjoint_small_cells = 600
+        # transposed smoothing sweeps per launch on LDS tiles (0: one launch pair per
+        # sweep, xb_elast_mg_adjoint + xb_elast_coupling_adjoint)
+        self.adjoint_tile_steps = int(getattr(p, "adjoint_tile_steps", 2))
         self._coarse_adjoint_graph()
 
     def _coarse_adjoint_graph(self):
@@ -212,6 +217,45 @@ class BiotAdjointSession:
             "xb_elast_coupling_adjoint")
         return out
 
+    def _smooth_adjoint_run(self, level, ab, states, rbar, acc):
+        """The level's ``k`` transposed smoothing sweeps, last sweep first (``states``:
+        their recorded inputs, or ``None`` entries): on LDS tiles, up to
+        ``adjoint_tile_steps`` sweeps per launch (``xb_elast_smooth_adjoint_tiles``),
+        or a launch pair per sweep."""
+        el = self.el
+        S = nat.XB_ADJ_SMOOTH
+        m = int(self.adjoint_tile_steps)
+        j = el.k_smooth - 1
+        if m < 1:
+            while j >= 0:
+                ab = self._sweep_adjoint(level, S, ab, None, states[j], rbar, acc)
+                j -= 1
+            return ab
+        nx, ny = el.grid_size(level)
+        x0, x1, y0, y1 = el.interior_box(level)
+        fields = el._fields() if level == 0 else (None, None)
+        table = el._tables.get(level)
+        gt = self.da._grad_table(nx, ny)
+        base = acc.get("base") if level == 0 else None
+        while j >= 0:
+            step = min(m, j + 1)
+            recs = [states[j - s] for s in range(step)]
+            grads = level == 0 and recs[0] is not None and acc.get("lam") is not None
+            out = torch.empty((8, nx, ny), dtype=self.dtype, device=self.device)
+            a = nat.ElastMGAdjointArgs(nx, ny, nat.dtype_code(self.dtype), S, el.lamda, el.mu, el.gamma,
+                                       fields[0], fields[1], x0, x1, y0, y1, nat.ptr(ab), None, nat.ptr(out),
+                                       nat.ptr(rbar), None, None, None,
+                                       nat.ptr(acc["lam"]) if grads else None, nat.ptr(acc["mu"]) if grads else None,
+                                       table.ref() if table is not None else None)
+            ptrs = (ctypes.c_void_p * step)(*[nat.ptr(r) for r in recs]) if grads else None
+            nat.check(nat.lib().xb_elast_smooth_adjoint_tiles(a, step, float(self.nlf.scale),
+                                                              gt.ref() if gt is not None else None, ptrs,
+                                                              nat.ptr(base), nat.stream()),
+                      "xb_elast_smooth_adjoint_tiles")
+            ab = out
+            j -= step
+        return ab
+
     def _level_adjoint(self, level, ab, states, rbar, acc):
         """Adjoint of one V-cycle from ``level`` down: ``ab`` is the adjoint of its
         output, ``states`` the finest-level sweep inputs (level 0) and ``rbar`` the
@@ -225,8 +269,7 @@ class BiotAdjointSession:
         if small:
             ab = self._smooth_adjoint_small(level, ab, k, rbar)
         else:
-            for j in range(k - 1, -1, -1):
-                ab = self._sweep_adjoint(level, S, ab, None, post[j], rbar, acc)
+            ab = self._smooth_adjoint_run(level, ab, post, rbar, acc)
         if level < el.levels - 1:
             nx, ny = el.grid_size(level)
             nxc, nyc = el.grid_size(level + 1)
@@ -247,9 +290,7 @@ class BiotAdjointSession:
         pre = states[:k] if states is not None else [None] * k
         if small:
             return self._smooth_adjoint_small(level, ab, k, rbar)
-        for j in range(k - 1, -1, -1):
-            ab = self._sweep_adjoint(level, S, ab, None, pre[j], rbar, acc)
-        return ab
+        return self._smooth_adjoint_run(level, ab, pre, rbar, acc)
 
     def _smooth_adjoint_small(self, level, ab_in, sweeps, rbar):
         """``sweeps`` transposed smoothing sweeps of a coarse level (no recorded
