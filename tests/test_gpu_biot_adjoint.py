@@ -222,8 +222,9 @@ def test_c4_geometry_gradient_matches_finite_differences_at_128():
 @pytest.mark.parametrize("level", [1, 2])
 def test_small_grid_transposed_smoothing_matches_sweeps(case, precision, level):
     """xb_elast_smooth_adjoint_small (a run of transposed smoothing sweeps with their
-    transposed coupling force in one workgroup) is bitwise the sequence of
-    xb_elast_mg_adjoint and xb_elast_coupling_adjoint calls it replaces."""
+    transposed coupling force in one workgroup) and xb_elast_smooth_adjoint_tiles
+    (1, 2 or 3 of them per launch on LDS tiles) are bitwise the sequence of
+    xb_elast_mg_adjoint and xb_elast_coupling_adjoint calls they replace."""
     from src import _native as nat
     from src.biot_adjoint import BiotAdjointSession
     say("small-grid transposed smoothing", case, precision, "level", level)
@@ -247,6 +248,15 @@ def test_small_grid_transposed_smoothing_matches_sweeps(case, precision, level):
         b = s._smooth_adjoint_small(level, ab, sweeps, rb2)
         assert torch.equal(a, b), (sweeps, (a - b).abs().max().item())
         assert torch.equal(rb1, rb2), sweeps
+        if sweeps == el.k_smooth:
+            # the same run on LDS tiles (xb_elast_smooth_adjoint_tiles), 1 to 3 sweeps
+            # per launch
+            for m in (1, 2, 3):
+                s.adjoint_tile_steps = m
+                rb3 = rbar0.clone()
+                c = s._smooth_adjoint_run(level, ab, [None] * sweeps, rb3, no_acc)
+                assert torch.equal(a, c), (m, (a - c).abs().max().item())
+                assert torch.equal(rb1, rb3), m
 
 
 @pytest.mark.parametrize("precision", ["f64/f64", "f32/f32"])
