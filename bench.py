@@ -336,11 +336,12 @@ def vcycle_bytes(n, levels, k, S=8, m=1):
     the divergence (20 S per cell); a tile launch of m steps forms the divergence
     itself (19 S per launch); coarse levels also read the restricted residual (26 S,
     25 S).  The residual sweep reads f, force and divergence and writes the
-    restriction (11 S + 8 S / 4; coarse levels 17 S + 8 S / 4); coarse states are
-    zero-filled (9 S) and prolonged: with tile launches (m > 1, k > 1) inside the
-    first post-smoothing launch, which then also reads the coarse correction (8 S / 4
-    per fine cell), else by a kernel of its own (8 S / 4 read coarse, 16 S + 1 S per
-    fine cell)."""
+    restriction (11 S + 8 S / 4; coarse levels 17 S + 8 S / 4).  With tile launches
+    (m > 1, k > 1) a coarse level's first launch starts from the zero state without
+    reading one (8 S less), and the prolongation rides in the first post-smoothing
+    launch, which then also reads the coarse correction (8 S / 4 per fine cell);
+    otherwise coarse states are zero-filled (9 S) and prolonged by a kernel of their
+    own (8 S / 4 read coarse, 16 S + 1 S per fine cell)."""
     total = 0.0
     launches = 2 * (-(-k // m))
     for lvl in range(levels):
@@ -350,11 +351,15 @@ def vcycle_bytes(n, levels, k, S=8, m=1):
             smooth = 20 if lvl == 0 else 26
         else:
             smooth = 19 if lvl == 0 else 25
+        fused = m > 1 and k > 1
         total += launches * smooth * c
+        if lvl > 0 and fused:
+            total -= 8 * c                                 # the zero start state is not read
         if not coarsest:
             total += ((11 if lvl == 0 else 17) + 2) * c   # residual + restriction out
-            total += 9 * c / 4                             # zero fill of the coarse state
-            total += (2 if m > 1 and k > 1 else 2 + 17) * c  # prolongation
+            if not fused:
+                total += 9 * c / 4                         # zero fill of the coarse state
+            total += (2 if fused else 2 + 17) * c          # prolongation
     return total * S
 
 

@@ -165,7 +165,9 @@ int xb_elast_stepn(const xb_elast_args *args, int32_t steps, void *stream);
  * prolong-add (reference src/elasticity.py:1290-1293, _prolong :878-906) fused in,
  * bit-identical to xb_elast_prolong_add followed by this call; f_in is not written
  * and the divergence of the corrected state is formed in the tile (div_in is then
- * only read as the coupling flag). */
+ * only read as the coupling flag).  A null f_in is the zero state (the first sweeps
+ * of a coarse level's V-cycle, reference :1288-1289): nothing is read for it, and a
+ * non-null div_in (not read) still marks the coupling force. */
 int xb_elast_smoothn(const xb_elast_args *args, int32_t steps, void *stream);
 
 /* Number of doubles in norm_partials for a RESIDUAL call with these args. */

@@ -1347,13 +1347,13 @@ template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st, i
 }
 
 // Argument checks of one elasticity sweep (xb_elast_step and every job of a batch).
-// coarse_add: the launch injects a coarse correction (coarse_out in SMOOTH mode), which
-// only the tile smoothing of xb_elast_smoothn does.
-int validate_elast(const xb_elast_args *a, const char *who, bool coarse_add = false) {
+// smoothn: the tile smoothing of xb_elast_smoothn, which may inject a coarse correction
+// (coarse_out in SMOOTH mode) and start from the zero state (null f_in).
+int validate_elast(const xb_elast_args *a, const char *who, bool smoothn = false) {
   if (a == nullptr) return fail(XB_EINVAL, "%s: null args", who);
   if (int r = check_dtype(a->dtype)) return r;
   if (a->nx <= 0 || a->ny <= 0) return fail(XB_EINVAL, "%s: bad grid %d x %d", who, a->nx, a->ny);
-  if (a->f_in == nullptr) return fail(XB_EINVAL, "%s: null f_in", who);
+  if (a->f_in == nullptr && !smoothn) return fail(XB_EINVAL, "%s: null f_in", who);
   if (a->mode < XB_MODE_STEP || a->mode > XB_MODE_RESIDUAL) return fail(XB_EINVAL, "%s: bad mode %d", who, a->mode);
   if (a->ix0 < 0 || a->ix1 > a->nx || a->iy0 < 0 || a->iy1 > a->ny || a->ix0 >= a->ix1 || a->iy0 >= a->iy1)
     return fail(XB_EINVAL, "%s: bad interior box [%d,%d)x[%d,%d)", who, a->ix0, a->ix1, a->iy0, a->iy1);
@@ -1366,7 +1366,7 @@ int validate_elast(const xb_elast_args *a, const char *who, bool coarse_add = fa
     if (((a->ix1 - a->ix0) & 1) || ((a->iy1 - a->iy0) & 1))
       return fail(XB_EINVAL, "%s: interior box must have even extents for restriction", who);
   }
-  if (a->mode != XB_MODE_RESIDUAL && a->coarse_out != nullptr && !coarse_add)
+  if (a->mode != XB_MODE_RESIDUAL && a->coarse_out != nullptr && !smoothn)
     return fail(XB_EINVAL, "%s: coarse_out is written in residual mode; in smoothing mode only xb_elast_smoothn "
                 "reads it (as a coarse correction)", who);
   if (a->rows != 0 && (a->rows < 1 || a->rows > 64 || (a->rows & (a->rows - 1))))
@@ -1669,19 +1669,44 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
   T kd[SMOOTH_CPT];
   int slot[SMOOTH_CPT], gslot[SMOOTH_CPT], ri[SMOOTH_CPT];
   bool mine = false, gmine = false;
+  if (A.fin == nullptr) {
+    // a coarse level's V-cycle starts from the zero state: nothing to load
+#pragma unroll
+    for (int k = 0; k < SMOOTH_CPT; ++k) {
+#pragma unroll
+      for (int d = 0; d < Q; ++d) f[k][d] = T(0);
+    }
+  } else if (A.coarse != nullptr) {
+    // the prolongation's f + c (the sum of k_prolong_add) on the interior box: no
+    // branch per cell, so the correction's loads go out with the populations'
+#pragma unroll
+    for (int k = 0; k < SMOOTH_CPT; ++k) {
+      const int c = threadIdx.x + k * SMOOTH_THREADS;
+      const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
+      const size_t cell = (size_t)gx * ny + gy;
+      const bool in = gx >= A.ix0 && gx < A.ix1 && gy >= A.iy0 && gy < A.iy1;
+      const size_t cc = in ? (size_t)(A.ix0 + ((gx - A.ix0) >> 1)) * A.cny + (A.iy0 + ((gy - A.iy0) >> 1)) : 0;
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        const T v = A.fin[d * N + cell];
+        const T cv = A.coarse[d * A.Nc + cc];
+        f[k][d] = in ? v + cv : v;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SMOOTH_CPT; ++k) {
+      const int c = threadIdx.x + k * SMOOTH_THREADS;
+      const size_t cell = (size_t)wrapm(x0 + c / TILE, nx) * ny + wrapm(y0 + c % TILE, ny);
+#pragma unroll
+      for (int d = 0; d < Q; ++d) f[k][d] = A.fin[d * N + cell];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < SMOOTH_CPT; ++k) {
     const int c = threadIdx.x + k * SMOOTH_THREADS;
     const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
     const size_t cell = (size_t)gx * ny + gy;
-#pragma unroll
-    for (int d = 0; d < Q; ++d) f[k][d] = A.fin[d * N + cell];
-    if (A.coarse != nullptr && gx >= A.ix0 && gx < A.ix1 && gy >= A.iy0 && gy < A.iy1) {
-      // the prolongation's f + c, the same sum as k_prolong_add
-      const size_t cc = (size_t)(A.ix0 + ((gx - A.ix0) >> 1)) * A.cny + (A.iy0 + ((gy - A.iy0) >> 1));
-#pragma unroll
-      for (int d = 0; d < Q; ++d) f[k][d] = f[k][d] + A.coarse[d * A.Nc + cc];
-    }
     aux[k] = A.coll.load_aux(cell);
     kd[k] = A.coll.kdiv(aux[k], A.nl.kdiv);
     slot[k] = (WALLS && A.bc.slot != nullptr) ? A.bc.slot[cell] : -1;
@@ -1929,6 +1954,8 @@ int check_smoothn(const xb_elast_args *a, int steps, const char *who) {
     return fail(XB_EINVAL, "%s: a coarse correction needs an interior box of even sides", who);
   if (a->coarse_out != nullptr && (a->coarse_out == a->f_out || a->coarse_out == a->f_in))
     return fail(XB_EINVAL, "%s: the coarse correction must not alias f_in or f_out", who);
+  if (a->f_in == nullptr && a->coarse_out != nullptr)
+    return fail(XB_EINVAL, "%s: a zero start state (null f_in) takes no coarse correction", who);
   const int R = a->div_in != nullptr ? 2 : 1;
   if (steps < 1 || TILE - 2 * R * steps < 8)
     return fail(XB_EINVAL, "%s: steps must be in [1, %d]", who, (TILE - 8) / (2 * R));

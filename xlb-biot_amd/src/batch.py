@@ -327,7 +327,9 @@ class ElasticityBatch:
                     continue
                 f, d = cur[i, level]
                 of, od = next_buf(i, level, last_step and level == 0)
-                a = self._sweep_args(I, level, nat.XB_MODE_SMOOTH, f, of, d, od)
+                # f None: a coarse level's zero start state (tile launches only; the
+                # divergence input is then not read and od marks the coupling)
+                a = self._sweep_args(I, level, nat.XB_MODE_SMOOTH, f, of, d if f is not None else od, od)
                 if m > 1:
                     a.rows = m
                 if coarse is not None and i in coarse:
@@ -354,6 +356,7 @@ class ElasticityBatch:
             if not deeper:
                 continue
             jobs, fills, fill_ids = [], [], []
+            zero_start = ks > 1 and self._tile_steps(level + 1) > 1
             for i in deeper:
                 I = self.inst[i]
                 f, d = cur[i, level]
@@ -364,13 +367,18 @@ class ElasticityBatch:
                 jobs.append(a)
                 if level == 0:
                     norm_jobs.append((nat.NormJob(nat.ptr(ws), count, 0), i))
+                if zero_start:
+                    # the first smoothing launch of the coarse level starts from zero
+                    cur[i, level + 1] = (None, None)
+                    continue
                 c0, d0 = I.C[level + 1][0], I.D[level + 1][0]
                 fills.append(nat.FillJob(c0.numel(), nat.ptr(c0), 0.0))
                 fills.append(nat.FillJob(d0.numel(), nat.ptr(d0), 0.0))
                 fill_ids += [i, i]
                 cur[i, level + 1] = (c0, d0)
             plans.append(BatchPlan(nat.XB_BATCH_ELAST_STEP, dt, jobs, deeper, self.device, tag=("residual", level)))
-            plans.append(BatchPlan(nat.XB_BATCH_FILL, dt, fills, fill_ids, self.device, tag=("fill", level + 1)))
+            if fills:
+                plans.append(BatchPlan(nat.XB_BATCH_FILL, dt, fills, fill_ids, self.device, tag=("fill", level + 1)))
         for level in range(L - 1, -1, -1):
             deeper = [i for i, I in enumerate(self.inst) if I.levels - 1 > level]
             if deeper and ks > 1 and self._tile_steps(level) > 1:

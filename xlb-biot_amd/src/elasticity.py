@@ -375,13 +375,15 @@ class LBMElasticity(object):
 
     def _launch(self, f_in, f_out, level, mode, ext=None, resid=None, fpc=None, coarse=None, partials=None,
                 use_force=None):
-        nx, ny = f_in.shape[1], f_in.shape[2]
+        # f_in None: the zero state (tile smoothing only; the shape comes from f_out)
+        ref = f_in if f_in is not None else f_out
+        nx, ny = ref.shape[1], ref.shape[2]
         x0, x1, y0, y1 = self.interior_box(level)
         table = self._tables.get(level)
         if use_force is None:
             use_force = level == 0
         fa = self._force_soa if use_force else None
-        a = nat.ElastArgs(nx, ny, self._code(f_in), mode, self.lamda, self.mu, self.gamma,
+        a = nat.ElastArgs(nx, ny, self._code(ref), mode, self.lamda, self.mu, self.gamma,
                           nat.ptr(f_in), nat.ptr(f_out), nat.ptr(fpc), nat.ptr(fa), nat.ptr(ext), nat.ptr(resid),
                           x0, x1, y0, y1, table.ref() if table is not None else None,
                           nat.ptr(coarse), nat.ptr(partials),
@@ -715,10 +717,8 @@ class LBMElasticity(object):
         norm = ws.finalize(n)[0] if level == 0 else None
         if tail is not None:
             return tail.launch(), norm
-        nxc, nyc = self.grid_size(level + 1)
-        f_c = nat.fill(torch.empty((8, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
-        d_c = nat.fill(torch.empty((1, nxc, nyc), dtype=f.dtype, device=f.device), 0.0)
-        f_c, _ = self._vcycle_nl(f_c, d_c, level + 1, res_c, None, nlf)
+        # the coarse V-cycle from the zero state (None: no fill, no read)
+        f_c, _ = self._vcycle_nl(None, None, level + 1, res_c, None, nlf)
         return f_c, norm
 
     def _smooth_tile_steps(self, level, nlf):
@@ -737,11 +737,15 @@ class LBMElasticity(object):
 
     def _smoothn_nl(self, f, div, level, resid, base, nlf, m, out=None, out_div=None, coarse=None):
         """``m`` coupled smoothing sweeps in one launch (``xb_elast_smoothn``), from
-        ``f`` plus the injection of the next level's correction ``coarse`` if given."""
-        out = torch.empty_like(f) if out is None else out
-        out_div = torch.empty_like(div) if out_div is None else out_div
-        a = self._nl_args(self._launch(f, out, level, nat.XB_MODE_SMOOTH, ext=base, resid=resid), div, out_div,
-                          nlf, level)
+        ``f`` plus the injection of the next level's correction ``coarse`` if given;
+        ``f`` None: from the zero state (``div`` is then not needed)."""
+        nx, ny = self.grid_size(level)
+        dt = self._tdt if f is None else f.dtype
+        out = torch.empty((8, nx, ny), dtype=dt, device=self.device) if out is None else out
+        out_div = torch.empty((1, nx, ny), dtype=dt, device=self.device) if out_div is None else out_div
+        # with a zero start the divergence input is not read: out_div marks the coupling
+        a = self._nl_args(self._launch(f, out, level, nat.XB_MODE_SMOOTH, ext=base, resid=resid),
+                          div if f is not None else out_div, out_div, nlf, level)
         if coarse is not None:
             a.coarse_out = nat.ptr(coarse)
         nat.check(nat.lib().xb_elast_smoothn(a, int(m), nat.stream()), "xb_elast_smoothn")
@@ -763,11 +767,17 @@ class LBMElasticity(object):
         the remainder as single sweeps; the last one writes ``out`` when given.
         ``coarse``: the next level's correction, prolonged into ``f`` first - inside
         the first tile launch when there is one (``f`` is then left as it was), else
-        by the prolongation kernel in place."""
+        by the prolongation kernel in place.  ``f`` None: the zero state (read by no
+        tile launch; zero-filled for single sweeps)."""
         m = self._smooth_tile_steps(level, nlf)
         if coarse is not None and not (m > 1 and k > 1):
             self._prolong_add_div(f, div, coarse, level)
             coarse = None
+        if f is None and not (m > 1 and k > 1):
+            # the zero start state, for the single sweeps
+            nx, ny = self.grid_size(level)
+            f = nat.fill(torch.empty((8, nx, ny), dtype=self._tdt, device=self.device), 0.0)
+            div = nat.fill(torch.empty((1, nx, ny), dtype=self._tdt, device=self.device), 0.0)
         done = 0
         while done < k:
             step = min(m, k - done) if m > 1 else 1
@@ -783,8 +793,9 @@ class LBMElasticity(object):
 
     def _vcycle_nl(self, f, div, level, residual, base, nlf, out=None, out_div=None, coarse=None):
         """``_vcycle`` with the Biot coupling force fused into the sweeps; ``div`` is
-        the divergence of ``f`` and is carried along.  Returns ``(f, div[, norm])``;
-        ``coarse`` (a list) receives the coarse correction prolonged into ``f``."""
+        the divergence of ``f`` and is carried along (``f`` None: the zero state of a
+        coarse level).  Returns ``(f, div[, norm])``; ``coarse`` (a list) receives the
+        coarse correction prolonged into ``f``."""
         f, div = self._smooth_seq_nl(f, div, level, residual, base, nlf, self.k_smooth)
         norm = None
         if level < self.levels - 1:
