@@ -326,7 +326,8 @@ int xb_elast_mg_adjoint(const xb_elast_mg_adjoint_args *args, void *stream);
 /* `sweeps` consecutive XB_ADJ_SMOOTH transposed sweeps of a grid of at most 65536
  * cells, each followed by xb_elast_coupling_adjoint of its force_bar (scale, grad_bc,
  * the args' lam / mu and fields), in one workgroup: bit-identical to the sequence of
- * those calls, in 1 launch instead of 2 * sweeps.  abar_out receives the result; tmp
+ * those calls, in 1 launch instead of 2 * sweeps.  Grids whose 18 nx ny (+ 8 per
+ * interior cell with rbar) values fit 64 KB run with these arrays in LDS (tmp unused).  abar_out receives the result; tmp
  * is (8, nx, ny) scratch; abar_in, abar_out and tmp differ; force_bar is written.  No
  * abar_add and no parameter derivatives (the coarse levels of the transposed
  * V-cycle).  No reference counterpart (the reference's gradients are jax.grad). */

@@ -2715,6 +2715,48 @@ k_smooth_adjoint_small(const ElastMGAdjDev<T> A, const GradAdjDev<T> G, int swee
   }
 }
 
+// The same runs with the grid's adjoint state, force adjoint and residual adjoint held
+// in LDS (copied in at the start, out at the end): each phase then waits on LDS
+// instead of global memory.  Same per-cell functions in the same order, so
+// bit-identical to k_smooth_adjoint_small; used when (18 nx ny + 8 nbox) values fit
+// ADJ_LDS_BYTES.
+constexpr int ADJ_LDS_BYTES = 65536;
+
+template <typename T, bool HAS_BC, bool HAS_GBC>
+__global__ void __launch_bounds__(ADJ_SMALL_THREADS)
+k_smooth_adjoint_small_lds(const ElastMGAdjDev<T> A, const GradAdjDev<T> G, int sweeps) {
+  __shared__ T arena[ADJ_LDS_BYTES / sizeof(T)];
+  const int N = A.nx * A.ny;
+  const int Ni = (A.ix1 - A.ix0) * (A.iy1 - A.iy0);
+  T *buf0 = arena, *buf1 = arena + 8 * N, *fb = arena + 16 * N;
+  T *rb = A.rbar != nullptr ? arena + 18 * N : nullptr;
+  for (int i = threadIdx.x; i < 8 * N; i += ADJ_SMALL_THREADS) buf0[i] = A.ab_in[i];
+  if (rb != nullptr)
+    for (int i = threadIdx.x; i < 8 * Ni; i += ADJ_SMALL_THREADS) rb[i] = A.rbar[i];
+  __syncthreads();
+  T *in = buf0;
+  for (int s = 0; s < sweeps; ++s) {
+    T *out = in == buf0 ? buf1 : buf0;
+    ElastMGAdjDev<T> a = A;
+    a.ab_in = in;
+    a.ab_out = out;
+    a.fbar = fb;
+    a.rbar = rb;
+    for (int i = threadIdx.x; i < N; i += ADJ_SMALL_THREADS) elast_mg_adjoint_cell<T, XB_ADJ_SMOOTH, HAS_BC>(a, i);
+    __syncthreads();
+    GradAdjDev<T> g = G;
+    g.gbar = fb;
+    g.fbar = out;
+    for (int i = threadIdx.x; i < N; i += ADJ_SMALL_THREADS) grad_adjoint_cell<T, HAS_GBC, true>(g, i);
+    __syncthreads();
+    in = out;
+  }
+  for (int i = threadIdx.x; i < 8 * N; i += ADJ_SMALL_THREADS) A.ab_out[i] = in[i];
+  for (int i = threadIdx.x; i < 2 * N; i += ADJ_SMALL_THREADS) A.fbar[i] = fb[i];
+  if (rb != nullptr)
+    for (int i = threadIdx.x; i < 8 * Ni; i += ADJ_SMALL_THREADS) A.rbar[i] = rb[i];
+}
+
 // coarse = P^T fine: the sum of the four children of each coarse interior cell, zero
 // on the coarse boundary layers (the transpose of the injection of xb_elast_prolong_add)
 template <typename T>
@@ -2974,6 +3016,14 @@ int smooth_adjoint_small_t(const xb_elast_mg_adjoint_args *a, int sweeps, double
   T *t = static_cast<T *>(tmp);
   const bool hb = A.bc.slot != nullptr, hg = G.bc.slot != nullptr;
   const dim3 gr(1), bl(ADJ_SMALL_THREADS);
+  const int64_t N = (int64_t)a->nx * a->ny, Ni = (int64_t)(a->ix1 - a->ix0) * (a->iy1 - a->iy0);
+  if ((18 * N + (a->rbar != nullptr ? 8 * Ni : 0)) * (int64_t)sizeof(T) <= ADJ_LDS_BYTES) {
+    if (hb && hg) hipLaunchKernelGGL((k_smooth_adjoint_small_lds<T, true, true>), gr, bl, 0, st, A, G, sweeps);
+    else if (hb) hipLaunchKernelGGL((k_smooth_adjoint_small_lds<T, true, false>), gr, bl, 0, st, A, G, sweeps);
+    else if (hg) hipLaunchKernelGGL((k_smooth_adjoint_small_lds<T, false, true>), gr, bl, 0, st, A, G, sweeps);
+    else hipLaunchKernelGGL((k_smooth_adjoint_small_lds<T, false, false>), gr, bl, 0, st, A, G, sweeps);
+    return check_launch("xb_elast_smooth_adjoint_small");
+  }
   if (hb && hg) hipLaunchKernelGGL((k_smooth_adjoint_small<T, true, true>), gr, bl, 0, st, A, G, sweeps, t);
   else if (hb) hipLaunchKernelGGL((k_smooth_adjoint_small<T, true, false>), gr, bl, 0, st, A, G, sweeps, t);
   else if (hg) hipLaunchKernelGGL((k_smooth_adjoint_small<T, false, true>), gr, bl, 0, st, A, G, sweeps, t);

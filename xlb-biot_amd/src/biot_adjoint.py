@@ -83,10 +83,13 @@ class BiotAdjointSession:
         self.ckpt_step = []
         self.ckpt_cycle = []
         # coarse grids of at most this many cells run their transposed smoothing in one
-        # workgroup (xb_elast_smooth_adjoint_small, one launch per run of sweeps); 0: the
-        # LDS tiles on every level (faster at 1024 x 1026: 0.353 s against 0.378 s for
-        # the C4 backward with 600)
-        self.adjoint_small_cells = int(getattr(p, "adjoint_small_cells", 0))
+        # workgroup (xb_elast_smooth_adjoint_small, one launch per run of sweeps; None:
+        # the grids it holds in LDS); default 0, the LDS tiles on every level.  C4
+        # backward (1024 x 1026): tiles 0.350 s, single workgroup from LDS on the four
+        # smallest levels 0.361 s, from global memory below 600 cells 0.378 s - the
+        # per-cell wall-rule reads of the single workgroup stay in global memory
+        small = getattr(p, "adjoint_small_cells", 0)
+        self.adjoint_small_cells = None if small is None else int(small)
         # transposed smoothing sweeps per launch on LDS tiles (0: one launch pair per
         # sweep, xb_elast_mg_adjoint + xb_elast_coupling_adjoint)
         self.adjoint_tile_steps = int(getattr(p, "adjoint_tile_steps", 2))
@@ -266,7 +269,7 @@ class BiotAdjointSession:
         el, k = self.el, self.el.k_smooth
         S, R = nat.XB_ADJ_SMOOTH, nat.XB_ADJ_RESIDUAL
         nx, ny = el.grid_size(level)
-        small = states is None and level > 0 and nx * ny <= self.adjoint_small_cells
+        small = states is None and level > 0 and self._small(level)
         post = states[k + 1:] if states is not None else [None] * k
         if small:
             ab = self._smooth_adjoint_small(level, ab, k, rbar)
@@ -293,6 +296,19 @@ class BiotAdjointSession:
         if small:
             return self._smooth_adjoint_small(level, ab, k, rbar)
         return self._smooth_adjoint_run(level, ab, pre, rbar, acc)
+
+    def _small(self, level):
+        """Whether ``level``'s transposed smoothing runs as one single-workgroup launch
+        per run of sweeps: when its adjoint state, force adjoint and residual adjoint
+        fit the kernel's 64 KB of LDS (18 values per cell and 8 per interior cell), or
+        by the ``adjoint_small_cells`` threshold when one is set."""
+        el = self.el
+        nx, ny = el.grid_size(level)
+        if self.adjoint_small_cells is not None:
+            return nx * ny <= self.adjoint_small_cells
+        x0, x1, y0, y1 = el.interior_box(level)
+        size = torch.tensor([], dtype=self.dtype).element_size()
+        return (18 * nx * ny + 8 * (x1 - x0) * (y1 - y0)) * size <= 65536
 
     def _smooth_adjoint_small(self, level, ab_in, sweeps, rbar):
         """``sweeps`` transposed smoothing sweeps of a coarse level (no recorded
