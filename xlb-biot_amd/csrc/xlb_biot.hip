@@ -1369,7 +1369,8 @@ int validate_elast(const xb_elast_args *a, const char *who, bool smoothn = false
   if (a->mode != XB_MODE_RESIDUAL && a->coarse_out != nullptr && !smoothn)
     return fail(XB_EINVAL, "%s: coarse_out is written in residual mode; in smoothing mode only xb_elast_smoothn "
                 "reads it (as a coarse correction)", who);
-  if (a->rows != 0 && (a->rows < 1 || a->rows > 64 || (a->rows & (a->rows - 1))))
+  // (the tile smoothing takes no sweep launch shape: a batch job's rows is its steps)
+  if (!smoothn && a->rows != 0 && (a->rows < 1 || a->rows > 64 || (a->rows & (a->rows - 1))))
     return fail(XB_EINVAL, "%s: rows must be 0 or a power of two in [1, 64]", who);
   if (a->mode == XB_MODE_RESIDUAL && a->rows == 1)
     return fail(XB_EINVAL, "%s: the residual sweep restricts row pairs (rows >= 2)", who);
