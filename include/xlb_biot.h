@@ -19,9 +19,13 @@
  *    arithmetic, FP32 storage): FP64 buffers and arithmetic, and every population
  *    value rounded to FP32 where the reference stores it in its FP32 arrays (the
  *    collision's cast_to_output, each boundary-rule stage, the sweep's output, the
- *    prolongation's sum).  Only xb_elast_step, xb_elast_partials_count,
- *    xb_darcy_step, xb_elast_prolong_add and xb_elast_prolong_add_div take it;
- *    every other entry point returns XB_EUNSUPPORTED for it.
+ *    prolongation's sum).  XB_F64_S16 and XB_F32_S16 are "f64/f16" and "f32/f16"
+ *    the same way: FP64 / FP32 buffers and arithmetic, every stored population value
+ *    rounded to FP16 (one correctly rounded conversion, as the reference's astype).
+ *    ("f16/f16", FP16 arithmetic, has no dtype here.)  Only xb_elast_step,
+ *    xb_elast_partials_count, xb_darcy_step, xb_elast_prolong_add and
+ *    xb_elast_prolong_add_div take these three; every other entry point returns
+ *    XB_EUNSUPPORTED for them.
  *  - Every call takes the HIP stream to launch on (hipStream_t as void*) and
  *    returns an int status: XB_OK, or a negative XB_E* code; the message of the
  *    last failure on the calling thread is returned by xb_last_error().
@@ -40,7 +44,7 @@ extern "C" {
 
 #define XB_ABI_VERSION 1
 
-enum { XB_F64 = 0, XB_F32 = 1, XB_F64_S32 = 2 };
+enum { XB_F64 = 0, XB_F32 = 1, XB_F64_S32 = 2, XB_F64_S16 = 3, XB_F32_S16 = 4 };
 
 enum {
   XB_OK = 0,

@@ -23,6 +23,8 @@ class PrecisionPolicy:
         "f64/f64": (torch.float64, torch.float64),
         "f32/f32": (torch.float32, torch.float32),
         "f64/f32": (torch.float64, torch.float32),
+        "f64/f16": (torch.float64, torch.float16),
+        "f32/f16": (torch.float32, torch.float16),
     }
 
     def __init__(self, precision):

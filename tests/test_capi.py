@@ -122,13 +122,16 @@ def test_solver_validation_errors_match_reference():
         LBMElasticity(**{**kw, "MG_BCs": {"left": 1}})
     with pytest.raises(ValueError):
         LBMElasticity(**{**kw, "lattice": LatticeD2Q9("f64/f64")})
-    # elasticity.py:606-629: "f64/f32" is FP64 arithmetic with FP32 storage; half
-    # precision storage is not carried by these kernels
-    mixed = LBMElasticity(**{**kw, "precision": "f64/f32"}).precisionPolicy
-    assert mixed.mixed and mixed.compute_dtype is np.float64 and mixed.output_dtype is np.float32
-    for prec in ("f32/f16", "f64/f16", "f16/f16"):
-        with pytest.raises(NotImplementedError):
-            LBMElasticity(**{**kw, "precision": prec})
+    # elasticity.py:606-629: "f64/f32", "f64/f16" and "f32/f16" are arithmetic in the
+    # first precision with storage in the second; FP16 arithmetic is not carried
+    from src import _native as nat
+    for prec, cdt, odt, code in (("f64/f32", np.float64, np.float32, nat.XB_F64_S32),
+                                 ("f64/f16", np.float64, np.float16, nat.XB_F64_S16),
+                                 ("f32/f16", np.float32, np.float16, nat.XB_F32_S16)):
+        pp = LBMElasticity(**{**kw, "precision": prec}).precisionPolicy
+        assert pp.mixed and pp.compute_dtype is cdt and pp.output_dtype is odt and pp.storage_code == code
+    with pytest.raises(NotImplementedError):
+        LBMElasticity(**{**kw, "precision": "f16/f16"})
     with pytest.raises(TypeError):
         P.biot_periodic("gpu", nx=16, coupling="sideways")
 

@@ -88,13 +88,21 @@ struct SweepGeom {
   int rows;               // rows per wavefront item
   int nstrips, nitems;
   int ix0, ix1, iy0, iy1; // box carrying the subtracted residual
-  int round32;            // XB_F64_S32: population values rounded to FP32 where stored
+  int rmode;              // storage rounding of population values where stored:
+                          // 0 none, 1 FP32 (XB_F64_S32), 2 FP16 (XB_F64_S16, XB_F32_S16)
 };
 
-// FP32 storage of FP64 arithmetic (XB_F64_S32): the value an FP32 array would hold
-template <typename T, int Q> __device__ __forceinline__ void store32(T (&v)[Q]) {
+// Storage in a narrower type than the arithmetic: the value an FP32 (mode 1) or FP16
+// (mode 2, one correctly rounded conversion from T, as the reference's astype) array
+// would hold
+template <typename T, int Q> __device__ __forceinline__ void store_round(T (&v)[Q], int mode) {
+  if (mode == 1) {
 #pragma unroll
-  for (int d = 0; d < Q; ++d) v[d] = T(float(v[d]));
+    for (int d = 0; d < Q; ++d) v[d] = T(float(v[d]));
+  } else {
+#pragma unroll
+    for (int d = 0; d < Q; ++d) v[d] = T(_Float16(v[d]));
+  }
 }
 
 // Rows per wavefront: enough items to give every SIMD several waves, at most 64
@@ -231,12 +239,12 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
     } else {
       coll.collide(pre, rw.aux, pc);
     }
-    if (g.round32) store32(pc);
+    if (g.rmode) store_round(pc, g.rmode);
     if constexpr (HAS_BC) {
       stage_if_needed(rw.slot);
       if (rw.slot >= 0 && bc.code_pc != nullptr) {
         rules_pc(pc, pre, rw.slot);
-        if (g.round32) store32(pc);
+        if (g.rmode) store_round(pc, g.rmode);
       }
     }
   };
@@ -265,7 +273,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
     if constexpr (HAS_BC) {
       if (S1 >= 0 && bc.code_ps != nullptr) {
         rules_ps(v, R1, S1);
-        if (g.round32) store32(v);
+        if (g.rmode) store_round(v, g.rmode);
       }
     }
     const size_t cell = (size_t)x * g.ny + y;
@@ -286,7 +294,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
         T o[Q];
 #pragma unroll
         for (int d = 0; d < Q; ++d) o[d] = omg * F1[d] + gamma * v[d];
-        if (g.round32) store32(o);
+        if (g.rmode) store_round(o, g.rmode);
 #pragma unroll
         for (int d = 0; d < Q; ++d) fout[d * N + cell] = o[d];
         if constexpr (HAS_NL && Q == 8) {
@@ -298,7 +306,7 @@ __device__ __forceinline__ void sweep_item(const SweepGeom &g, const Coll &coll,
       T pair[Q], r[Q];
 #pragma unroll
       for (int d = 0; d < Q; ++d) r[d] = F1[d] - v[d];
-      if (g.round32) store32(r);
+      if (g.rmode) store_round(r, g.rmode);
 #pragma unroll
       for (int d = 0; d < Q; ++d) {
         if (out_lane) nrm += (double)r[d] * (double)r[d];
@@ -660,7 +668,7 @@ inline dim3 ew_grid(int64_t n) {
 
 template <typename T>
 __global__ void k_prolong_add(int nx, int ny, T *__restrict__ f, const T *__restrict__ c, int cnx, int cny, int ix0,
-                              int ix1, int iy0, int iy1, int round32) {
+                              int ix1, int iy0, int iy1, int rmode) {
   const int nxi = ix1 - ix0, nyi = iy1 - iy0;
   const int64_t n = (int64_t)nxi * nyi;
   const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
@@ -671,7 +679,7 @@ __global__ void k_prolong_add(int nx, int ny, T *__restrict__ f, const T *__rest
     T v[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) v[d] = f[d * N + cell] + c[d * Nc + cc];
-    if (round32) store32(v);
+    if (rmode) store_round(v, rmode);
 #pragma unroll
     for (int d = 0; d < 8; ++d) f[d * N + cell] = v[d];
   }
@@ -680,7 +688,7 @@ __global__ void k_prolong_add(int nx, int ny, T *__restrict__ f, const T *__rest
 template <typename T>
 __global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__restrict__ c, int cnx, int cny,
                                   int ix0, int ix1, int iy0, int iy1, T k, T *__restrict__ div,
-                                  const T *__restrict__ lamf, const T *__restrict__ muf, int round32) {
+                                  const T *__restrict__ lamf, const T *__restrict__ muf, int rmode) {
   const int nxi = ix1 - ix0, nyi = iy1 - iy0;
   const int64_t n = (int64_t)nxi * nyi;
   const size_t N = (size_t)nx * ny, Nc = (size_t)cnx * cny;
@@ -691,7 +699,7 @@ __global__ void k_prolong_add_div(int nx, int ny, T *__restrict__ f, const T *__
     T v[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) v[d] = f[d * N + cell] + c[d * Nc + cc];
-    if (round32) store32(v);
+    if (rmode) store_round(v, rmode);
 #pragma unroll
     for (int d = 0; d < 8; ++d) f[d * N + cell] = v[d];
     div[cell] = elast_div_of<T>(v, lamf != nullptr ? elast_kdiv<T>(lamf[cell], muf[cell]) : k);
@@ -1281,6 +1289,17 @@ int check_dtype(int32_t dtype) {
   return XB_OK;
 }
 
+// A storage-rounding dtype as its arithmetic dtype and rounding mode (0 none, 1 to
+// FP32, 2 to FP16); other dtypes pass unchanged with mode 0.
+int split_storage(int32_t dtype, int32_t &arith) {
+  switch (dtype) {
+  case XB_F64_S32: arith = XB_F64; return 1;
+  case XB_F64_S16: arith = XB_F64; return 2;
+  case XB_F32_S16: arith = XB_F32; return 2;
+  default: arith = dtype; return 0;
+  }
+}
+
 int check_bc(const xb_bc_table *t, bool need_ps) {
   if (t == nullptr) return XB_OK;
   if (t->nslots < 0) return fail(XB_EINVAL, "bc table: negative slot count");
@@ -1325,14 +1344,14 @@ template <typename T> void elast_coll(const xb_elast_args *a, ElastColl<T> &c, N
   }
 }
 
-template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st, int round32 = 0) {
+template <typename T> int elast_step_t(const xb_elast_args *a, hipStream_t st, int rmode = 0) {
   ElastColl<T> c;
   NLDev<T> nl;
   elast_coll<T>(a, c, nl);
   const NLDev<T> *nlp = a->div_in != nullptr ? &nl : nullptr;
   SweepGeom geo;
   elast_geom(a, geo);
-  geo.round32 = round32;
+  geo.rmode = rmode;
   switch (a->mode) {
   case XB_MODE_STEP:
     return launch_sweep<T, ElastColl<T>, XB_MODE_STEP>(geo, c, a->f_in, a->f_out, a->fpc_out, a->resid, a->bc, 1.0,
@@ -3610,12 +3629,13 @@ int64_t xb_elast_partials_count(const xb_elast_args *a) {
 #if XB_IN_PART(1)
 int xb_elast_step(const xb_elast_args *a, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (a != nullptr && a->dtype == XB_F64_S32) {
-    // FP64 buffers and arithmetic, FP32 storage: the FP64 sweep with its stores rounded
+  int32_t arith = 0;
+  if (const int rm = a != nullptr ? split_storage(a->dtype, arith) : 0) {
+    // buffers and arithmetic in `arith`, narrower storage: the sweep with its stores rounded
     xb_elast_args b = *a;
-    b.dtype = XB_F64;
+    b.dtype = arith;
     if (int r = validate_elast(&b, "xb_elast_step")) return r;
-    return elast_step_t<double>(&b, st, 1);
+    return arith == XB_F64 ? elast_step_t<double>(&b, st, rm) : elast_step_t<float>(&b, st, rm);
   }
   if (int r = validate_elast(a, "xb_elast_step")) return r;
   return a->dtype == XB_F64 ? elast_step_t<double>(a, st) : elast_step_t<float>(a, st);
@@ -3646,8 +3666,7 @@ int xb_elast_step2(const xb_elast_args *a, void *stream) {
 #if XB_IN_PART(0)
 int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const void *coarse, int32_t cnx, int32_t cny,
                          int32_t ix0, int32_t ix1, int32_t iy0, int32_t iy1, void *stream) {
-  const int round32 = dtype == XB_F64_S32 ? 1 : 0; // FP64 arithmetic, FP32 storage
-  if (round32) dtype = XB_F64;
+  const int rmode = split_storage(dtype, dtype); // narrower storage: the sums rounded
   if (int r = check_dtype(dtype)) return r;
   if (f == nullptr || coarse == nullptr) return fail(XB_EINVAL, "xb_elast_prolong_add: null pointer");
   if (ix0 < 0 || ix1 > nx || iy0 < 0 || iy1 > ny || ix0 >= ix1 || iy0 >= iy1 || ((ix1 - ix0) & 1) || ((iy1 - iy0) & 1))
@@ -3658,18 +3677,17 @@ int xb_elast_prolong_add(int32_t nx, int32_t ny, int32_t dtype, void *f, const v
   const dim3 gr = ew_grid((int64_t)(ix1 - ix0) * (iy1 - iy0));
   if (dtype == XB_F64)
     hipLaunchKernelGGL(k_prolong_add<double>, gr, dim3(256), 0, st, nx, ny, (double *)f, (const double *)coarse, cnx,
-                       cny, ix0, ix1, iy0, iy1, round32);
+                       cny, ix0, ix1, iy0, iy1, rmode);
   else
     hipLaunchKernelGGL(k_prolong_add<float>, gr, dim3(256), 0, st, nx, ny, (float *)f, (const float *)coarse, cnx, cny,
-                       ix0, ix1, iy0, iy1, 0);
+                       ix0, ix1, iy0, iy1, rmode);
   return check_launch("xb_elast_prolong_add");
 }
 
 int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, double mu, void *f,
                              const void *coarse, int32_t cnx, int32_t cny, int32_t ix0, int32_t ix1, int32_t iy0,
                              int32_t iy1, void *div, const void *lam_field, const void *mu_field, void *stream) {
-  const int round32 = dtype == XB_F64_S32 ? 1 : 0; // FP64 arithmetic, FP32 storage
-  if (round32) dtype = XB_F64;
+  const int rmode = split_storage(dtype, dtype); // narrower storage: the sums rounded
   if (int r = check_dtype(dtype)) return r;
   if (f == nullptr || coarse == nullptr || div == nullptr)
     return fail(XB_EINVAL, "xb_elast_prolong_add_div: null pointer");
@@ -3685,11 +3703,11 @@ int xb_elast_prolong_add_div(int32_t nx, int32_t ny, int32_t dtype, double lam, 
   if (dtype == XB_F64)
     hipLaunchKernelGGL(k_prolong_add_div<double>, gr, dim3(256), 0, st, nx, ny, (double *)f, (const double *)coarse,
                        cnx, cny, ix0, ix1, iy0, iy1, k, (double *)div, (const double *)lam_field,
-                       (const double *)mu_field, round32);
+                       (const double *)mu_field, rmode);
   else
     hipLaunchKernelGGL(k_prolong_add_div<float>, gr, dim3(256), 0, st, nx, ny, (float *)f, (const float *)coarse, cnx,
                        cny, ix0, ix1, iy0, iy1, (float)k, (float *)div, (const float *)lam_field,
-                       (const float *)mu_field, 0);
+                       (const float *)mu_field, rmode);
   return check_launch("xb_elast_prolong_add_div");
 }
 
@@ -3763,8 +3781,8 @@ int xb_elast_equilibrium(int32_t nx, int32_t ny, int32_t dtype, double lam, doub
 #if XB_IN_PART(2)
 int xb_darcy_step(const xb_darcy_args *a, void *stream) {
   if (a == nullptr) return fail(XB_EINVAL, "xb_darcy_step: null args");
-  const int round32 = a->dtype == XB_F64_S32 ? 1 : 0; // FP64 arithmetic, FP32 storage
-  const int32_t dtype = round32 ? XB_F64 : a->dtype;
+  int32_t dtype = 0;
+  const int rmode = split_storage(a->dtype, dtype); // narrower storage: the stores rounded
   if (int r = check_dtype(dtype)) return r;
   if (a->nx <= 0 || a->ny <= 0 || a->f_in == nullptr || a->f_out == nullptr)
     return fail(XB_EINVAL, "xb_darcy_step: bad arguments");
@@ -3773,7 +3791,7 @@ int xb_darcy_step(const xb_darcy_args *a, void *stream) {
     return fail(XB_EINVAL, "xb_darcy_step: rows must be 0 or a power of two in [1, 64]");
   hipStream_t st = static_cast<hipStream_t>(stream);
   SweepGeom g = make_geom(a->nx, a->ny, 0, 0, a->nx, a->ny, 0, a->nx, 0, a->ny, a->rows, STRIP, 1);
-  g.round32 = round32;
+  g.rmode = rmode;
   if (dtype == XB_F64) {
     DarcyColl<double> c;
     c.omega = a->omega;

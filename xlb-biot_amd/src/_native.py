@@ -9,7 +9,7 @@ import os
 
 import torch
 
-XB_F64, XB_F32, XB_F64_S32 = 0, 1, 2
+XB_F64, XB_F32, XB_F64_S32, XB_F64_S16, XB_F32_S16 = 0, 1, 2, 3, 4
 XB_MODE_STEP, XB_MODE_SMOOTH, XB_MODE_RESIDUAL = 0, 1, 2
 XB_RULE_ZERO = 100
 XB_EINVAL, XB_EUNSUPPORTED, XB_EHIP, XB_ENONFINITE = -1, -2, -3, -4

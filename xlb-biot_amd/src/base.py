@@ -38,9 +38,11 @@ class PrecisionPolicy:
 
     ``compute_dtype`` / ``output_dtype`` are NumPy dtypes, so driver hooks that pass
     them to NumPy keep working; ``torch_compute`` / ``torch_output`` are the torch ones.
-    Supported: ``"f64/f64"``, ``"f32/f32"`` and ``"f64/f32"`` (``mixed``: FP64 buffers and
-    arithmetic, every population value rounded to FP32 where the reference stores
-    it, the states handed out in FP32).  Half-precision storage raises.
+    Supported: ``"f64/f64"``, ``"f32/f32"``, and the narrower-storage policies
+    ``"f64/f32"``, ``"f64/f16"`` and ``"f32/f16"`` (``mixed``: buffers and arithmetic in
+    the compute precision, every population value rounded to the storage precision
+    where the reference stores it, the states handed out in the storage precision;
+    ``storage_code`` is their kernel dtype).  ``"f16/f16"`` (FP16 arithmetic) raises.
     """
     TABLE = {"f64/f64": (np.float64, np.float64), "f32/f32": (np.float32, np.float32),
              "f32/f16": (np.float32, np.float16), "f16/f16": (np.float16, np.float16),
@@ -49,19 +51,21 @@ class PrecisionPolicy:
     def __init__(self, precision):
         # elasticity.py:606-629: unknown strings fall back to f32/f32
         cdt, odt = self.TABLE.get(precision, (np.float32, np.float32))
-        if np.float16 in (cdt, odt):
+        if cdt is np.float16:
             raise NotImplementedError(
-                f"precision {precision!r}: half-precision storage is not implemented on this backend "
-                "(use 'f64/f64', 'f32/f32' or 'f64/f32')")
+                f"precision {precision!r}: half-precision arithmetic is not implemented on this backend "
+                "(FP16 storage: 'f32/f16' or 'f64/f16')")
         self.compute_dtype = cdt
         self.param_dtype = cdt
         self.output_dtype = odt
         self.torch_compute = torch_dtype(cdt)
         self.torch_output = torch_dtype(odt)
         self.mixed = cdt is not odt
+        self.storage_code = {(np.float64, np.float32): nat.XB_F64_S32, (np.float64, np.float16): nat.XB_F64_S16,
+                             (np.float32, np.float16): nat.XB_F32_S16}.get((cdt, odt))
 
     def require_single(self, what):
-        """Raise for the ``"f64/f32"`` policy in paths that have no FP32-storage form."""
+        """Raise for the narrower-storage policies in paths that have no such form."""
         if self.mixed:
             raise NotImplementedError(f"{what} is implemented for 'f64/f64' and 'f32/f32' only")
 

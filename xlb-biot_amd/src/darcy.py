@@ -53,9 +53,9 @@ class AdvectionDiffusionReactionBGK(LBMBase):
         return self.precisionPolicy.torch_compute
 
     def _code(self, t):
-        """Kernel dtype of a population buffer: FP32 storage of FP64 arithmetic under
-        the ``"f64/f32"`` policy."""
-        return nat.XB_F64_S32 if self.precisionPolicy.mixed else nat.dtype_code(t.dtype)
+        """Kernel dtype of a population buffer: the storage-rounding dtype under the
+        ``"f64/f32"``, ``"f64/f16"`` and ``"f32/f16"`` policies."""
+        return self.precisionPolicy.storage_code if self.precisionPolicy.mixed else nat.dtype_code(t.dtype)
 
     def _out(self, soa):
         """A population state handed to the caller, in the storage precision."""
@@ -206,8 +206,8 @@ class AdvectionDiffusionReactionBGK(LBMBase):
     def _multi(self):
         """The plain-step launch of step loops (choice, steps per launch,
         launch(src, dst)): ``self.step_launch`` when set (an ``autotune`` choice),
-        else the tuned one.  FP32 storage (``"f64/f32"``) is rounded by the single-step
-        sweep only."""
+        else the tuned one.  Narrower storage (``"f64/f32"``, ``"f64/f16"``,
+        ``"f32/f16"``) is rounded by the single-step sweep only."""
         if self.precisionPolicy.mixed:
             return autotune.SWEEP1, 1, None
         choice = getattr(self, "step_launch", None) or autotune.lookup(self._tune_key() + "/multi")

@@ -37,7 +37,7 @@ class LBMElasticity(object):
         self.precision = kwargs.get("precision")
         self.precisionPolicy = PrecisionPolicy(self.precision)
         if self.precisionPolicy.mixed:
-            # FP32 storage is rounded by the single sweeps: no multi-step smoothing
+            # narrower storage is rounded by the single sweeps: no multi-step smoothing
             # tiles, no single-workgroup coarse tail
             self.use_tail = False
             self.smooth_tile_steps = 1
@@ -108,9 +108,9 @@ class LBMElasticity(object):
         return self.precisionPolicy.torch_compute
 
     def _code(self, t):
-        """Kernel dtype of a population buffer: FP32 storage of FP64 arithmetic under
-        the ``"f64/f32"`` policy."""
-        return nat.XB_F64_S32 if self.precisionPolicy.mixed else nat.dtype_code(t.dtype)
+        """Kernel dtype of a population buffer: the storage-rounding dtype under the
+        ``"f64/f32"``, ``"f64/f16"`` and ``"f32/f16"`` policies."""
+        return self.precisionPolicy.storage_code if self.precisionPolicy.mixed else nat.dtype_code(t.dtype)
 
     def _out(self, soa):
         """A population state handed to the caller, in the storage precision."""
@@ -484,7 +484,7 @@ class LBMElasticity(object):
 
     def _multi(self, ext):
         if self.precisionPolicy.mixed:
-            # FP32 storage is rounded by the single-step sweep only
+            # narrower storage is rounded by the single-step sweep only
             return autotune.SWEEP1, 1, None
         choice = autotune.lookup(self._multi_key(ext is not None))
         if choice == 0:
