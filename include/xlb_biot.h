@@ -237,6 +237,9 @@ typedef struct xb_tail_plan {   /* filled by xb_vcycle_tail_prepare             
   int32_t dtype, nlevels, k_smooth, coupled;
   double lam, mu, gamma, nl_scale;
   void *table;                  /* device copy of the level descriptors                */
+  int64_t lds_bytes;            /* LDS the levels' arrays and rule tables take in the
+                                   kernel's 128 KB arena, or 0: then they stay in global
+                                   memory (the plan's buffers)                          */
 } xb_tail_plan;
 
 /* Device bytes of the level descriptors of a tail of nlevels (<= 16) levels. */
@@ -250,7 +253,9 @@ int xb_vcycle_tail_prepare(int32_t dtype, double lam, double mu, double gamma, i
                            double nl_scale, int32_t coupled, const xb_tail_level *levels,
                            int32_t nlevels, void *table, int64_t table_bytes, xb_tail_plan *plan);
 
-/* The tail V-cycle: one workgroup, graph-capturable. */
+/* The tail V-cycle: one workgroup, graph-capturable.  With plan->lds_bytes set it
+ * copies the rule tables and the first level's residual into LDS, runs there and
+ * writes the first level's f0 (the other buffers are then not written). */
 int xb_vcycle_tail(const xb_tail_plan *plan, void *stream);
 
 /* ---- adjoint steps ---------------------------------------------------------------

@@ -3568,6 +3568,149 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(const TailLevelDev
   }
 }
 
+// The same V-cycle with every level's states, scratch, residual and rule tables in LDS:
+// the levels' descriptors are re-pointed into an LDS arena, the rule tables and the
+// first level's residual are copied in, and the first level's state 0 is copied out at
+// the end.  The phases then wait on LDS instead of global memory (the per-cell
+// arithmetic is unchanged, so the result is bit-identical).  Used when the levels fit
+// TAIL_LDS_BYTES (tail_lds_bytes; the plan records the size, 0 when they do not).
+constexpr int TAIL_LDS_BYTES = 128 * 1024;
+
+__host__ __device__ inline size_t tail_rnd8(size_t b) { return (b + 7) & ~size_t(7); }
+
+// LDS bytes of one level's arrays: states f0 f1 and the scratch (8 N each), the
+// divergence carry (2 N, coupled), the residual (8 Ni), the rule tables present
+struct TailLdsDims {
+  int N, Ni, bc_slot, bc_ns, bc_pc, bc_ps, gb_slot, gb_ns;
+};
+
+__host__ __device__ inline size_t tail_level_lds_bytes(const TailLdsDims &d, size_t ts, bool coupled) {
+  size_t b = 3 * tail_rnd8((size_t)8 * d.N * ts) + tail_rnd8((size_t)8 * d.Ni * ts);
+  if (coupled) b += 2 * tail_rnd8((size_t)d.N * ts);
+  if (d.bc_slot) b += tail_rnd8((size_t)d.N * 4);
+  if (d.bc_pc) b += tail_rnd8((size_t)d.bc_ns * 8) + tail_rnd8((size_t)d.bc_ns * 8 * ts);
+  if (d.bc_ps) b += tail_rnd8((size_t)d.bc_ns * 8) + tail_rnd8((size_t)d.bc_ns * 8 * ts);
+  if (coupled && d.gb_slot) b += tail_rnd8((size_t)d.N * 4) + tail_rnd8((size_t)d.gb_ns * 9) +
+                                 tail_rnd8((size_t)d.gb_ns * 9 * ts);
+  return b;
+}
+
+template <typename T> __device__ TailLdsDims tail_dims(const TailLevelDev<T> &L) {
+  TailLdsDims d;
+  d.N = L.nx * L.ny;
+  d.Ni = (L.ix1 - L.ix0) * (L.iy1 - L.iy0);
+  d.bc_slot = L.bc.slot != nullptr;
+  d.bc_ns = L.bc.nslots;
+  d.bc_pc = L.bc.slot != nullptr && L.bc.code_pc != nullptr;
+  d.bc_ps = L.bc.slot != nullptr && L.bc.code_ps != nullptr;
+  d.gb_slot = L.gbc.slot != nullptr;
+  d.gb_ns = L.gbc.nslots;
+  return d;
+}
+
+template <typename U> __device__ __forceinline__ void lds_copy(U *dst, const U *src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+template <typename T, bool COUPLED>
+__global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail_lds(const TailLevelDev<T> *__restrict__ lv, int nlev,
+                                                                  int k, TailDev<T> P) {
+  __shared__ double arena_raw[TAIL_LDS_BYTES / sizeof(double)];
+  // the re-pointed descriptors (raw storage: the descriptor type has default member
+  // initialisers, which __shared__ variables may not run)
+  __shared__ double ls_raw[(sizeof(TailLevelDev<T>) * TAIL_MAX_LEVELS + 7) / 8];
+  TailLevelDev<T> *Ls = reinterpret_cast<TailLevelDev<T> *>(ls_raw);
+  char *arena = reinterpret_cast<char *>(arena_raw);
+  if (threadIdx.x == 0) {
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+      char *p = arena + off;
+      off += tail_rnd8(bytes);
+      return p;
+    };
+    for (int l = 0; l < nlev; ++l) {
+      const TailLevelDev<T> &G = lv[l];
+      TailLevelDev<T> D = G;
+      const TailLdsDims d = tail_dims(G);
+      D.f[0] = reinterpret_cast<T *>(carve((size_t)8 * d.N * sizeof(T)));
+      D.f[1] = reinterpret_cast<T *>(carve((size_t)8 * d.N * sizeof(T)));
+      D.pc = reinterpret_cast<T *>(carve((size_t)8 * d.N * sizeof(T)));
+      D.resid = reinterpret_cast<T *>(carve((size_t)8 * d.Ni * sizeof(T)));
+      if (COUPLED) {
+        D.div[0] = reinterpret_cast<T *>(carve((size_t)d.N * sizeof(T)));
+        D.div[1] = reinterpret_cast<T *>(carve((size_t)d.N * sizeof(T)));
+      }
+      if (d.bc_slot) D.bc.slot = reinterpret_cast<int32_t *>(carve((size_t)d.N * 4));
+      if (d.bc_pc) {
+        D.bc.code_pc = reinterpret_cast<int8_t *>(carve((size_t)d.bc_ns * 8));
+        D.bc.const_pc = reinterpret_cast<T *>(carve((size_t)d.bc_ns * 8 * sizeof(T)));
+      } else {
+        D.bc.code_pc = nullptr;
+      }
+      if (d.bc_ps) {
+        D.bc.code_ps = reinterpret_cast<int8_t *>(carve((size_t)d.bc_ns * 8));
+        D.bc.const_ps = reinterpret_cast<T *>(carve((size_t)d.bc_ns * 8 * sizeof(T)));
+      } else {
+        D.bc.code_ps = nullptr;
+      }
+      if (COUPLED && d.gb_slot) {
+        D.gbc.slot = reinterpret_cast<int32_t *>(carve((size_t)d.N * 4));
+        D.gbc.code_ps = reinterpret_cast<int8_t *>(carve((size_t)d.gb_ns * 9));
+        D.gbc.const_ps = reinterpret_cast<T *>(carve((size_t)d.gb_ns * 9 * sizeof(T)));
+      }
+      Ls[l] = D;
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < nlev; ++l) {
+    const TailLevelDev<T> &G = lv[l];
+    const TailLevelDev<T> &D = Ls[l];
+    const TailLdsDims d = tail_dims(G);
+    if (d.bc_slot) lds_copy(const_cast<int32_t *>(D.bc.slot), G.bc.slot, d.N);
+    if (d.bc_pc) {
+      lds_copy(const_cast<int8_t *>(D.bc.code_pc), G.bc.code_pc, d.bc_ns * 8);
+      lds_copy(const_cast<T *>(D.bc.const_pc), G.bc.const_pc, d.bc_ns * 8);
+    }
+    if (d.bc_ps) {
+      lds_copy(const_cast<int8_t *>(D.bc.code_ps), G.bc.code_ps, d.bc_ns * 8);
+      lds_copy(const_cast<T *>(D.bc.const_ps), G.bc.const_ps, d.bc_ns * 8);
+    }
+    if (COUPLED && d.gb_slot) {
+      lds_copy(const_cast<int32_t *>(D.gbc.slot), G.gbc.slot, d.N);
+      lds_copy(const_cast<int8_t *>(D.gbc.code_ps), G.gbc.code_ps, d.gb_ns * 9);
+      lds_copy(const_cast<T *>(D.gbc.const_ps), G.gbc.const_ps, d.gb_ns * 9);
+    }
+    if (l == 0) lds_copy(D.resid, G.resid, 8 * d.Ni);
+  }
+  __syncthreads();
+  unsigned cur = 0; // bit l: the state buffer of level l
+  tail_fill_zero<T, COUPLED>(Ls[0]);
+  __syncthreads();
+  for (int l = 0; l < nlev; ++l) {
+    const TailLevelDev<T> &L = Ls[l];
+    for (int j = 0; j < k; ++j) {
+      tail_smooth<T, COUPLED>(L, P, (cur >> l) & 1, true);
+      cur ^= 1u << l;
+    }
+    if (l + 1 < nlev) {
+      tail_residual<T, COUPLED>(L, Ls[l + 1], P, (cur >> l) & 1, true);
+      tail_fill_zero<T, COUPLED>(Ls[l + 1]);
+      cur &= ~(1u << (l + 1));
+      __syncthreads();
+    }
+  }
+  for (int l = nlev - 1; l >= 0; --l) {
+    const TailLevelDev<T> &L = Ls[l];
+    if (l + 1 < nlev) tail_prolong<T, COUPLED>(L, Ls[l + 1], P, (cur >> l) & 1);
+    for (int j = 0; j < k; ++j) {
+      tail_smooth<T, COUPLED>(L, P, (cur >> l) & 1, true);
+      cur ^= 1u << l;
+    }
+  }
+  // the result: the first level's state (cur bit 0 is back at 0 after 2 k sweeps)
+  lds_copy(lv[0].f[(cur & 1)], Ls[0].f[(cur & 1)], 8 * lv[0].nx * lv[0].ny);
+}
+
 template <typename T> TailDev<T> tail_params(const xb_tail_plan *p) {
   const double lm = p->lam + p->mu;
   double g = 1.0 / (12.0 * lm - 4.0);
@@ -4345,6 +4488,23 @@ int xb_vcycle_tail_prepare(int32_t dtype, double lam, double mu, double gamma, i
   plan->gamma = gamma;
   plan->nl_scale = nl_scale;
   plan->table = table;
+  // the LDS form when every level's arrays and rule tables fit its arena
+  size_t lds = 0;
+  const size_t ts = dtype == XB_F64 ? sizeof(double) : sizeof(float);
+  for (int l = 0; l < nlevels; ++l) {
+    const xb_tail_level &L = levels[l];
+    TailLdsDims d;
+    d.N = L.nx * L.ny;
+    d.Ni = (L.ix1 - L.ix0) * (L.iy1 - L.iy0);
+    d.bc_slot = L.bc != nullptr && L.bc->cell_slot != nullptr;
+    d.bc_ns = L.bc != nullptr ? L.bc->nslots : 0;
+    d.bc_pc = d.bc_slot && L.bc->code_pc != nullptr;
+    d.bc_ps = d.bc_slot && L.bc->code_ps != nullptr;
+    d.gb_slot = L.grad_bc != nullptr && L.grad_bc->cell_slot != nullptr;
+    d.gb_ns = L.grad_bc != nullptr ? L.grad_bc->nslots : 0;
+    lds += tail_level_lds_bytes(d, ts, coupled != 0);
+  }
+  plan->lds_bytes = lds <= (size_t)TAIL_LDS_BYTES ? (int64_t)lds : 0;
   return XB_OK;
 }
 
@@ -4354,6 +4514,26 @@ int xb_vcycle_tail(const xb_tail_plan *plan, void *stream) {
   if (int r = check_dtype(plan->dtype)) return r;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 gr(1), bl(TAIL_THREADS);
+  if (plan->lds_bytes > 0) {
+    if (plan->dtype == XB_F64) {
+      const auto *t = static_cast<const TailLevelDev<double> *>(plan->table);
+      if (plan->coupled)
+        hipLaunchKernelGGL((k_vcycle_tail_lds<double, true>), gr, bl, 0, st, t, plan->nlevels, plan->k_smooth,
+                           tail_params<double>(plan));
+      else
+        hipLaunchKernelGGL((k_vcycle_tail_lds<double, false>), gr, bl, 0, st, t, plan->nlevels, plan->k_smooth,
+                           tail_params<double>(plan));
+    } else {
+      const auto *t = static_cast<const TailLevelDev<float> *>(plan->table);
+      if (plan->coupled)
+        hipLaunchKernelGGL((k_vcycle_tail_lds<float, true>), gr, bl, 0, st, t, plan->nlevels, plan->k_smooth,
+                           tail_params<float>(plan));
+      else
+        hipLaunchKernelGGL((k_vcycle_tail_lds<float, false>), gr, bl, 0, st, t, plan->nlevels, plan->k_smooth,
+                           tail_params<float>(plan));
+    }
+    return check_launch("xb_vcycle_tail");
+  }
   if (plan->dtype == XB_F64) {
     const auto *t = static_cast<const TailLevelDev<double> *>(plan->table);
     if (plan->coupled)

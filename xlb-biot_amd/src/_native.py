@@ -75,7 +75,7 @@ class TailLevel(ctypes.Structure):
 class TailPlan(ctypes.Structure):
     _fields_ = [("dtype", c_int32), ("nlevels", c_int32), ("k_smooth", c_int32), ("coupled", c_int32),
                 ("lam", c_double), ("mu", c_double), ("gamma", c_double), ("nl_scale", c_double),
-                ("table", c_void_p)]
+                ("table", c_void_p), ("lds_bytes", ctypes.c_int64)]
 
 
 class ElastMGAdjointArgs(ctypes.Structure):
