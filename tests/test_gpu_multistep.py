@@ -130,3 +130,17 @@ def test_coupled_smoothn_with_coarse_correction_matches_prolong_then_sweeps(case
     assert torch.equal(f, f0)
     assert torch.equal(a, b), (a - b).abs().max().item()
     assert torch.equal(da, db), (da - db).abs().max().item()
+
+
+@pytest.mark.parametrize("case", ["periodic_biot", "coarse"])
+@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (3, "f32/f32")])
+def test_coupled_smoothn_from_zero_state_matches_zero_input(case, m, precision):
+    """xb_elast_smoothn with a null f_in (a coarse level's zero start state, nothing
+    read) equals the same launch from a zero-filled state, bit for bit."""
+    say("coupled smoothing on tiles from the zero state", case, m, precision)
+    el, nlf, level, f, r, base = _smooth_cases(case, precision)
+    z = torch.zeros_like(f)
+    a, da = el._smoothn_nl(z, el._divergence_soa(z), level, r, base, nlf, m)
+    b, db = el._smoothn_nl(None, None, level, r, base, nlf, m)
+    assert torch.equal(a, b), (a - b).abs().max().item()
+    assert torch.equal(da, db), (da - db).abs().max().item()

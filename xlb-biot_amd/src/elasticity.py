@@ -994,6 +994,9 @@ class VCycleTail:
         nat.check(lib.xb_vcycle_tail_prepare(nat.dtype_code(dt), el.lamda, el.mu, el.gamma, el.k_smooth, scale,
                                              int(coupled), arr, n, nat.ptr(self.table), nbytes,
                                              ctypes.byref(self.plan)), "xb_vcycle_tail_prepare")
+        if not getattr(el, "tail_in_lds", True):
+            # the global-memory form even where the levels fit the LDS arena
+            self.plan.lds_bytes = 0
         self.resid = self._keep[0]["resid"]
         self.result = self._keep[0]["f0"]
 
