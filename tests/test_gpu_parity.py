@@ -370,6 +370,24 @@ def test_darcy_run_loop_matches_oracle():
     assert rel(g.run(150), o.run(150)) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["f64/f64", "f32/f32"])
+def test_c2_full_size_matches_oracle(precision):
+    """C2 at its full size (512 x 512 cells with the ghost walls, log-normal K of
+    variance 1 and correlation 16, seed 0) through the bench's launch path: the tuned
+    multi-step tile launches replayed from captured graphs (run_steps), against 24
+    steps of the oracle."""
+    say("C2 514^2 run_steps vs oracle", precision)
+    g = P.darcy_c2("gpu", n=512, precision=precision, heterogeneous=True, seed=0, variance=1.0, corr=16.0)
+    o = P.darcy_c2("oracle", n=512, precision=precision, heterogeneous=True, seed=0, variance=1.0, corr=16.0)
+    g.autotune()
+    say("launch", g._multi()[:2])
+    fg = g.run_steps(g.assign_fields_sharded(), 24, chunk=12)
+    fo = o.assign_fields_sharded()
+    for t in range(24):
+        fo, _ = o.step(fo, t + 1)
+    assert rel(fg, fo) < TOL[precision]
+
+
 def test_nonfinite_residual_stops_the_inner_loop():
     # the V-cycle of the reference has no exact coarse solve; at 64^2 with 3 levels the
     # periodic Biot problem diverges (in the oracle too): the loop must stop and say so
