@@ -730,6 +730,62 @@ def bench_biot_adjoint(torch, n=1024, steps=50, ne=2, levels=None, K=50, precisi
             "grad_mu_norm": float(mug.grad.double().norm()), "finite": finite}
 
 
+def bench_steady_adjoint(torch, n=512, tol=1e-8, check_every=1000, checks=10):
+    """Adjoint / forward time at a relative residual of 1e-8 on C2 (Darcy 514^2, FP64,
+    log-normal K of variance 1 and correlation 16, seed 0).  Forward: steps to the
+    steady state (``run_steps_to``: the tuned tile launches, the residual tested on the
+    device).  Gradient of J = 1/2 |p - p*|^2 with respect to the relaxation rate
+    omega(x) (so to K) by the fixed-point adjoint ``a = (dS/df)^T a + dJ/df`` iterated
+    to the same relative residual (``darcy_steady_adjoint``: one adjoint step per
+    launch, device-side test), then one parameter pass; p* from the steady state with
+    K perturbed by a log-normal factor (variance 0.1, seed 2).  Both loops test every
+    ``check_every`` steps and read the device-side latch every ``checks`` tests, as
+    ``c2_time_to_solution``; each solve runs once untimed first (graph captures).
+    Both residuals are relative to the first step's: the adjoint's first step is
+    ``(dS/df)^T dJ/df``, a smooth field, so it needs more steps than the forward (whose
+    first step is the start's transient) to fall by the same factor."""
+    import problems as P
+    from src.adjoint import darcy_steady_adjoint
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    d = P.darcy_c2("gpu", n=n, precision="f64/f64", heterogeneous=True, seed=0, variance=1.0, corr=16.0)
+    d.autotune()
+    nx, ny = d.nx, d.ny
+    om = d.omega_field.clone()
+    K = (1.0 / om - 0.5) / 3.0
+    Kp = K * torch.as_tensor(P.lognormal_field(nx, ny, 0.1, 16.0, 2), dtype=K.dtype, device=K.device).reshape(nx, ny, 1)
+    f0 = d.assign_fields_sharded()
+    mx = 4000000
+    d.set_omega_field(1.0 / (3.0 * Kp + 0.5))
+    ft, kt, _ = d.run_steps_to(f0, tol, check_every, max_steps=mx, checks=checks)
+    p_star = ft.sum(-1)
+    d.set_omega_field(om)
+    d.run_steps_to(f0, tol, check_every, max_steps=check_every * checks + 1, checks=checks)
+    (fs, kf, rf), t_fwd = timed(lambda: d.run_steps_to(f0, tol, check_every, max_steps=mx, checks=checks))
+    dJdf = (fs.sum(-1) - p_star).unsqueeze(0).expand(9, nx, ny).contiguous()
+    fs_soa = fs.permute(2, 0, 1).contiguous()
+    om2 = om.reshape(nx, ny)
+    darcy_steady_adjoint(d, om2, fs_soa, dJdf, tol=tol, max_steps=check_every * checks + 1, check_every=check_every)
+    (res, t_adj) = timed(lambda: darcy_steady_adjoint(d, om2, fs_soa, dJdf, tol=tol, max_steps=mx,
+                                                      check_every=check_every))
+    grads, _, ka, ra = res
+    gK = grads["omega"].reshape(nx, ny) * (-3.0 * om.reshape(nx, ny).double() ** 2)
+    return {"grid": [nx, ny], "tol": tol, "precision": "f64/f64",
+            "forward_steps": kf, "forward_s": round(t_fwd, 4), "forward_rel_residual": rf,
+            "adjoint_steps": ka, "adjoint_s": round(t_adj, 4), "adjoint_rel_residual": ra,
+            "adjoint_over_forward": round(t_adj / t_fwd, 3),
+            "gradient_over_forward": round((t_fwd + t_adj) / t_fwd, 3),
+            "target_steps": kt, "check_every": check_every, "host_reads_every_steps": check_every * checks,
+            "grad_K_norm": float(gK.norm()),
+            "misfit": float(0.5 * ((fs.sum(-1) - p_star).double() ** 2).sum())}
+
+
 def profiled_traffic(kernel, blocks, cells):
     """L2-to-memory bytes per launch of this kernel configuration from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, calibrated on a copy), or None
@@ -806,6 +862,8 @@ def only(torch, args):
         print(json.dumps(bench_adjoint(torch)), flush=True)
     elif args.only == "biot_adjoint":
         print(json.dumps(bench_biot_adjoint(torch, n=args.n, steps=args.steps)), flush=True)
+    elif args.only == "steady_adjoint":
+        print(json.dumps(bench_steady_adjoint(torch, n=args.n)), flush=True)
     elif args.only == "large_elast":
         r = bench_large_elasticity(torch, n, args.steps)
         log(f"elasticity {n}^2: {r['steps_per_launch']} steps per launch, {r['kernel_s'] * 1e6:.1f} us/launch, "
@@ -836,7 +894,7 @@ def main():
                     help="with --tile-steps: threads per tile block, which fixes the tile shape (256, 512: "
                          "32 x 32; 704: 44 x 48; 0: the library's choice)")
     ap.add_argument("--only", choices=["large_elast", "large_darcy", "copy", "c2", "c5", "c5_kernel", "adjoint",
-                                       "biot_vcycle", "biot_vcycle_eager", "biot_adjoint", "c1"],
+                                       "biot_vcycle", "biot_vcycle_eager", "biot_adjoint", "c1", "steady_adjoint"],
                     help="run one kernel family only (for counter profiles); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -32,7 +32,7 @@ import math
 import torch
 
 from . import _native as nat
-from .base import graph_capture
+from .base import ConvergedStepRunner, graph_capture
 
 
 def _f64_zeros(shape, device):
@@ -358,29 +358,53 @@ def darcy_solve(da, omega_field, f0, n, source=None, checkpoint_every=50, use_gr
     return _DarcySolveFn.apply(omega_field, source, f0, da, int(n), int(checkpoint_every), use_graphs, reuse)
 
 
-def _fixed_point_adjoint(stepper, f_star, dJdf, tol, max_steps, check_every, want_extra):
-    """Solve ``a = (dS/df)^T a + dJ/df`` by iteration, then add ``a . dS/dtheta`` at
-    the fixed point ``f_star`` to fresh FP64 gradient buffers."""
-    b = dJdf.contiguous().to(stepper.dtype)
-    a0, a1 = b.clone(), torch.empty_like(b)
-    r0, k, rel = None, 0, float("inf")
-    while k < max_steps:
-        stepper.adjoint(a0, f_star, a1, None, b)
-        k += 1
-        if k % check_every == 0 or k == 1:
-            r = torch.linalg.vector_norm((a1 - a0).double()).item()
-            if not math.isfinite(r):
-                raise FloatingPointError(f"adjoint iteration is not finite after {k} steps")
-            r0 = r if r0 is None else r0
-            rel = r / r0 if r0 > 0 else 0.0
-            if rel <= tol:
-                a0, a1 = a1, a0
-                break
-        a0, a1 = a1, a0
+class _AdjointIteration:
+    """The iteration ``a <- (dS/df)^T a + b`` of one stepper at a fixed state, on the
+    device-tested step loop (:class:`ConvergedStepRunner`: captured blocks of
+    ``check_every`` adjoint steps, the relative step-to-step norm latched on the
+    device).  The fixed state and ``b`` live in buffers of its own, refreshed per call."""
+
+    def __init__(self, stepper, check_every, checks):
+        self.f = torch.zeros(stepper.shape, dtype=stepper.dtype, device=stepper.device)
+        self.b = torch.zeros(stepper.shape, dtype=stepper.dtype, device=stepper.device)
+        f, b = self.f, self.b
+        self.runner = ConvergedStepRunner(lambda src, dst: stepper.adjoint(src, f, dst, None, b), self.b,
+                                          check_every, checks)
+
+
+def _fixed_point_adjoint(stepper, f_star, dJdf, tol, max_steps, check_every, want_extra, checks=10):
+    """Solve ``a = (dS/df)^T a + dJ/df`` by iteration from ``a_0 = dJ/df`` until
+    ``|a_k - a_{k-1}| <= tol |a_1 - a_0|``, then add ``a . dS/dtheta`` at the fixed
+    point ``f_star`` to fresh FP64 gradient buffers.  Returns ``(grads, a, k, rel)``;
+    ``k`` is ``max_steps`` when the iteration did not get there."""
+    key = (int(check_every), int(checks))
+    cache = stepper.__dict__.setdefault("_adjoint_iterations", {})
+    it = cache.get(key)
+    if it is None:
+        it = cache[key] = _AdjointIteration(stepper, check_every, checks)
+    it.f.copy_(f_star)
+    it.b.copy_(dJdf.contiguous().to(stepper.dtype))
+    a, k, rel = it.runner.run(it.b, tol, max_steps)
+    if not math.isfinite(rel):
+        raise FloatingPointError(f"adjoint iteration is not finite after {k} steps")
+    if k < 0:
+        k = int(max_steps)
     grads = stepper.grad_buffers(want_extra)
-    scratch = torch.empty_like(b)
-    stepper.adjoint(a0, f_star, scratch, grads)
-    return grads, a0, k, rel
+    scratch = torch.empty_like(a)
+    stepper.adjoint(a, it.f, scratch, grads)
+    return grads, a, k, rel
+
+
+def _steady_stepper(solver, key, make, bind):
+    """The solver's stepper for steady adjoints of this kind, kept with its captured
+    adjoint iterations and rebound to the call's parameters."""
+    cache = solver.__dict__.setdefault("_steady_steppers", {})
+    st = cache.get(key)
+    if st is None:
+        st = cache[key] = make()
+    else:
+        bind(st)
+    return st
 
 
 def elasticity_steady_adjoint(el, lam_field, mu_field, f_star, dJdf, force=None, tol=1e-10, max_steps=200000,
@@ -389,7 +413,9 @@ def elasticity_steady_adjoint(el, lam_field, mu_field, f_star, dJdf, force=None,
     elasticity step with per-cell Lame fields: returns ``(grads, a, steps, rel)``
     with ``grads["lam"]``, ``grads["mu"]`` (and ``grads["force"]``) FP64 fields."""
     el.precisionPolicy.require_single("elasticity_steady_adjoint")
-    st = ElasticityStepper(el, lam_field, mu_field, force)
+    st = _steady_stepper(el, ("elasticity", force is not None),
+                         lambda: ElasticityStepper(el, lam_field, mu_field, force),
+                         lambda st: st.bind(lam_field, mu_field, force))
     f = _soa(f_star, 8, el.nx, el.ny, el._tdt, el.device)
     return _fixed_point_adjoint(st, f, dJdf, tol, max_steps, check_every, force is not None)
 
@@ -398,6 +424,7 @@ def darcy_steady_adjoint(da, omega_field, f_star, dJdf, source=None, tol=1e-10, 
     """As :func:`elasticity_steady_adjoint` for the Darcy step: ``grads["omega"]``
     (and ``grads["source"]``)."""
     da.precisionPolicy.require_single("darcy_steady_adjoint")
-    st = DarcyStepper(da, omega_field, source)
+    st = _steady_stepper(da, ("darcy", source is not None), lambda: DarcyStepper(da, omega_field, source),
+                         lambda st: st.bind(omega_field, source))
     f = _soa(f_star, 9, da.nx, da.ny, da._tdt, da.device)
     return _fixed_point_adjoint(st, f, dJdf, tol, max_steps, check_every, source is not None)
