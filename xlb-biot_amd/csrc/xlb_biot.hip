@@ -976,6 +976,13 @@ __device__ __forceinline__ int wrapm(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
+// Grid coordinates of a tile's cells along one axis: the tile's origin o (at least
+// -n on a grid of at least `t` cells along that axis, where every coordinate o + r,
+// 0 <= r < t, lies in [-n, 2n)) plus r, wrapped onto [0, n).  On such grids one
+// conditional add or subtract wraps; on smaller ones (coarse levels) the remainder.
+// `big` is uniform over the block, so the choice is one branch per tile.
+__device__ __forceinline__ int wrap_axis(int x, int n, bool big) { return big ? wrapi(x, n) : wrapm(x, n); }
+
 // Rules of one cell without branches (the tile kernel unrolls its cells, and the
 // branchy form of apply_rules would not fit the instruction cache): the same values
 // as apply_rules, with the rule sources read from the cell's LDS slots (source
@@ -1181,10 +1188,11 @@ __device__ __forceinline__ void tile_steps_body(int nx, int ny, int steps, int t
   T f[CPT][Q];
   Aux aux[CPT];
   int slot[CPT];
+  const bool big = nx >= TX && ny >= TY; // the halo (steps) is below the tile size
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
     const int c = threadIdx.x + k * NT;
-    const int gx = wrapm(x0 + c / TY, nx), gy = wrapm(y0 + c % TY, ny);
+    const int gx = wrap_axis(x0 + c / TY, nx, big), gy = wrap_axis(y0 + c % TY, ny, big);
     const size_t cell = (size_t)gx * ny + gy;
 #pragma unroll
     for (int d = 0; d < Q; ++d) f[k][d] = fin[d * N + cell];
@@ -1689,6 +1697,15 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
   T kd[SMOOTH_CPT];
   int slot[SMOOTH_CPT], gslot[SMOOTH_CPT], ri[SMOOTH_CPT];
   bool mine = false, gmine = false;
+  // grid coordinates of the thread's cells: one column (SMOOTH_THREADS is a multiple
+  // of TILE), one row per cell; the halo (R steps) is below the tile size
+  static_assert(SMOOTH_THREADS % TILE == 0, "a thread's cells share their column");
+  const bool big = nx >= TILE && ny >= TILE;
+  const int gy = wrap_axis(y0 + (int)threadIdx.x % TILE, ny, big);
+  int gxs[SMOOTH_CPT];
+#pragma unroll
+  for (int k = 0; k < SMOOTH_CPT; ++k)
+    gxs[k] = wrap_axis(x0 + ((int)threadIdx.x + k * SMOOTH_THREADS) / TILE, nx, big);
   if (A.fin == nullptr) {
     // a coarse level's V-cycle starts from the zero state: nothing to load
 #pragma unroll
@@ -1701,8 +1718,7 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
     // branch per cell, so the correction's loads go out with the populations'
 #pragma unroll
     for (int k = 0; k < SMOOTH_CPT; ++k) {
-      const int c = threadIdx.x + k * SMOOTH_THREADS;
-      const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
+      const int gx = gxs[k];
       const size_t cell = (size_t)gx * ny + gy;
       const bool in = gx >= A.ix0 && gx < A.ix1 && gy >= A.iy0 && gy < A.iy1;
       const size_t cc = in ? (size_t)(A.ix0 + ((gx - A.ix0) >> 1)) * A.cny + (A.iy0 + ((gy - A.iy0) >> 1)) : 0;
@@ -1716,16 +1732,14 @@ __device__ __forceinline__ void tile_smooth_body(const TileSmoothDev<T> &A, int 
   } else {
 #pragma unroll
     for (int k = 0; k < SMOOTH_CPT; ++k) {
-      const int c = threadIdx.x + k * SMOOTH_THREADS;
-      const size_t cell = (size_t)wrapm(x0 + c / TILE, nx) * ny + wrapm(y0 + c % TILE, ny);
+      const size_t cell = (size_t)gxs[k] * ny + gy;
 #pragma unroll
       for (int d = 0; d < Q; ++d) f[k][d] = A.fin[d * N + cell];
     }
   }
 #pragma unroll
   for (int k = 0; k < SMOOTH_CPT; ++k) {
-    const int c = threadIdx.x + k * SMOOTH_THREADS;
-    const int gx = wrapm(x0 + c / TILE, nx), gy = wrapm(y0 + c % TILE, ny);
+    const int gx = gxs[k];
     const size_t cell = (size_t)gx * ny + gy;
     aux[k] = A.coll.load_aux(cell);
     kd[k] = A.coll.kdiv(aux[k], A.nl.kdiv);
