@@ -820,6 +820,101 @@ def c2_cpu_record(n, precision, seconds, threads=None):
                        f"(torch CPU, {threads} threads), {el:.1f} s")
 
 
+def c1_cpu_record(threads=None, tol=1e-10, check_every=10, max_steps=20000):
+    """C1 on the CPU restatement, the whole solve: plain steps of the periodic 64^2
+    elasticity from the zero state until |f_k - f_(k-1)| < tol |f_1 - f_0|, tested
+    every ``check_every`` steps (k = 1 + j check_every), the rule of the GPU run."""
+    import torch
+    import problems as P
+    threads = threads or os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    o = P.periodic_elasticity("oracle", nx=64)
+    f0 = o.assign_fields_sharded()
+    o.step(f0, 0)  # warm-up (the reference starts its clock after step 1)
+    t0 = time.perf_counter()
+    f, _ = o.step(f0, 0)
+    r1 = float(torch.linalg.norm((f - f0).reshape(-1)))
+    k, ratio = 1, 1.0
+    while k < max_steps:
+        for _ in range(check_every - 1):
+            f, _ = o.step(f, k)
+            k += 1
+        prev = f
+        f, _ = o.step(f, k)
+        k += 1
+        ratio = float(torch.linalg.norm((f - prev).reshape(-1))) / r1
+        if ratio < tol:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=64 * 64 * k / el / 1e6, unit="MLUPS", cores=threads, kind="port", steps=k,
+                seconds=round(el, 3), rel_residual=ratio,
+                sample=f"the whole C1 solve ({k} steps of the 64x64 FP64 workload to {tol:g}) on the CPU restatement "
+                       f"(torch CPU, {threads} threads), {el:.2f} s")
+
+
+def _oracle_step_costs(o, state, t, cycles=(1, 1, 3)):
+    """Seconds of one physical step of an oracle Biot problem with 1 and with 3 inner
+    V-cycles (the first 1-cycle step warms the oracle up): (glue, per V-cycle, spent)."""
+    ts, spent = {}, 0.0
+    o.elasticityTol = None
+    for cyc in cycles:
+        o.elasticitySteps = cyc
+        t0 = time.perf_counter()
+        o.step(*state, t)
+        ts[cyc] = time.perf_counter() - t0
+        spent += ts[cyc]
+    per_cycle = max((ts[3] - ts[1]) / 2, 0.0)
+    return max(ts[1] - per_cycle, 0.0), per_cycle, spent
+
+
+def c3_cpu_record(gpu, threads=None, n=256, levels=8):
+    """The CPU restatement on the periodic Biot workload the GPU runs with the reference
+    driver's material (``gpu``: that run's record): one physical step with 1 and with 3
+    V-cycles gives the step's glue and the cost per V-cycle; the GPU run's physical
+    steps and V-cycles are priced at those costs."""
+    import torch
+    import problems as P
+    threads = threads or os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    lam, mu = 0.11 * 0.8 / (1 - 0.8 ** 2), 0.5 * 0.11 / 1.8
+    o = P.biot_periodic("oracle", nx=n, levels=levels, steps=1, lam=lam, mu=mu, K=0.1)
+    glue, per_cycle, spent = _oracle_step_costs(o, o.initial_state(), 1)
+    steps, cycles = gpu["physical_steps"], gpu["vcycles"]
+    total = steps * glue + cycles * per_cycle
+    return dict(value=n * n * steps / total / 1e6, unit="MLUPS", cores=threads, kind="port",
+                seconds_priced=round(total, 2), per_vcycle_s=round(per_cycle, 4), step_glue_s=round(glue, 4),
+                sample=(f"oracle (torch CPU, {threads} threads, float64): one physical step of the {n}x{n} Biot "
+                        f"workload ({levels} levels) with 1 and with 3 V-cycles ({spent:.1f} s); the GPU run's "
+                        f"{steps} physical steps and {cycles} V-cycles priced at those costs: {total:.1f} CPU s"))
+
+
+def c4_cpu_record(gpu, threads=None, n=1024, k_smooth=8):
+    """The CPU restatement on the C4 forward solve (2-D consolidation, 1024 x 1026,
+    FP32, log-normal K, lambda and mu as fields): one physical step with 1 and with 3
+    V-cycles; the GPU run's forward (its physical steps, ``vcycles_per_step`` each) is
+    priced at those costs.  The gradient is not timed on the CPU."""
+    import torch
+    import problems as P
+    threads = threads or os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    levels, steps, ne = gpu["levels"], gpu["physical_steps"], gpu["vcycles_per_step"]
+    nx, ny = n, n + 2
+    K = 0.5 * torch.as_tensor(P.lognormal_field(nx, ny, 2.0, 32.0, 1), dtype=torch.float64)
+    o = P.biot_consolidation("oracle", nx=n, levels=levels, steps=ne, k_smooth=k_smooth, precision="f32/f32",
+                             lambda_field=torch.full((nx, ny), 2.0, dtype=torch.float32),
+                             mu_field=torch.full((nx, ny), 1.0, dtype=torch.float32),
+                             omega_field=(1.0 / (3.0 * K + 0.5)).to(torch.float32)[..., None])
+    glue, per_cycle, spent = _oracle_step_costs(o, o.initial_state(), 0)
+    total = steps * glue + steps * ne * per_cycle
+    return dict(value=nx * ny * steps / total / 1e6, unit="MLUPS", cores=threads, kind="port",
+                seconds_priced=round(total, 2), per_vcycle_s=round(per_cycle, 3), step_glue_s=round(glue, 3),
+                gpu_forward_s=gpu.get("forward_s"),
+                sample=(f"oracle (torch CPU, {threads} threads, float32 state): one physical step of the C4 "
+                        f"consolidation ({nx}x{ny}, {levels} levels, k = {k_smooth}) with 1 and with 3 V-cycles "
+                        f"({spent:.1f} s); the GPU forward's {steps} steps x {ne} V-cycles priced at those costs: "
+                        f"{total:.1f} CPU s"))
+
+
 def only(torch, args):
     """Counter-profile entry: one kernel family, ``--steps`` launches."""
     n = 4096
@@ -1000,6 +1095,15 @@ def main():
             # one thread: the CPU restatement without the host's parallelism
             extra("c2_cpu_oracle_1_thread", lambda: c2_cpu_record(args.n, "f64/f64", args.cpu_seconds / 2, threads=1))
             extra("c5_cpu_oracle_1_thread", lambda: c5_cpu_baseline(torch, r, seconds_cap=40.0, threads=1))
+            log("cpu: the oracle on C1 (the whole solve), C3 and the C4 forward (priced samples)")
+            extra("c1_cpu_oracle", lambda: c1_cpu_record())
+            extra("c1_cpu_oracle_1_thread", lambda: c1_cpu_record(threads=1))
+            c3 = extras.get("c3_biot_256_mg8_ref_material_fp64", {})
+            if "vcycles" in c3 and "diverged" not in c3:
+                extra("c3_cpu_oracle", lambda: c3_cpu_record(c3))
+            c4 = extras.get("c4_biot_adjoint_1024_fp32", {})
+            if "levels" in c4:
+                extra("c4_cpu_oracle_forward", lambda: c4_cpu_record(c4))
             torch.set_num_threads(os.cpu_count() or 1)
     line["extra"] = extras
     print(json.dumps(line), flush=True)
