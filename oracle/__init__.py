@@ -39,8 +39,12 @@ Parity pins (what checks each part; ``tests/test_oracle.py``,
   converges at O(dx^2), Terzaghi's consolidation (``Terzaghi.py``) at O(dx); a
   wrong sign planted in the lattice gradient breaks the former.
 
-The JAX out-of-bounds semantics used when fine-level Darcy boundary indices are
-applied to coarse multigrid arrays (clamp on gather, drop on scatter) are taken
-from JAX's documented ``.at[]`` behaviour; no reference run pins them
-("parity unpinned" for that combination: multigrid + Biot + Darcy walls).
+Fine-level Darcy wall indices applied to coarse multigrid arrays (multigrid + Biot +
+Darcy walls) follow JAX's out-of-bounds semantics: clamp on gather, drop on scatter.
+The clamp is pinned as irrelevant (tests/test_oracle_oob.py: the ADR wall rules are
+row-local, so the gradient is bit-identical when the out-of-range rows gather NaN,
+wrapped or zero indices instead, and cells named by no in-range row keep the streamed
+stencil).  The drop is JAX's documented default for ``.at[].set`` ("The Sharp Bits",
+out-of-bounds indexing); JAX is absent here, so no reference run checks that one
+convention.
 """
