@@ -40,11 +40,13 @@ def _rel(a, b):
     return abs(a - b) / max(abs(a), abs(b), 1e-300)
 
 
-@pytest.mark.parametrize("levels", [2, 3])
-def test_vcycle_adjoint_is_the_transpose(levels):
+@pytest.mark.parametrize("nx,levels,k", [(16, 2, 2), (16, 3, 2), (1024, 10, 8)])
+def test_vcycle_adjoint_is_the_transpose(nx, levels, k):
+    """At 16 x 18 with the Lame-parameter derivatives against central differences; at
+    the C4 size (1024 x 1026, 10 levels, k = 8) the exact transposes only."""
     from src.biot_adjoint import BiotAdjointSession
-    say("transposed V-cycle, consolidation 16 x 18, levels", levels)
-    p = _problem(16, levels, 2, 3)
+    say("transposed V-cycle, consolidation", nx, "x", nx + 2, "levels", levels, "k", k)
+    p = _problem(nx, levels, k, 3)
     s = BiotAdjointSession(p, 1, 4)
     el = p.elasticity
     nx, ny = el.nx, el.ny
@@ -72,6 +74,8 @@ def test_vcycle_adjoint_is_the_transpose(levels):
     rhs = (dF * acc["base"]).sum().item()
     say("static force", lhs, rhs)
     assert _rel(lhs, rhs) < 1e-10
+    if nx > 64:
+        return
     lam0, mu0 = el._lam_field.clone().reshape(nx, ny), el._mu_field.clone().reshape(nx, ny)
     h = 1e-6
     for name in ("lam", "mu"):

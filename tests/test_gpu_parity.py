@@ -388,6 +388,27 @@ def test_c2_full_size_matches_oracle(precision):
     assert rel(fg, fo) < TOL[precision]
 
 
+def test_c3_full_size_matches_oracle():
+    """C3 at its size and depth: periodic Biot 256 x 256, elasticity V-cycles of 4
+    levels with k = 2 and lambda = mu = 1 (as specified), FP64; three physical steps of
+    two V-cycles each against the oracle, all seven outputs to 1e-12."""
+    say("C3 256^2, 4 levels, k = 2, lambda = mu = 1")
+    g = P.biot_periodic("gpu", nx=256, levels=4, steps=2, k_smooth=2)
+    o = P.biot_periodic("oracle", nx=256, levels=4, steps=2, k_smooth=2)
+    sg = g.initial_state()
+    so = o.initial_state()
+    fe, fd, dde = (t.permute(1, 2, 0) for t in sg)
+    oe, od, odde = so
+    for t in range(3):
+        outg = g.step(fe, fd, dde, t)
+        outo = o.step(oe, od, odde, t)
+        errs = P.step_rel(outg, outo)
+        say("  step", t, "relative errors", [f"{e:.1e}" for e in errs])
+        assert max(errs) < 1e-12, (t, errs)
+        fe, fd, dde = outg[0], outg[1], outg[4]
+        oe, od, odde = outo[0], outo[1], outo[4]
+
+
 def test_nonfinite_residual_stops_the_inner_loop():
     # the V-cycle of the reference has no exact coarse solve; at 64^2 with 3 levels the
     # periodic Biot problem diverges (in the oracle too): the loop must stop and say so
