@@ -409,6 +409,34 @@ def test_c3_full_size_matches_oracle():
         oe, od, odde = outo[0], outo[1], outo[4]
 
 
+def test_c4_full_size_forward_matches_oracle_fp32():
+    """C4's forward at its size and depth: the 2-D consolidation 1024 x 1026, 10 levels,
+    k = 8, FP32 state, lambda = 2 and mu = 1 held as fields, the Darcy rate from the
+    seeded log-normal K (variance 2, correlation 32); one physical step of two V-cycles
+    against the FP32 oracle, all seven outputs to 1e-5."""
+    import numpy as np
+    say("C4 1024 x 1026, 10 levels, k = 8, FP32")
+    n = 1024
+    nx, ny = n, n + 2
+    K = 0.5 * P.lognormal_field(nx, ny, 2.0, 32.0, 1)
+    om = 1.0 / (3.0 * K + 0.5)
+    lam, mu = np.full((nx, ny), 2.0), np.full((nx, ny), 1.0)
+    g = P.biot_consolidation("gpu", nx=n, levels=10, steps=2, k_smooth=8, precision="f32/f32")
+    g.elasticity.set_lame_fields(lam, mu)
+    g.darcy.set_omega_field(om)
+    f32 = lambda a: torch.as_tensor(a, dtype=torch.float32)
+    o = P.biot_consolidation("oracle", nx=n, levels=10, steps=2, k_smooth=8, precision="f32/f32",
+                             lambda_field=f32(lam), mu_field=f32(mu), omega_field=f32(om)[..., None])
+    sg = g.initial_state()
+    so = o.initial_state()
+    fe, fd, dde = (t.permute(1, 2, 0) for t in sg)
+    outg = g.step(fe, fd, dde, 0)
+    outo = o.step(*so, 0)
+    errs = P.step_rel(outg, outo, against_state=(4,))
+    say("  relative errors", [f"{e:.1e}" for e in errs])
+    assert max(errs) < 1e-5, errs
+
+
 def test_nonfinite_residual_stops_the_inner_loop():
     # the V-cycle of the reference has no exact coarse solve; at 64^2 with 3 levels the
     # periodic Biot problem diverges (in the oracle too): the loop must stop and say so
