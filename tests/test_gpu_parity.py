@@ -437,6 +437,22 @@ def test_c4_full_size_forward_matches_oracle_fp32():
     assert max(errs) < 1e-5, errs
 
 
+def test_biot_vcycle_4096_matches_oracle():
+    """The HBM-resident Biot workload at its size: periodic Biot 4096 x 4096 (manufactured
+    forcing, lambda = mu = 1, K = 1), elasticity V-cycle of 6 levels with k = 2 and the
+    fused coupling force, FP64; one physical step (one V-cycle and the Darcy step)
+    against the oracle, all seven outputs to 1e-12."""
+    say("Biot 4096^2, 6 levels, k = 2, one physical step")
+    g = P.biot_periodic("gpu", nx=4096, levels=6, steps=1)
+    o = P.biot_periodic("oracle", nx=4096, levels=6, steps=1)
+    fe, fd, dde = (t.permute(1, 2, 0) for t in g.initial_state())
+    outg = g.step(fe, fd, dde, 1)
+    outo = o.step(*o.initial_state(), 1)
+    errs = P.step_rel(outg, outo)
+    say("  relative errors", [f"{e:.1e}" for e in errs])
+    assert max(errs) < 1e-12, errs
+
+
 def test_nonfinite_residual_stops_the_inner_loop():
     # the V-cycle of the reference has no exact coarse solve; at 64^2 with 3 levels the
     # periodic Biot problem diverges (in the oracle too): the loop must stop and say so
