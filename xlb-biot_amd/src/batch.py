@@ -206,9 +206,11 @@ class ElasticityBatch:
         # 64-instance set: 212 us per launch against 236 us with every instance active,
         # 82 us against 45 us with an eighth of them); a solve replays the cycle of one
         # workgroup per block until the active instances hold less than
-        # ``list_switch`` of the finest-level cells, then the listed cycle.
+        # ``list_switch`` of the finest-level cells, then the listed cycle (0.9: on the
+        # 64-instance set, 4 physical steps took 0.806-0.816 s against 0.835-0.844 s
+        # with 0.7, 0.819-0.828 s with the listed cycle throughout, at 2-8 cycles per poll).
         self.work_lists = bool(work_lists)
-        self.list_switch = 0.7
+        self.list_switch = 0.9
         self._plans_dev = None
         if self.work_lists:
             for sched in self.schedules:
