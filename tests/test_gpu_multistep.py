@@ -96,7 +96,8 @@ def _smooth_cases(case, precision):
 
 
 @pytest.mark.parametrize("case", ["periodic_biot", "consolidation", "fields", "coarse"])
-@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (3, "f64/f64"), (2, "f32/f32")])
+@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (3, "f64/f64"), (4, "f64/f64"), (6, "f64/f64"),
+                                         (2, "f32/f32"), (4, "f32/f32")])
 def test_coupled_smoothn_matches_sweeps(case, m, precision):
     say("coupled smoothing on tiles", case, m, precision)
     el, nlf, level, f, r, base = _smooth_cases(case, precision)
@@ -110,7 +111,7 @@ def test_coupled_smoothn_matches_sweeps(case, m, precision):
 
 
 @pytest.mark.parametrize("case", ["periodic_biot", "consolidation", "fields", "coarse"])
-@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (3, "f64/f64"), (2, "f32/f32")])
+@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (3, "f64/f64"), (4, "f64/f64"), (2, "f32/f32")])
 def test_coupled_smoothn_with_coarse_correction_matches_prolong_then_sweeps(case, m, precision):
     """The post-smoothing launch with the prolongation fused in (coarse_out read by
     xb_elast_smoothn) against xb_elast_prolong_add_div followed by single sweeps;
@@ -133,7 +134,7 @@ def test_coupled_smoothn_with_coarse_correction_matches_prolong_then_sweeps(case
 
 
 @pytest.mark.parametrize("case", ["periodic_biot", "coarse"])
-@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (3, "f32/f32")])
+@pytest.mark.parametrize("m,precision", [(2, "f64/f64"), (4, "f64/f64"), (3, "f32/f32")])
 def test_coupled_smoothn_from_zero_state_matches_zero_input(case, m, precision):
     """xb_elast_smoothn with a null f_in (a coarse level's zero start state, nothing
     read) equals the same launch from a zero-filled state, bit for bit."""
