@@ -277,10 +277,12 @@ def test_tiled_transposed_smoothing_matches_sweep_pairs(precision):
     lam, mu, om = _fields(nx, ny, 7)
     te, tp = _misfit_targets(nx, ny, 8)
     grads, Js = {}, {}
-    for m in (0, 2, 3):
+    for m in (0, 2, 3, None):
+        # None: the default, 2 sweeps per launch on levels 0 and 1 and 3 on level 2
         say("transposed smoothing sweeps per launch", m, precision)
         p = P.biot_consolidation("gpu", nx=nx, levels=levels, steps=n_inner, k_smooth=k, precision=precision)
-        p.adjoint_tile_steps = m
+        if m is not None:
+            p.adjoint_tile_steps = m
         ts = [torch.tensor(v, dtype=dt, device="cuda", requires_grad=True) for v in (om, lam, mu)]
         eta, pr = biot_solve(p, *ts, steps, checkpoint_every=2)
         J = 0.5 * ((eta - te.to(eta)) ** 2).sum(dtype=torch.float64) + \
@@ -289,7 +291,7 @@ def test_tiled_transposed_smoothing_matches_sweep_pairs(precision):
         Js[m] = J.item()
         grads[m] = [t.grad.clone() for t in ts]
         say("J", Js[m], "grad norms", [g.double().norm().item() for g in grads[m]])
-    for m in (2, 3):
+    for m in (2, 3, None):
         assert Js[m] == Js[0]
         for name, a, b in zip(("omega", "lambda", "mu"), grads[0], grads[m]):
             assert torch.equal(a, b), (name, m, (a - b).abs().max().item())
