@@ -976,11 +976,11 @@ __device__ __forceinline__ int wrapm(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
-// Grid coordinates of a tile's cells along one axis: the tile's origin o (at least
-// -n on a grid of at least `t` cells along that axis, where every coordinate o + r,
-// 0 <= r < t, lies in [-n, 2n)) plus r, wrapped onto [0, n).  On such grids one
-// conditional add or subtract wraps; on smaller ones (coarse levels) the remainder.
-// `big` is uniform over the block, so the choice is one branch per tile.
+// A tile cell's coordinate x along one axis (the tile's origin plus the cell's offset)
+// wrapped onto [0, n).  On a grid at least a tile wide along that axis (`big`) every
+// such x lies in [-n, 2n), because the tile's halo is narrower than the tile, and one
+// conditional add or subtract wraps it; on smaller grids (coarse levels) the remainder
+// does.  `big` is uniform over the block, so the choice is one branch per tile.
 __device__ __forceinline__ int wrap_axis(int x, int n, bool big) { return big ? wrapi(x, n) : wrapm(x, n); }
 
 // Rules of one cell without branches (the tile kernel unrolls its cells, and the

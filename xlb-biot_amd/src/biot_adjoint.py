@@ -91,10 +91,9 @@ class BiotAdjointSession:
         small = getattr(p, "adjoint_small_cells", 0)
         self.adjoint_small_cells = None if small is None else int(small)
         # transposed smoothing sweeps per launch on LDS tiles (0: one launch pair per
-        # sweep, xb_elast_mg_adjoint + xb_elast_coupling_adjoint)
-        # transposed smoothing sweeps per tile launch: the problem's adjoint_tile_steps on
-        # every level when set, else 2 on levels 0 and 1 and 3 below them (the C4
-        # backward: 0.348 against 0.355 s with 2 everywhere; the same gradients)
+        # sweep, xb_elast_mg_adjoint + xb_elast_coupling_adjoint): the problem's
+        # adjoint_tile_steps on every level when set, else 2 on levels 0 and 1 and 3 below
+        # them (the C4 backward: 0.348 against 0.355 s with 2 everywhere; the same gradients)
         self.adjoint_tile_steps = getattr(p, "adjoint_tile_steps", None)
         self._coarse_adjoint_graph()
 
